@@ -1,0 +1,221 @@
+"""Benchmark of the SDF ray-march hot path on one MI355X.
+
+Default workload (the metric's 1080p configuration): 4096 primitives
+(sphere/box/capsule, size U[0.1,4] m full extent, random rotation, smooth-min
+k=0.1 in groups of 8) in a 256 m cube, voxelised at 0.02 m into a resident
+two-level sparse scene; 1920x1080 primary rays plus one soft-shadow ray per hit
+toward normalize(0.5, 1, 0.3); 128 near steps, hit eps 1e-3 t, far 100 m.
+
+A step is one launch of the march over the frame (primary + shadow rays),
+inputs resident in HBM. Timed with HIP events on the launch stream. The
+algorithmic bytes of a launch come from the evaluation counts the kernel
+reports in its debug plane (equal to the oracle's, checked by the parity
+tests): 36 B per SDF evaluation (8 corner words of 4 B + one 4 B index
+entry), 6 * 36 + 20 B per hit (normal, material/blend voxel, map), 44 B
+G-buffer write per pixel, 4 B per shadow-ray result.
+
+--workload config0 runs the drop-in kernel on the reference layouts instead
+(64 voxelised spheres, 640x360).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def build_config1(ctx, M, L, scenes, width, height):
+    prims = scenes.primitive_field(seed=1, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8)
+    lo, hi = scenes.prim_bounds(prims, 1.0)
+    t0 = time.time()
+    sc = M.SparseScene.build(ctx, prims, 0.02, lo, hi, smin_k=0.1, mat_seed=scenes.material_seed(1))
+    ctx.sync()
+    log(f"config1 scene: {int(sc.info['n_bricks'])} bricks, {int(sc.info['n_blocks'])} L1 blocks, "
+        f"cells {list(sc.info['cells'])}, {sc.info['device_bytes'] / 2**30:.2f} GiB, built in {time.time() - t0:.1f} s")
+    rng = scenes.Pcg32(1001)
+    cam_pos = rng.unit_vector() * 64.0
+    cam = M.camera_look_at(cam_pos, (0.0, 0.0, 0.0), 60.0, width, height)
+    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
+    light = np.array([0.5, 1.0, 0.3])
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    return sc, cam, su, ext
+
+
+def algorithmic_bytes(dbg, hit):
+    evals = dbg["evals"].astype(np.int64)
+    sh = dbg["shadow_evals"].astype(np.int64)
+    b = 36 * evals.sum() + 44 * evals.size + int(hit.sum()) * (6 * 36 + 20)
+    b += 36 * sh.sum() + 4 * int(hit.sum())
+    return int(b)
+
+
+def cpu_baseline_sparse(sc, cam, su, ext, width, height, seconds):
+    from oracle import oracle_py as O
+    l2, l1, dist, mat = sc.download()
+    si = O.SparseInputs(sc.info["origin"], float(sc.info["voxel_size"]), sc.info["cells"], l2, l1, dist, mat)
+    # calibrate on 4 rows around the centre, then size a band for ~`seconds`
+    mid = height // 2
+    t0 = time.time()
+    res, used = O.sparse_trace_parallel(si, cam, su, ext, width, height, mid * width, (mid + 4) * width)
+    dt = max(time.time() - t0, 1e-3)
+    rows = int(max(4, min(height, 4 * seconds / dt)))
+    r0 = max(0, mid - rows // 2)
+    t0 = time.time()
+    res, used = O.sparse_trace_parallel(si, cam, su, ext, width, height, r0 * width, (r0 + rows) * width)
+    dt = time.time() - t0
+    rays = res.size + int(res["hit"].sum())
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=int(used), kind="port",
+                sample=f"FP32 oracle twin (C, OpenMP dynamic/64) on rows {r0}..{r0 + rows - 1} of the "
+                       f"1920x1080 frame: {res.size} primary + {int(res['hit'].sum())} shadow rays in {dt:.1f} s; "
+                       f"host {cpu_model()}")
+
+
+def cpu_baseline_ref(sc, ext, idx, counts, cells, u, seconds):
+    from oracle import oracle_py as O
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=u)
+    t0 = time.time()
+    res, used = O.ref_trace_parallel(inp)
+    dt = time.time() - t0
+    return dict(value=res.size / dt / 1e6, unit="Mrays/s", cores=int(used), kind="port",
+                sample=f"FP32 oracle twin (C, OpenMP) on the full 640x360 frame in {dt:.2f} s; host {cpu_model()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="config1", choices=["config1", "config0"])
+    ap.add_argument("--variant", type=int, default=0, help="sparse kernel: 0 tuned, 1 plain")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the path shards rays across GPUs with no exchange, "
+              "and runs on more than one GPU are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from rkifield_amd import layouts as L
+    from rkifield_amd import march as M
+    from rkifield_amd import scenes
+
+    ctx = M.Context(0)
+    if args.workload == "config1":
+        width, height = 1920, 1080
+        sc, cam, su, ext = build_config1(ctx, M, L, scenes, width, height)
+        launch = lambda gb: sc.march(cam, su, ext, gb, variant=args.variant)
+        workload = ("config1: 4096 sphere/box/capsule prims, size U[0.1,4] m, smin k=0.1 groups of 8, "
+                    "256 m cube, vs 0.02 m two-level sparse index; 1920x1080 primary + 1 shadow ray per hit")
+        kernel = "sparse_march_tuned" if args.variant == 0 else "sparse_march_plain"
+    else:
+        sc = scenes.spheres_scene(seed=0)
+        width, height = sc.width, sc.height
+        gs = M.GpuScene(ctx)
+        gs.upload_brick_pool(sc.pool)
+        gs.upload_brick_maps(sc.maps)
+        gs.upload_objects(sc.objects)
+        gs.update_camera(sc.cam)
+        gs.update_scene(sc.su, sc.ext)
+        tc = M.TileObjectCullPass(ctx, width, height)
+        tc.dispatch()
+        cf = M.CoarseField.from_scene_aabbs(ctx, sc.aabbs())
+        cf.populate_and_upload()
+        rp = M.RayMarchPass(ctx)
+        launch = lambda gb: rp.dispatch(gs, gb, tc, cf)
+        workload = "config0: 64 voxelised spheres (seed 0), vs 0.05 m, reference layouts, 640x360 primary rays"
+        kernel = "ray_march_ref_kernel"
+
+    # counting launch (debug plane on): evaluation counts -> algorithmic bytes
+    gbd = M.GBuffer(ctx, width, height, debug=True, shadow=args.workload == "config1")
+    launch(gbd)
+    out = gbd.read()
+    hit = out["position"][:, 3] != L.MAX_FLOAT
+    nbytes = algorithmic_bytes(out["debug"], hit)
+    n_primary = width * height
+    n_shadow = int(hit.sum()) if args.workload == "config1" else 0
+    log(f"hits {int(hit.sum())} / {n_primary}; evals {int(out['debug']['evals'].sum())}, "
+        f"shadow evals {int(out['debug']['shadow_evals'].sum())}; algorithmic bytes {nbytes / 1e9:.3f} GB")
+    del gbd
+
+    gb = M.GBuffer(ctx, width, height, shadow=args.workload == "config1")
+    for _ in range(args.warmup):
+        launch(gb)
+    ctx.sync()
+    ctx.timer_begin()
+    for _ in range(args.steps):
+        launch(gb)
+    total_ms = ctx.timer_end()
+    ctx.sync()
+    ms = total_ms / args.steps
+    rays = n_primary + n_shadow
+    mrays = rays / (ms / 1e3) / 1e6
+    achieved = nbytes / (ms / 1e3) / 1e9
+    log(f"{kernel}: {ms:.3f} ms/launch, {mrays:.1f} Mrays/s, {achieved:.1f} GB/s algorithmic")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        if args.workload == "config1":
+            cpu = cpu_baseline_sparse(sc, cam, su, ext, width, height, args.cpu_seconds)
+        else:
+            idx, counts = tc.read()
+            cells, u = cf.read()
+            cpu = cpu_baseline_ref(sc, sc.ext, idx, counts, cells, u, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "Mrays/s + % HBM roofline at 1080p; p50/p99 latency served vs plain",
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded splitmix64/PCG32 scene and camera)",
+        "config": {"workload": workload, "kernel": kernel, "width": width, "height": height,
+                   "primary_rays": n_primary, "shadow_rays": n_shadow},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

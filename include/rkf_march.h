@@ -1,0 +1,408 @@
+/*
+ * rkf_march.h — C ABI of the MI355X (gfx950) SDF ray-march library.
+ *
+ * Drop-in replacement for rkf-render's ray-march pass
+ * (crates/rkf-render/src/ray_march.rs:29-118, kernel
+ * crates/rkf-render/shaders/ray_march.wgsl). The byte layouts of every input
+ * record are the reference's own (static-asserted below and in the tests):
+ *
+ *   rkf_voxel_sample       crates/rkf-core/src/voxel.rs:15-20          8 B
+ *   rkf_gpu_object         crates/rkf-render/src/gpu_object.rs:41-95   256 B
+ *   rkf_camera_uniforms    crates/rkf-render/src/camera.rs:194-213     144 B
+ *   rkf_scene_uniforms     crates/rkf-render/src/gpu_scene.rs:21-43    16 B
+ *   rkf_coarse_field_uniforms crates/rkf-render/src/coarse_field.rs:31-42 48 B
+ *   rkf_bvh_node           crates/rkf-render/src/gpu_bvh.rs:24-41      32 B
+ *
+ * Conventions
+ *   - Every function returns an int status: RKFM_OK (0) or a negative error
+ *     code; rkfm_last_error_string() gives the message of the last failure
+ *     on the calling thread.
+ *   - A context (rkfm_ctx) owns its device memory and is thread-affine: use
+ *     one host thread per context.
+ *   - Host pointers passed to upload/set functions are copied before return.
+ *   - G-buffer and query output pointers are device pointers owned by the
+ *     caller; launches are asynchronous on the given stream (NULL = the
+ *     context's own stream).
+ */
+#ifndef RKF_MARCH_H
+#define RKF_MARCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define RKFM_OK 0
+#define RKFM_ERR_INVALID_ARG (-1)  /* null pointer, zero size, bad enum      */
+#define RKFM_ERR_OUT_OF_RANGE (-2) /* slot/offset/index beyond an upload     */
+#define RKFM_ERR_OOM (-3)          /* device allocation failed / over budget */
+#define RKFM_ERR_DEVICE (-4)       /* HIP runtime error                      */
+#define RKFM_ERR_LAYOUT (-5)       /* inconsistent layout (e.g. map dims)    */
+#define RKFM_ERR_NOT_READY (-6)    /* required input never uploaded          */
+#define RKFM_ERR_BUSY (-7)         /* request queue full                     */
+
+/* ------------------------------------------------------------------------ */
+/* Reference constants                                                       */
+/* ------------------------------------------------------------------------ */
+#define RKF_EMPTY_SLOT 0xFFFFFFFFu    /* brick_map.rs:31                      */
+#define RKF_INTERIOR_SLOT 0xFFFFFFFEu /* brick_map.rs:37                      */
+#define RKF_BVH_INVALID 0xFFFFFFFFu   /* ray_march.wgsl:27                    */
+#define RKF_BRICK_DIM 8u              /* constants.rs:10                      */
+#define RKF_TILE_SIZE 16u             /* ray_march.wgsl:47                    */
+#define RKF_TILE_MAX_OBJECTS 32u      /* ray_march.wgsl:48                    */
+#define RKF_SDF_TYPE_NONE 0u
+#define RKF_SDF_TYPE_ANALYTICAL 1u
+#define RKF_SDF_TYPE_VOXELIZED 2u
+
+/* ------------------------------------------------------------------------ */
+/* Reference layouts                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* voxel.rs:15-20. word0 = f16 distance (bits 0-15) | material (16-21) |
+ * secondary material (22-27); word1 byte 3 = blend weight. */
+typedef struct rkf_voxel_sample {
+    uint32_t word0;
+    uint32_t word1;
+} rkf_voxel_sample;
+
+/* gpu_object.rs:41-95 (WGSL view ray_march.wgsl:60-97). inverse_world is
+ * column-major: inverse_world[c*4 + r]. */
+typedef struct rkf_gpu_object {
+    float inverse_world[16];
+    float aabb_min[4];
+    float aabb_max[4];
+    uint32_t brick_map_offset;
+    uint32_t brick_map_dims[3];
+    float voxel_size;
+    uint32_t material_id;
+    uint32_t sdf_type;
+    uint32_t blend_mode;
+    float blend_radius;
+    float sdf_params[4];
+    float accumulated_scale[3];
+    uint32_t lod_level;
+    uint32_t object_id;
+    uint32_t primitive_type;
+    float geometry_aabb_min[3];
+    float geometry_aabb_max[3];
+    float _padding[15];
+} rkf_gpu_object;
+
+/* camera.rs:194-213. prev_vp column-major. */
+typedef struct rkf_camera_uniforms {
+    float position[4];
+    float forward[4];
+    float right[4];
+    float up[4];
+    float resolution[2];
+    float jitter[2];
+    float prev_vp[16];
+} rkf_camera_uniforms;
+
+/* gpu_scene.rs:21-43; defaults 0/128/100.0/0.001 (gpu_scene.rs:34-43). */
+typedef struct rkf_scene_uniforms {
+    uint32_t num_objects;
+    uint32_t max_steps;
+    float max_distance;
+    float hit_threshold;
+} rkf_scene_uniforms;
+
+/* coarse_field.rs:31-42. */
+typedef struct rkf_coarse_field_uniforms {
+    float origin_cam_rel[4];
+    uint32_t dims[4];
+    float voxel_size;
+    float inv_voxel_size;
+    float _pad[2];
+} rkf_coarse_field_uniforms;
+
+/* gpu_bvh.rs:24-41 (WGSL ray_march.wgsl:99-108). left == RKF_BVH_INVALID
+ * marks a leaf whose right_or_object is the object index. */
+typedef struct rkf_bvh_node {
+    float aabb_min[3];
+    uint32_t left;
+    float aabb_max[3];
+    uint32_t right_or_object;
+} rkf_bvh_node;
+
+/* ------------------------------------------------------------------------ */
+/* Extension records (no reference counterpart; zero-initialised = reference */
+/* behaviour)                                                                */
+/* ------------------------------------------------------------------------ */
+
+#define RKFM_EPS_ABSOLUTE 0u /* hit if d < hit_threshold (ray_march.wgsl:966) */
+#define RKFM_EPS_RELATIVE 1u /* hit if d < hit_threshold * t                  */
+
+/* Strategy override for the drop-in kernel: AUTO follows ray_march.wgsl:1056-1063
+ * (tiled if tile lists are set, else BVH if nodes are set, else brute). */
+#define RKFM_STRATEGY_AUTO 0u
+#define RKFM_STRATEGY_TILED 1u
+#define RKFM_STRATEGY_BVH 2u
+#define RKFM_STRATEGY_BRUTE 3u
+
+typedef struct rkfm_scene_ext {
+    uint32_t hit_eps_mode;   /* RKFM_EPS_*                                  */
+    uint32_t strategy;       /* RKFM_STRATEGY_*                             */
+    uint32_t shadow_enable;  /* 1: trace one soft-shadow ray per hit        */
+    uint32_t _reserved0;
+    float light_dir[3];      /* unit vector toward the light                */
+    float shadow_max_dist;   /* 0 => 25.0  (shade_common.wgsl:276)          */
+    float shadow_k;          /* 0 => 16.0  (shade_common.wgsl:275)          */
+    float _reserved1[3];
+} rkfm_scene_ext;
+
+/* Per-pixel debug record (optional plane). All integer fields are exact. */
+typedef struct rkfm_debug_px {
+    float normal[3];     /* f32 normal before the f16 G-buffer store        */
+    uint32_t brick_slot; /* pool slot of the voxel holding the hit point    */
+                         /* (RKF_EMPTY_SLOT on miss / sentinel)             */
+    uint32_t voxel_index;/* x + 8y + 64z inside that brick (0xFFFFFFFF miss) */
+    uint32_t cell_index; /* flat L2 cell (sparse scenes) or object index    */
+    uint32_t steps;      /* march loop iterations (incl. DDA skips)         */
+    uint32_t evals;      /* SDF evaluations made (incl. bisection)          */
+    uint32_t shadow_steps; /* shadow-ray loop iterations (0 if none)        */
+    uint32_t shadow_evals; /* shadow-ray SDF evaluations                    */
+    uint32_t _pad[2];
+} rkfm_debug_px;
+
+/* G-buffer planes, row-major (x fastest), in the reference formats
+ * (gbuffer.rs:52-58): position rgba32f, normal rgba16f, material r32u,
+ * motion rgba32f. Optional planes may be NULL. All are device pointers. */
+typedef struct rkfm_gbuffer {
+    void *position;     /* float[4] per pixel                               */
+    void *normal;       /* uint16_t[4] (binary16) per pixel                 */
+    void *material;     /* uint32_t per pixel                               */
+    void *motion;       /* float[4] per pixel                               */
+    void *debug;        /* rkfm_debug_px per pixel, optional                */
+    void *shadow;       /* float per pixel (visibility), optional           */
+    uint32_t width;
+    uint32_t height;
+} rkfm_gbuffer;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct rkfm_ctx rkfm_ctx;
+
+typedef struct rkfm_config {
+    int device;               /* HIP device ordinal                          */
+    uint32_t flags;           /* reserved, 0                                 */
+    uint64_t memory_budget;   /* bytes the served loop may hold; 0 = no cap  */
+} rkfm_config;
+
+const char *rkfm_last_error_string(void);
+const char *rkfm_version(void);
+
+/* RayMarchPass::new (ray_march.rs:42-80): creates device state and loads the
+ * kernels; no pipeline compilation happens at run time. */
+int rkfm_create(const rkfm_config *cfg, rkfm_ctx **out);
+int rkfm_destroy(rkfm_ctx *ctx);
+int rkfm_sync(rkfm_ctx *ctx);
+/* The context's stream (hipStream_t) for callers that want to chain work. */
+void *rkfm_stream(rkfm_ctx *ctx);
+
+/* GpuScene uploads (render.rs:106-137, gpu_scene.rs:141-195). The device
+ * buffers grow as needed; first_slot / offset address the destination. */
+int rkfm_upload_brick_pool(rkfm_ctx *ctx, const rkf_voxel_sample *bricks,
+                           uint64_t n_bricks, uint64_t first_slot);
+int rkfm_upload_brick_maps(rkfm_ctx *ctx, const uint32_t *entries,
+                           uint64_t n, uint64_t offset);
+int rkfm_upload_objects(rkfm_ctx *ctx, const rkf_gpu_object *objects,
+                        uint32_t n);
+int rkfm_upload_bvh(rkfm_ctx *ctx, const rkf_bvh_node *nodes, uint32_t n);
+int rkfm_set_camera(rkfm_ctx *ctx, const rkf_camera_uniforms *camera);
+int rkfm_set_scene(rkfm_ctx *ctx, const rkf_scene_uniforms *scene,
+                   const rkfm_scene_ext *ext /* NULL = reference */);
+/* Per-tile object lists as produced by tile_object_cull.wgsl:117-204:
+ * idx holds tiles_x*tiles_y*32 u32, counts tiles_x*tiles_y u32. Passing
+ * tiles_x == 0 clears the lists (strategy AUTO then falls back to BVH/brute). */
+int rkfm_set_tiles(rkfm_ctx *ctx, const uint32_t *idx, const uint32_t *counts,
+                   uint32_t tiles_x, uint32_t tiles_y);
+/* Coarse field (coarse_field.rs:240-307): dims.x*dims.y*dims.z binary16
+ * cells, x fastest. NULL cells clears the field (every lookup returns 0). */
+int rkfm_set_coarse_field(rkfm_ctx *ctx, const uint16_t *f16_cells,
+                          const rkf_coarse_field_uniforms *uniforms);
+
+/* Producers of the march inputs, on the GPU (tile_object_cull.wgsl:117-204 and
+ * CoarseField::populate coarse_field.rs:240-266). Tile lists are ordered by
+ * ascending object index, keeping the first 32. */
+int rkfm_build_tiles(rkfm_ctx *ctx, uint32_t width, uint32_t height);
+int rkfm_build_coarse_field(rkfm_ctx *ctx, const float origin[3],
+                            const uint32_t dims[3], float voxel_size);
+/* Read back what the producers built (host pointers; sizes in elements). */
+int rkfm_read_tiles(rkfm_ctx *ctx, uint32_t *idx, uint32_t *counts,
+                    uint64_t n_tiles);
+int rkfm_read_coarse_field(rkfm_ctx *ctx, uint16_t *cells, uint64_t n_cells,
+                           rkf_coarse_field_uniforms *uniforms);
+
+/* RayMarchPass::dispatch (ray_march.rs:96-118): one thread per pixel of
+ * out->width x out->height, 8x8 pixel tiles per wavefront. */
+int rkfm_dispatch(rkfm_ctx *ctx, const rkfm_gbuffer *out, void *hip_stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side scene building (restates rkf-core voxelize_object.rs:53-182)    */
+/* ------------------------------------------------------------------------ */
+
+/* Analytic primitive record used by the scene builders. type: 0 sphere
+ * (p0 = radius), 1 box (p0..p2 = half extents), 2 capsule (p0 = radius,
+ * p1 = half height along local y). rot is a row-major 3x3 world->local
+ * rotation applied after subtracting centre. group: primitives with the same
+ * group are blended with smin(k) (sdf.rs:48-51), groups are unioned by min. */
+typedef struct rkfm_prim {
+    float centre[3];
+    uint32_t type;
+    float params[4];
+    float rot[9];
+    uint32_t group;
+    uint32_t material; /* used when the scene has no per-voxel material hash */
+    uint32_t _pad;
+} rkfm_prim;
+
+/* voxelize_sdf (voxelize_object.rs:53-157) of one sphere |p| - r with the
+ * AABB +-(r + margin): writes the brick map (dims product entries, slots
+ * numbered first_slot, first_slot+1, ... in z-y-x order as Pool::allocate_range
+ * yields them) and the bricks (4 KiB each). Per-voxel material is
+ * hash(seed, object, voxel) & 15 when mat_seed != 0, else `material`.
+ * Call with bricks == NULL to size: *n_bricks and dims are filled. */
+int rkfm_voxelize_sphere(float radius, float margin, float voxel_size,
+                         uint32_t object_index, uint64_t mat_seed,
+                         uint32_t material, uint32_t first_slot,
+                         uint32_t dims[3], uint32_t *map_entries,
+                         rkf_voxel_sample *bricks, uint64_t *n_bricks);
+
+/* The 5-sphere smooth union of crates/rkf-core/examples/gen_test_rkf.rs:22-127
+ * (vs 0.03, k 0.15, material 6), voxelised the same way. */
+int rkfm_voxelize_test_cross(uint32_t dims[3], uint32_t *map_entries,
+                             rkf_voxel_sample *bricks, uint64_t *n_bricks);
+
+/* Camera looking from `position` at `target` with the reference fly-camera
+ * parameterisation (camera.rs:75-93, uniforms camera.rs:161-184). */
+int rkfm_camera_look_at(const float position[3], const float target[3],
+                        float fov_degrees, uint32_t width, uint32_t height,
+                        rkf_camera_uniforms *out);
+
+/* Median-split BVH over object AABBs (aabb_min/aabb_max of the objects).
+ * nodes must hold 2*n-1 entries; *n_nodes receives the count. */
+int rkfm_build_bvh(const rkf_gpu_object *objects, uint32_t n,
+                   rkf_bvh_node *nodes, uint32_t *n_nodes);
+
+/* ------------------------------------------------------------------------ */
+/* Sparse world scenes: two-level index (L2 cells of 8^3 bricks -> L1 brick   */
+/* slots) over a structure-of-arrays brick pool (f16 distance plane + u8     */
+/* material plane), resident in HBM. No reference counterpart: it holds      */
+/* scenes the reference's 1 GiB pool binding (context.rs:46) cannot.         */
+/* ------------------------------------------------------------------------ */
+typedef struct rkfm_sparse_scene rkfm_sparse_scene;
+
+typedef struct rkfm_sparse_params {
+    float voxel_size;       /* vs; brick extent = 8 vs, L2 cell = 64 vs      */
+    float smin_k;           /* group blend radius (0 = hard min)           */
+    uint64_t mat_seed;      /* per-voxel material hash seed (0 = prim mat) */
+    float bounds_min[3];    /* world region to voxelise                    */
+    float bounds_max[3];
+    uint64_t max_bricks;    /* capacity cap (0 = no cap beyond memory)     */
+} rkfm_sparse_params;
+
+typedef struct rkfm_sparse_info {
+    float origin[3];        /* world position of global voxel corner 0     */
+    float voxel_size;
+    uint32_t cells[3];      /* L2 grid dimensions                          */
+    uint32_t n_blocks;      /* allocated L1 blocks (occupied L2 cells)     */
+    uint64_t n_bricks;      /* allocated bricks                            */
+    uint64_t device_bytes;  /* HBM held by the scene                       */
+} rkfm_sparse_info;
+
+/* Voxelise primitives on the GPU into a new resident sparse scene. */
+int rkfm_sparse_build(rkfm_ctx *ctx, const rkfm_prim *prims, uint32_t n_prims,
+                      const rkfm_sparse_params *params,
+                      rkfm_sparse_scene **out);
+/* Build a sparse scene from a reference dense brick map + AoS bricks (one
+ * object, translation only: `origin` = world position of the map's grid
+ * corner). Used to stream .rkf assets and fixtures into resident scenes. */
+int rkfm_sparse_from_dense(rkfm_ctx *ctx, const uint32_t *map,
+                           const uint32_t dims[3],
+                           const rkf_voxel_sample *bricks, uint64_t n_bricks,
+                           float voxel_size, const float origin[3],
+                           rkfm_sparse_scene **out);
+int rkfm_sparse_info_get(const rkfm_sparse_scene *scene,
+                         rkfm_sparse_info *info);
+int rkfm_sparse_destroy(rkfm_ctx *ctx, rkfm_sparse_scene *scene);
+/* Host copies of the index and pool (sizes from rkfm_sparse_info): l2 has
+ * cells.x*cells.y*cells.z u32, l1 n_blocks*512 u32, dist n_bricks*512
+ * binary16, mat n_bricks*512 u8. Any pointer may be NULL. */
+int rkfm_sparse_download(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                         uint32_t *l2, uint32_t *l1, uint16_t *dist,
+                         uint8_t *mat);
+
+/* March a sparse scene: ray generation, sphere tracing with the reference
+ * step rules, multi-level DDA over free space, normals, G-buffer write and
+ * optional shadow rays. variant 0 = tuned kernel, 1 = plain one-lane-per-ray
+ * kernel (same results). */
+int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                      const rkf_camera_uniforms *camera,
+                      const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
+                      const rkfm_gbuffer *out, uint32_t variant,
+                      void *hip_stream);
+
+/* Point queries (SdfQueryable semantics, sdf_collision.rs:38-56; result
+ * fields as SpatialQueryResult automation/types.rs:151-158 plus gradient). */
+typedef struct rkfm_point_result {
+    float distance;
+    uint32_t material;
+    uint32_t inside;
+    float gradient[3];
+} rkfm_point_result;
+
+int rkfm_sparse_query_points(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                             const float *xyz_dev, uint32_t n,
+                             rkfm_point_result *out_dev, void *hip_stream);
+/* Free-space masks used by the DDA: free_idx has one u32 per L2 cell
+ * (0xFFFFFFFF = every brick of the cell is free, else a row of free_bits);
+ * free_bits holds 8 u64 per row, bit (x + 8y + 64z) set = that brick is not
+ * free. *n_rows receives the row count (free_bits needs 8 * n_rows words). */
+int rkfm_sparse_download_free(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                              uint32_t *free_idx, uint64_t *free_bits,
+                              uint64_t *n_rows);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and timing helpers (the C ABI takes device pointers)        */
+/* ------------------------------------------------------------------------ */
+int rkfm_alloc(rkfm_ctx *ctx, uint64_t bytes, void **dev_ptr);
+int rkfm_free(rkfm_ctx *ctx, void *dev_ptr);
+int rkfm_memcpy_h2d(rkfm_ctx *ctx, void *dst_dev, const void *src_host,
+                    uint64_t bytes);
+int rkfm_memcpy_d2h(rkfm_ctx *ctx, void *dst_host, const void *src_dev,
+                    uint64_t bytes);
+int rkfm_memset(rkfm_ctx *ctx, void *dst_dev, int value, uint64_t bytes);
+/* HIP events on the context stream around the work enqueued in between. */
+int rkfm_timer_begin(rkfm_ctx *ctx);
+int rkfm_timer_end(rkfm_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+/* Layout checks (the reference asserts the same sizes in its unit tests:
+ * gpu_object.rs:272, camera.rs:220, gpu_scene.rs:380). */
+#ifdef __cplusplus
+static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
+static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
+static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes");
+static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
+static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
+static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
+#else
+_Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
+_Static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
+_Static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes");
+_Static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
+_Static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
+_Static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
+#endif
+
+#endif /* RKF_MARCH_H */

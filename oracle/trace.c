@@ -1,0 +1,282 @@
+/*
+ * trace.c — instantiates trace_impl.h as the FP32 twin and the FP64 oracle,
+ * and restates the march's producers (coarse field, tile cull). Test
+ * infrastructure only (see oracle.h). Build with -ffp-contract=off.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle.h"
+
+#define R float
+#define SFX _f32
+#define SQRT sqrtf
+#define FLOOR floorf
+#define FABS fabsf
+#define FMIN fminf
+#define FMAX fmaxf
+#define ISFINITE isfinite
+#include "trace_impl.h"
+#undef R
+#undef SFX
+#undef SQRT
+#undef FLOOR
+#undef FABS
+#undef FMIN
+#undef FMAX
+
+#define R double
+#define SFX _f64
+#define SQRT sqrt
+#define FLOOR floor
+#define FABS fabs
+#define FMIN fmin
+#define FMAX fmax
+#include "trace_impl.h"
+#undef R
+#undef SFX
+
+void orc_ref_trace(const orc_ref_scene *s, int prec, uint64_t p0, uint64_t p1, orc_ray_result *out) {
+    for (uint64_t p = p0; p < p1; p++) {
+        if (prec == 64) ref_pixel_f64(s, p, out + (p - p0));
+        else ref_pixel_f32(s, p, out + (p - p0));
+    }
+}
+
+int orc_ref_trace_parallel(const orc_ref_scene *s, orc_ray_result *out, int threads) {
+    uint64_t n = (uint64_t)s->width * s->height;
+    int used = 1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+#pragma omp single
+        used = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t p = 0; p < (int64_t)n; p++) ref_pixel_f32(s, (uint64_t)p, out + p);
+    }
+#else
+    (void)threads;
+    for (uint64_t p = 0; p < n; p++) ref_pixel_f32(s, p, out + p);
+#endif
+    return used;
+}
+
+uint64_t orc_sparse_n_bricks_global(const orc_sparse_scene *s) {
+    return (uint64_t)s->cells[0] * 8u * s->cells[1] * 8u * s->cells[2] * 8u;
+}
+
+static int brick_empty(const orc_sparse_scene *s, uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t cell = (uint64_t)(x >> 3) + (uint64_t)(y >> 3) * s->cells[0] +
+                    (uint64_t)(z >> 3) * s->cells[0] * s->cells[1];
+    uint32_t blk = s->l2[cell];
+    if (blk == RKF_EMPTY_SLOT) return 1;
+    return s->l1[(uint64_t)blk * 512u + (x & 7u) + (y & 7u) * 8u + (z & 7u) * 64u] == RKF_EMPTY_SLOT;
+}
+
+/* Brick b is free when every brick of {b..b+1}^3 (clamped to the grid) is EMPTY. */
+int orc_sparse_brick_free(const orc_sparse_scene *s, uint32_t bx, uint32_t by, uint32_t bz) {
+    uint32_t nb[3] = {s->cells[0] * 8u, s->cells[1] * 8u, s->cells[2] * 8u};
+    for (uint32_t dz = 0; dz < 2; dz++)
+        for (uint32_t dy = 0; dy < 2; dy++)
+            for (uint32_t dx = 0; dx < 2; dx++) {
+                uint32_t x = bx + dx < nb[0] ? bx + dx : nb[0] - 1;
+                uint32_t y = by + dy < nb[1] ? by + dy : nb[1] - 1;
+                uint32_t z = bz + dz < nb[2] ? bz + dz : nb[2] - 1;
+                if (!brick_empty(s, x, y, z)) return 0;
+            }
+    return 1;
+}
+
+/* free_l2[cell] = every brick b of the cell is free, i.e. no allocated brick
+ * b' lies in [8C, 8C+8]^3 (brick coordinates, clamped): each allocated brick
+ * b' clears the cells C with 8C <= b' <= 8C+8 per axis. */
+void orc_sparse_free_l2(const orc_sparse_scene *s, uint8_t *free_l2) {
+    uint64_t nc = (uint64_t)s->cells[0] * s->cells[1] * s->cells[2];
+    for (uint64_t c = 0; c < nc; c++) free_l2[c] = 1;
+    for (uint32_t qz = 0; qz < s->cells[2]; qz++)
+        for (uint32_t qy = 0; qy < s->cells[1]; qy++)
+            for (uint32_t qx = 0; qx < s->cells[0]; qx++) {
+                uint64_t cell = (uint64_t)qx + (uint64_t)qy * s->cells[0] + (uint64_t)qz * s->cells[0] * s->cells[1];
+                uint32_t blk = s->l2[cell];
+                if (blk == RKF_EMPTY_SLOT) continue;
+                for (uint32_t lb = 0; lb < 512; lb++) {
+                    if (s->l1[(uint64_t)blk * 512u + lb] == RKF_EMPTY_SLOT) continue;
+                    uint32_t b[3] = {qx * 8u + (lb & 7u), qy * 8u + ((lb >> 3) & 7u), qz * 8u + (lb >> 6)};
+                    uint32_t lo[3], hi[3];
+                    for (int a = 0; a < 3; a++) {
+                        /* cells C with 8C <= b <= 8C + 8 */
+                        hi[a] = b[a] >> 3;
+                        lo[a] = (b[a] >= 8u && (b[a] & 7u) == 0) ? hi[a] - 1 : hi[a];
+                    }
+                    for (uint32_t z = lo[2]; z <= hi[2]; z++)
+                        for (uint32_t y = lo[1]; y <= hi[1]; y++)
+                            for (uint32_t x = lo[0]; x <= hi[0]; x++)
+                                free_l2[(uint64_t)x + (uint64_t)y * s->cells[0] +
+                                        (uint64_t)z * s->cells[0] * s->cells[1]] = 0;
+                }
+            }
+}
+
+void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
+                      const rkfm_scene_ext *ext, uint32_t w, uint32_t h, int prec, uint64_t p0, uint64_t p1,
+                      orc_ray_result *out) {
+    for (uint64_t p = p0; p < p1; p++) {
+        if (prec == 64) sp_pixel_f64(s, cam, su, ext, w, h, p, out + (p - p0));
+        else sp_pixel_f32(s, cam, su, ext, w, h, p, out + (p - p0));
+    }
+}
+
+int orc_sparse_trace_parallel(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
+                              const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, uint32_t w, uint32_t h,
+                              uint64_t p0, uint64_t p1, orc_ray_result *out, int threads) {
+    int used = 1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+#pragma omp single
+        used = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t p = (int64_t)p0; p < (int64_t)p1; p++)
+            sp_pixel_f32(s, cam, su, ext, w, h, (uint64_t)p, out + (p - (int64_t)p0));
+    }
+#else
+    (void)threads;
+    for (uint64_t p = p0; p < p1; p++) sp_pixel_f32(s, cam, su, ext, w, h, p, out + (p - p0));
+#endif
+    return used;
+}
+
+void orc_sparse_query(const orc_sparse_scene *s, const float *xyz, uint64_t n, int prec, rkfm_point_result *out) {
+    for (uint64_t i = 0; i < n; i++) {
+        if (prec == 64) sp_query_f64(s, xyz + 3 * i, out + i);
+        else sp_query_f32(s, xyz + 3 * i, out + i);
+    }
+}
+
+double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec) {
+    if (prec == 64) {
+        sctx_f64 c;
+        sp_init_f64(&c, s);
+        return sp_sample_f64(&c, mk_f64(p[0], p[1], p[2]), 0, 0);
+    }
+    sctx_f32 c;
+    sp_init_f32(&c, s);
+    return (double)sp_sample_f32(&c, mk_f32(p[0], p[1], p[2]), 0, 0);
+}
+
+/* G-buffer packing (ray_march.wgsl:1083-1111). Miss: (0,0,0,MAX_FLOAT), 0, 0, 0. */
+void orc_pack_gbuffer(const orc_ray_result *r, uint64_t n, float *position, uint16_t *normal, uint32_t *material,
+                      float *motion) {
+    for (uint64_t i = 0; i < n; i++) {
+        const orc_ray_result *x = r + i;
+        if (x->hit) {
+            position[4 * i + 0] = (float)x->pos[0];
+            position[4 * i + 1] = (float)x->pos[1];
+            position[4 * i + 2] = (float)x->pos[2];
+            position[4 * i + 3] = (float)x->t;
+            normal[4 * i + 0] = orc_f32_to_f16((float)x->normal[0]);
+            normal[4 * i + 1] = orc_f32_to_f16((float)x->normal[1]);
+            normal[4 * i + 2] = orc_f32_to_f16((float)x->normal[2]);
+            normal[4 * i + 3] = 0;
+            material[i] = x->packed_material;
+            motion[4 * i + 0] = (float)x->motion[0];
+            motion[4 * i + 1] = (float)x->motion[1];
+            motion[4 * i + 2] = 1.0f;
+            motion[4 * i + 3] = (float)x->sdf_at_hit;
+        } else {
+            position[4 * i + 0] = position[4 * i + 1] = position[4 * i + 2] = 0.0f;
+            position[4 * i + 3] = 3.402823e+38f;
+            normal[4 * i + 0] = normal[4 * i + 1] = normal[4 * i + 2] = normal[4 * i + 3] = 0;
+            material[i] = 0;
+            motion[4 * i + 0] = motion[4 * i + 1] = motion[4 * i + 2] = motion[4 * i + 3] = 0.0f;
+        }
+    }
+}
+
+/* ---- producers ---- */
+
+/* coarse_field.rs:76-79 aabb_unsigned_distance, :240-266 populate. */
+void orc_coarse_populate(const float origin[3], const uint32_t dims[3], float vs, const rkf_gpu_object *objs,
+                         uint32_t n, uint16_t *cells) {
+    uint32_t dmax = dims[0] > dims[1] ? dims[0] : dims[1];
+    dmax = dmax > dims[2] ? dmax : dims[2];
+    for (uint32_t z = 0; z < dims[2]; z++)
+        for (uint32_t y = 0; y < dims[1]; y++)
+            for (uint32_t x = 0; x < dims[0]; x++) {
+                float wp[3] = {origin[0] + ((float)x + 0.5f) * vs, origin[1] + ((float)y + 0.5f) * vs,
+                               origin[2] + ((float)z + 0.5f) * vs};
+                float md = vs * (float)dmax;
+                for (uint32_t i = 0; i < n; i++) {
+                    float d[3];
+                    for (int a = 0; a < 3; a++)
+                        d[a] = fmaxf(fmaxf(objs[i].aabb_min[a] - wp[a], wp[a] - objs[i].aabb_max[a]), 0.0f);
+                    float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    md = fminf(md, l);
+                }
+                cells[(uint64_t)x + (uint64_t)y * dims[0] + (uint64_t)z * dims[0] * dims[1]] = orc_f32_to_f16(md);
+            }
+}
+
+/* tile_object_cull.wgsl:90-114 project_to_pixel */
+static void project(const rkf_camera_uniforms *c, const float cr[3], float out[3]) {
+    float fx = c->forward[0], fy = c->forward[1], fz = c->forward[2];
+    float depth = cr[0] * fx + cr[1] * fy + cr[2] * fz;
+    if (depth <= 0.0f) { out[0] = -1.0f; out[1] = -1.0f; out[2] = depth; return; }
+    const float *r = c->right, *u = c->up;
+    float rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    float nx = (cr[0] * r[0] + cr[1] * r[1] + cr[2] * r[2]) / (depth * rsq);
+    float ny = (cr[0] * u[0] + cr[1] * u[1] + cr[2] * u[2]) / (depth * usq);
+    out[0] = (nx * 0.5f + 0.5f) * c->resolution[0];
+    out[1] = (0.5f - ny * 0.5f) * c->resolution[1];
+    out[2] = depth;
+}
+
+/* tile_object_cull.wgsl:117-204 */
+void orc_tile_cull(const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su, const rkf_gpu_object *objs,
+                   uint32_t w, uint32_t h, uint32_t *idx, uint32_t *counts) {
+    uint32_t ntx = (w + 15u) / 16u, nty = (h + 15u) / 16u;
+    for (uint32_t ty = 0; ty < nty; ty++)
+        for (uint32_t tx = 0; tx < ntx; tx++) {
+            float tmin[2] = {(float)(tx * 16u), (float)(ty * 16u)};
+            float tmax[2] = {tmin[0] + 16.0f, tmin[1] + 16.0f};
+            uint32_t cnt = 0;
+            uint32_t tid = ty * ntx + tx;
+            for (uint32_t i = 0; i < su->num_objects && cnt < 32u; i++) {
+                const rkf_gpu_object *o = &objs[i];
+                if (o->sdf_type == RKF_SDF_TYPE_NONE) continue;
+                float smin[2] = {cam->resolution[0], cam->resolution[1]}, smax[2] = {0.0f, 0.0f};
+                int front = 0, behind = 0;
+                for (uint32_t c = 0; c < 8; c++) {
+                    float corner[3] = {(c & 1u) ? o->aabb_max[0] : o->aabb_min[0],
+                                       (c & 2u) ? o->aabb_max[1] : o->aabb_min[1],
+                                       (c & 4u) ? o->aabb_max[2] : o->aabb_min[2]};
+                    float cr[3] = {corner[0] - cam->position[0], corner[1] - cam->position[1],
+                                   corner[2] - cam->position[2]};
+                    float pr[3];
+                    project(cam, cr, pr);
+                    if (pr[2] > 0.0f) {
+                        front = 1;
+                        smin[0] = fminf(smin[0], pr[0]);
+                        smin[1] = fminf(smin[1], pr[1]);
+                        smax[0] = fmaxf(smax[0], pr[0]);
+                        smax[1] = fmaxf(smax[1], pr[1]);
+                    } else {
+                        behind = 1;
+                    }
+                }
+                int ov = 0;
+                if (behind && front) ov = 1;
+                else if (!behind && front)
+                    ov = smax[0] >= tmin[0] && smin[0] < tmax[0] && smax[1] >= tmin[1] && smin[1] < tmax[1];
+                if (ov) idx[tid * 32u + cnt++] = i;
+            }
+            counts[tid] = cnt;
+        }
+}

@@ -1,0 +1,66 @@
+// host_internal.h — context and buffer types private to librkf_march.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/rkf_march.h"
+#include "kernels.h"
+
+int rkf_set_error(int code, const char *fmt, ...);
+int rkf_hip_check(hipError_t e, const char *what);
+int rkf_fill_gbuf(const rkfm_gbuffer *out, rkf::GbufPtrs &g);
+
+// Device buffer that grows by reallocation (GpuScene's buffers grow the same
+// way, gpu_scene.rs:141-195).
+struct DevBuf {
+    void *ptr = nullptr;
+    uint64_t cap = 0;
+    uint64_t size = 0;
+    int reserve(uint64_t want, bool keep);
+    void release();
+};
+
+struct rkfm_sparse_scene {
+    rkfm_ctx *owner = nullptr;
+    rkfm_sparse_info info{};
+    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits;
+    uint64_t n_free_rows = 0;
+    rkf::SparseDev dev() const;
+};
+
+struct rkfm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t memory_budget = 0;
+    // GpuScene (reference layout)
+    DevBuf pool, maps, objects, bvh, tile_idx, tile_counts, coarse;
+    uint64_t n_pool_voxels = 0, n_maps = 0;
+    uint32_t n_objects = 0, n_bvh = 0, tiles_x = 0, tiles_y = 0;
+    rkf_camera_uniforms cam{};
+    rkf_scene_uniforms su{};
+    rkfm_scene_ext ext{};
+    rkf_coarse_field_uniforms coarse_u{};
+    bool cam_set = false, scene_set = false, coarse_set = false;
+    // Resident sparse scenes.
+    std::vector<rkfm_sparse_scene *> scenes;
+};
+
+void rkf_sparse_release_all(rkfm_ctx *c);
+
+// 64-bit finaliser (splitmix64) used for the per-voxel material hash of the
+// synthetic scenes; the same function runs on the device in sparse.hip.
+static inline uint64_t rkf_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// U{0..15} per (seed, object, global voxel coordinate).
+static inline uint32_t rkf_voxel_material(uint64_t seed, uint32_t object, uint32_t gx, uint32_t gy, uint32_t gz) {
+    uint64_t key = (uint64_t)gx | ((uint64_t)gy << 21) | ((uint64_t)gz << 42);
+    return (uint32_t)(rkf_mix64(seed ^ rkf_mix64(key ^ ((uint64_t)object * 0xD6E8FEB86659FD93ull))) & 15u);
+}

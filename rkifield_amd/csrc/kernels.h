@@ -1,0 +1,83 @@
+// kernels.h — launch parameter records shared by the host code and kernels.
+#pragma once
+
+#include "device_common.h"
+
+namespace rkf {
+
+// Drop-in march over the reference layouts (march_ref.hip).
+struct RefParams {
+    const rkf_voxel_sample *pool;
+    uint64_t n_pool_voxels;
+    const uint32_t *maps;
+    uint64_t n_maps;
+    const rkf_gpu_object *objects;
+    const rkf_bvh_node *bvh;
+    uint32_t n_bvh;
+    const uint32_t *tile_idx;
+    const uint32_t *tile_counts;
+    uint32_t n_tiles;
+    const uint16_t *coarse;  // nullptr: no coarse field
+    rkf_coarse_field_uniforms cu;
+    rkf_camera_uniforms cam;
+    rkf_scene_uniforms su;
+    uint32_t eps_mode;
+    uint32_t mode;  // 0 tiled, 1 bvh, 2 brute
+    GbufPtrs g;
+};
+
+// Producers (producers.hip).
+struct TileCullParams {
+    const rkf_gpu_object *objects;
+    uint32_t num_objects;
+    rkf_camera_uniforms cam;
+    uint32_t width, height, tiles_x, tiles_y;
+    uint32_t *idx;
+    uint32_t *counts;
+};
+struct CoarseParams {
+    const rkf_gpu_object *objects;
+    uint32_t n_objects;
+    float origin[3];
+    uint32_t dims[3];
+    float voxel_size;
+    uint16_t *cells;
+};
+
+// Sparse two-level scene as the kernels see it (sparse.hip).
+constexpr uint32_t kFreeCell = 0xFFFFFFFFu;
+struct SparseDev {
+    float origin[3];
+    float vs;
+    uint32_t cells[3];
+    uint32_t nb[3];   // bricks per axis = cells * 8
+    int32_t tot[3];   // voxels per axis = cells * 64
+    const uint32_t *l2;        // per L2 cell: L1 block or EMPTY
+    const uint32_t *l1;        // per block: 512 slots (brick x + 8y + 64z in cell)
+    const uint16_t *dist;      // per slot: 512 binary16 distances
+    const uint8_t *mat;        // per slot: 512 material ids (6 bits)
+    const uint16_t *aux;       // per slot: 512 (secondary | blend << 8), or null
+    const uint32_t *free_idx;  // per L2 cell: kFreeCell or row of free_bits
+    const uint64_t *free_bits; // 8 words per row: bit set = brick NOT free
+    uint64_t n_cells;
+};
+
+struct SparseMarchParams {
+    SparseDev S;
+    rkf_camera_uniforms cam;
+    rkf_scene_uniforms su;
+    rkfm_scene_ext ext;
+    GbufPtrs g;
+    uint32_t object_id;
+};
+
+}  // namespace rkf
+
+extern "C" {
+hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream);
+hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream);
+hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream);
+hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, hipStream_t stream);
+hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,
+                                   hipStream_t stream);
+}

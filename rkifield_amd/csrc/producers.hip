@@ -1,0 +1,124 @@
+// producers.hip — the march's input producers on the GPU.
+//
+// * Tile object cull: tile_object_cull.wgsl:117-204. One 256-lane workgroup
+//   per 16x16 pixel tile tests objects in strides of 256; the list keeps the
+//   first 32 overlapping objects in ascending index order (the WGSL appends in
+//   atomic order, which is not deterministic).
+// * Coarse field: CoarseField::populate (coarse_field.rs:240-266), which the
+//   reference runs on the CPU every frame; one lane per cell.
+#include "kernels.h"
+
+namespace rkf {
+
+// tile_object_cull.wgsl:90-114
+RKF_DEV void project(const rkf_camera_uniforms &c, float crx, float cry, float crz, float &ox, float &oy, float &oz) {
+    float depth = crx * c.forward[0] + cry * c.forward[1] + crz * c.forward[2];
+    if (depth <= 0.0f) {
+        ox = -1.0f;
+        oy = -1.0f;
+        oz = depth;
+        return;
+    }
+    float rsq = c.right[0] * c.right[0] + c.right[1] * c.right[1] + c.right[2] * c.right[2];
+    float usq = c.up[0] * c.up[0] + c.up[1] * c.up[1] + c.up[2] * c.up[2];
+    float nx = (crx * c.right[0] + cry * c.right[1] + crz * c.right[2]) / (depth * rsq);
+    float ny = (crx * c.up[0] + cry * c.up[1] + crz * c.up[2]) / (depth * usq);
+    ox = (nx * 0.5f + 0.5f) * c.resolution[0];
+    oy = (0.5f - ny * 0.5f) * c.resolution[1];
+    oz = depth;
+}
+
+RKF_DEV bool overlaps(const TileCullParams &P, const rkf_gpu_object &o, float tminx, float tminy) {
+    if (o.sdf_type == RKF_SDF_TYPE_NONE) return false;
+    float sminx = P.cam.resolution[0], sminy = P.cam.resolution[1], smaxx = 0.0f, smaxy = 0.0f;
+    bool front = false, behind = false;
+    for (uint32_t c = 0; c < 8; c++) {
+        float cx = (c & 1u) ? o.aabb_max[0] : o.aabb_min[0];
+        float cy = (c & 2u) ? o.aabb_max[1] : o.aabb_min[1];
+        float cz = (c & 4u) ? o.aabb_max[2] : o.aabb_min[2];
+        float px, py, pz;
+        project(P.cam, cx - P.cam.position[0], cy - P.cam.position[1], cz - P.cam.position[2], px, py, pz);
+        if (pz > 0.0f) {
+            front = true;
+            sminx = fminf(sminx, px);
+            sminy = fminf(sminy, py);
+            smaxx = fmaxf(smaxx, px);
+            smaxy = fmaxf(smaxy, py);
+        } else {
+            behind = true;
+        }
+    }
+    if (behind && front) return true;
+    if (!behind && front)
+        return smaxx >= tminx && sminx < tminx + 16.0f && smaxy >= tminy && sminy < tminy + 16.0f;
+    return false;
+}
+
+__global__ __launch_bounds__(256) void tile_cull_kernel(TileCullParams P) {
+    __shared__ uint32_t list[RKF_TILE_MAX_OBJECTS];
+    __shared__ uint32_t wave_cnt[4];
+    __shared__ uint32_t total;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t tx = blockIdx.x, ty = blockIdx.y;
+    const float tminx = (float)(tx * 16u), tminy = (float)(ty * 16u);
+    if (tid == 0) total = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < P.num_objects; base += 256u) {
+        uint32_t i = base + tid;
+        bool ov = i < P.num_objects && overlaps(P, P.objects[i], tminx, tminy);
+        uint64_t ball = __ballot(ov);
+        if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(ball);
+        __syncthreads();
+        uint32_t before = total;
+        for (uint32_t w = 0; w < wave; w++) before += wave_cnt[w];
+        uint32_t rank = before + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+        if (ov && rank < RKF_TILE_MAX_OBJECTS) list[rank] = i;
+        __syncthreads();
+        if (tid == 0) total += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+        if (total >= RKF_TILE_MAX_OBJECTS) break;
+    }
+    uint32_t cnt = total < RKF_TILE_MAX_OBJECTS ? total : RKF_TILE_MAX_OBJECTS;
+    uint32_t t = ty * P.tiles_x + tx;
+    if (tid < RKF_TILE_MAX_OBJECTS) P.idx[t * RKF_TILE_MAX_OBJECTS + tid] = tid < cnt ? list[tid] : 0u;
+    if (tid == 0) P.counts[t] = cnt;
+}
+
+// coarse_field.rs:76-79 (aabb_unsigned_distance) and :240-266.
+__global__ __launch_bounds__(256) void coarse_kernel(CoarseParams P) {
+    uint64_t n = (uint64_t)P.dims[0] * P.dims[1] * P.dims[2];
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x = (uint32_t)(i % P.dims[0]);
+    uint32_t y = (uint32_t)((i / P.dims[0]) % P.dims[1]);
+    uint32_t z = (uint32_t)(i / ((uint64_t)P.dims[0] * P.dims[1]));
+    float vs = P.voxel_size;
+    float wx = P.origin[0] + ((float)x + 0.5f) * vs;
+    float wy = P.origin[1] + ((float)y + 0.5f) * vs;
+    float wz = P.origin[2] + ((float)z + 0.5f) * vs;
+    uint32_t dmax = P.dims[0] > P.dims[1] ? P.dims[0] : P.dims[1];
+    dmax = dmax > P.dims[2] ? dmax : P.dims[2];
+    float md = vs * (float)dmax;
+    for (uint32_t k = 0; k < P.n_objects; k++) {
+        const rkf_gpu_object &o = P.objects[k];
+        float dx = fmaxf(fmaxf(o.aabb_min[0] - wx, wx - o.aabb_max[0]), 0.0f);
+        float dy = fmaxf(fmaxf(o.aabb_min[1] - wy, wy - o.aabb_max[1]), 0.0f);
+        float dz = fmaxf(fmaxf(o.aabb_min[2] - wz, wz - o.aabb_max[2]), 0.0f);
+        md = fminf(md, __builtin_sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+    P.cells[i] = (uint16_t)f2h(md);
+}
+
+}  // namespace rkf
+
+extern "C" hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream) {
+    hipLaunchKernelGGL(rkf::tile_cull_kernel, dim3(P->tiles_x, P->tiles_y, 1), dim3(256, 1, 1), 0, stream, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream) {
+    uint64_t n = (uint64_t)P->dims[0] * P->dims[1] * P->dims[2];
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rkf::coarse_kernel, dim3((unsigned)((n + 255) / 256), 1, 1), dim3(256, 1, 1), 0, stream, *P);
+    return hipGetLastError();
+}

@@ -1,0 +1,667 @@
+// sparse_build.hip — building resident sparse scenes on the GPU.
+//
+// rkfm_sparse_build voxelises analytic primitives the way voxelize_sdf does
+// (crates/rkf-core/src/voxelize_object.rs:53-182): a brick is allocated iff
+// |field(brick centre)| < 1.8 * brick extent, voxels store binary16 RNE of the
+// field at voxel centres brick_min + (v * vs + vs/2), slots are numbered in
+// brick order. Here the brick order is (L2 cell ascending, then z-y-x inside
+// the cell) and the field is
+//     field(p) = min over groups g of chain_g(p),
+//     chain_g  = d_0; d = smin(d, d_i, k) over the group's primitives in order
+// (sdf.rs:48-51 smin; evaluate_primitive voxelize_object.rs:439-474 after the
+// primitive's translation and rotation).
+//
+// Exactness of the culling: primitive j is binned into every L2 cell its
+// bounding sphere grown by M = 3 * brick + 2k touches, and a cell evaluates
+// only the groups binned there. An unbinned group has every primitive at
+// distance > M from the cell, so its chain exceeds M - 7k/4 > 3 bricks
+// (smin lowers its inputs' minimum by at most k/4 per step), while the field
+// at any voxel of an allocated brick is below 1.8 + 0.87 bricks: the culled
+// evaluation equals the full one wherever it is stored or tested.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "host_internal.h"
+
+namespace rkf {
+
+struct BuildDev {
+    const rkfm_prim *prims;
+    const uint32_t *group_prims;   // prim indices grouped, in original order
+    const uint32_t *group_start;   // per group
+    const uint32_t *group_count;
+    const uint32_t *cand_cell;     // per candidate cell: flat L2 index
+    const uint32_t *cand_off;      // CSR into cand_groups (n_cand + 1)
+    const uint32_t *cand_groups;
+    float origin[3];
+    float vs;
+    float k;
+    uint32_t cells[3];
+    uint64_t mat_seed;
+};
+
+RKF_DEV uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+RKF_DEV float box_sdf(float hx, float hy, float hz, float px, float py, float pz) {
+    float qx = fabsf(px) - hx, qy = fabsf(py) - hy, qz = fabsf(pz) - hz;
+    float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f), oz = fmaxf(qz, 0.0f);
+    float outside = __builtin_sqrtf(ox * ox + oy * oy + oz * oz);
+    float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+    return outside + inside;
+}
+
+// sdf.rs:21-37 capsule with a = (0,-hh,0), b = (0,hh,0).
+RKF_DEV float capsule_sdf(float r, float hh, float px, float py, float pz) {
+    float pax = px - 0.0f, pay = py - -hh, paz = pz - 0.0f;
+    float bax = 0.0f - 0.0f, bay = hh - -hh, baz = 0.0f - 0.0f;
+    float h = (pax * bax + pay * bay + paz * baz) / (bax * bax + bay * bay + baz * baz);
+    h = clampf(h, 0.0f, 1.0f);
+    float dx = pax - bax * h, dy = pay - bay * h, dz = paz - baz * h;
+    return __builtin_sqrtf(dx * dx + dy * dy + dz * dz) - r;
+}
+
+RKF_DEV float prim_eval(const rkfm_prim &pr, float x, float y, float z) {
+    float d0 = x - pr.centre[0], d1 = y - pr.centre[1], d2 = z - pr.centre[2];
+    const float *R = pr.rot;
+    float lx = R[0] * d0 + R[1] * d1 + R[2] * d2;
+    float ly = R[3] * d0 + R[4] * d1 + R[5] * d2;
+    float lz = R[6] * d0 + R[7] * d1 + R[8] * d2;
+    switch (pr.type) {
+    case 0: return __builtin_sqrtf(lx * lx + ly * ly + lz * lz) - pr.params[0];
+    case 1: return box_sdf(pr.params[0], pr.params[1], pr.params[2], lx, ly, lz);
+    case 2: return capsule_sdf(pr.params[0], pr.params[1], lx, ly, lz);
+    default: return kMaxFloat;
+    }
+}
+
+// sdf.rs:48-51
+RKF_DEV float smin(float a, float b, float k) {
+    float h = fmaxf(k - fabsf(a - b), 0.0f) / k;
+    return fminf(a, b) - h * h * k * 0.25f;
+}
+
+RKF_DEV float scene_field(const BuildDev &B, uint32_t cand, float x, float y, float z) {
+    float d = INFINITY;
+    uint32_t g0 = B.cand_off[cand], g1 = B.cand_off[cand + 1];
+    for (uint32_t gi = g0; gi < g1; gi++) {
+        uint32_t g = B.cand_groups[gi];
+        uint32_t s = B.group_start[g], n = B.group_count[g];
+        float r = prim_eval(B.prims[B.group_prims[s]], x, y, z);
+        for (uint32_t j = 1; j < n; j++) {
+            float dj = prim_eval(B.prims[B.group_prims[s + j]], x, y, z);
+            r = B.k > 0.0f ? smin(r, dj, B.k) : fminf(r, dj);
+        }
+        d = fminf(d, r);
+    }
+    return d;
+}
+
+// One block per candidate cell, one lane per brick (x + 8y + 64z in the cell).
+__global__ __launch_bounds__(512) void alloc_flags_kernel(BuildDev B, uint8_t *flags, uint32_t *counts) {
+    __shared__ uint32_t wsum[8];
+    uint32_t cand = blockIdx.x, t = threadIdx.x;
+    uint32_t cell = B.cand_cell[cand];
+    uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
+    uint32_t gbx = qx * 8u + (t & 7u), gby = qy * 8u + ((t >> 3) & 7u), gbz = qz * 8u + (t >> 6);
+    float bws = B.vs * 8.0f;
+    float hb = bws * 0.5f;
+    float cx = (B.origin[0] + (float)gbx * bws) + hb;
+    float cy = (B.origin[1] + (float)gby * bws) + hb;
+    float cz = (B.origin[2] + (float)gbz * bws) + hb;
+    float d = scene_field(B, cand, cx, cy, cz);
+    bool need = fabsf(d) < bws * 1.8f;
+    flags[(uint64_t)cand * 512u + t] = need ? 1 : 0;
+    uint64_t ball = __ballot(need);
+    if ((t & 63u) == 0) wsum[t >> 6] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < 8; w++) s += wsum[w];
+        counts[cand] = s;
+    }
+}
+
+// Assign slots (base + rank in z-y-x order) and write L1/L2 entries.
+__global__ __launch_bounds__(512) void assign_kernel(BuildDev B, const uint8_t *flags, const uint32_t *blk_of,
+                                                     const uint32_t *slot_base, uint32_t *l2, uint32_t *l1,
+                                                     uint32_t *slot_cand, uint32_t *slot_brick) {
+    __shared__ uint32_t wsum[8];
+    uint32_t cand = blockIdx.x, t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t blk = blk_of[cand];
+    if (blk == RKF_EMPTY_SLOT) return;  // uniform per block
+    bool need = flags[(uint64_t)cand * 512u + t] != 0;
+    uint64_t ball = __ballot(need);
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < w; i++) before += wsum[i];
+    uint32_t rank = before + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+    uint32_t slot = slot_base[cand] + rank;
+    l1[(uint64_t)blk * 512u + t] = need ? slot : RKF_EMPTY_SLOT;
+    if (need) {
+        slot_cand[slot] = cand;
+        slot_brick[slot] = t;
+    }
+    if (t == 0) l2[B.cand_cell[cand]] = blk;
+}
+
+// One block per allocated brick, one lane per voxel (populate_brick).
+__global__ __launch_bounds__(512) void fill_kernel(BuildDev B, const uint32_t *slot_cand, const uint32_t *slot_brick,
+                                                   uint16_t *dist, uint8_t *mat) {
+    uint32_t slot = blockIdx.x, v = threadIdx.x;
+    uint32_t cand = slot_cand[slot], lb = slot_brick[slot];
+    uint32_t cell = B.cand_cell[cand];
+    uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
+    uint32_t gbx = qx * 8u + (lb & 7u), gby = qy * 8u + ((lb >> 3) & 7u), gbz = qz * 8u + (lb >> 6);
+    uint32_t vx = v & 7u, vy = (v >> 3) & 7u, vz = v >> 6;
+    float vs = B.vs, bws = vs * 8.0f, half = vs * 0.5f;
+    float px = (B.origin[0] + (float)gbx * bws) + ((float)vx * vs + half);
+    float py = (B.origin[1] + (float)gby * bws) + ((float)vy * vs + half);
+    float pz = (B.origin[2] + (float)gbz * bws) + ((float)vz * vs + half);
+    float d = scene_field(B, cand, px, py, pz);
+    uint64_t idx = (uint64_t)slot * 512u + v;
+    dist[idx] = (uint16_t)f2h(d);
+    uint32_t m = 0;
+    if (B.mat_seed) {
+        uint64_t key = (uint64_t)(gbx * 8u + vx) | ((uint64_t)(gby * 8u + vy) << 21) | ((uint64_t)(gbz * 8u + vz) << 42);
+        m = (uint32_t)(mix64(B.mat_seed ^ mix64(key ^ 0ull)) & 15u);
+    }
+    mat[idx] = (uint8_t)m;
+}
+
+// Free masks: one block per row cell, one lane per brick b of the cell;
+// bit set = some brick of {b..b+1}^3 (clamped to the grid) is allocated.
+__global__ __launch_bounds__(512) void free_mask_kernel(SparseDev S, const uint32_t *row_cell, uint64_t *bits,
+                                                        uint32_t *row_any) {
+    __shared__ uint32_t any;
+    uint32_t row = blockIdx.x, t = threadIdx.x;
+    if (t == 0) any = 0;
+    __syncthreads();
+    uint32_t cell = row_cell[row];
+    uint32_t qx = cell % S.cells[0], qy = (cell / S.cells[0]) % S.cells[1], qz = cell / (S.cells[0] * S.cells[1]);
+    uint32_t bx = qx * 8u + (t & 7u), by = qy * 8u + ((t >> 3) & 7u), bz = qz * 8u + (t >> 6);
+    bool occupied = false;
+    for (uint32_t dz = 0; dz < 2; dz++)
+        for (uint32_t dy = 0; dy < 2; dy++)
+            for (uint32_t dx = 0; dx < 2; dx++) {
+                uint32_t x = min(bx + dx, S.nb[0] - 1), y = min(by + dy, S.nb[1] - 1), z = min(bz + dz, S.nb[2] - 1);
+                uint64_t c = (uint64_t)(x >> 3) + (uint64_t)(y >> 3) * S.cells[0] +
+                             (uint64_t)(z >> 3) * S.cells[0] * S.cells[1];
+                uint32_t blk = S.l2[c];
+                if (blk == RKF_EMPTY_SLOT) continue;
+                uint32_t s = S.l1[(uint64_t)blk * 512u + (x & 7u) + (y & 7u) * 8u + (z & 7u) * 64u];
+                if (s != RKF_EMPTY_SLOT) occupied = true;
+            }
+    uint64_t ball = __ballot(occupied);
+    if ((t & 63u) == 0) {
+        bits[(uint64_t)row * 8u + (t >> 6)] = ball;
+        if (ball) atomicOr(&any, 1u);
+    }
+    __syncthreads();
+    if (t == 0) row_any[row] = any;
+}
+
+}  // namespace rkf
+
+rkf::SparseDev rkfm_sparse_scene::dev() const {
+    rkf::SparseDev d;
+    memset(&d, 0, sizeof(d));
+    for (int a = 0; a < 3; a++) {
+        d.origin[a] = info.origin[a];
+        d.cells[a] = info.cells[a];
+        d.nb[a] = info.cells[a] * 8u;
+        d.tot[a] = (int32_t)(info.cells[a] * 64u);
+    }
+    d.vs = info.voxel_size;
+    d.l2 = (const uint32_t *)l2.ptr;
+    d.l1 = (const uint32_t *)l1.ptr;
+    d.dist = (const uint16_t *)dist.ptr;
+    d.mat = (const uint8_t *)mat.ptr;
+    d.aux = (const uint16_t *)aux.ptr;
+    d.free_idx = (const uint32_t *)free_idx.ptr;
+    d.free_bits = (const uint64_t *)free_bits.ptr;
+    d.n_cells = (uint64_t)info.cells[0] * info.cells[1] * info.cells[2];
+    return d;
+}
+
+static void release_scene(rkfm_sparse_scene *s) {
+    s->l2.release();
+    s->l1.release();
+    s->dist.release();
+    s->mat.release();
+    s->aux.release();
+    s->free_idx.release();
+    s->free_bits.release();
+}
+
+void rkf_sparse_release_all(rkfm_ctx *c) {
+    for (rkfm_sparse_scene *s : c->scenes) {
+        release_scene(s);
+        delete s;
+    }
+    c->scenes.clear();
+}
+
+namespace {
+
+struct TmpBufs {
+    std::vector<DevBuf *> bufs;
+    ~TmpBufs() {
+        for (DevBuf *b : bufs) {
+            b->release();
+            delete b;
+        }
+    }
+    DevBuf *make() {
+        bufs.push_back(new DevBuf());
+        return bufs.back();
+    }
+};
+
+int put(hipStream_t st, DevBuf &b, const void *src, uint64_t bytes) {
+    int rc = b.reserve(bytes ? bytes : 4, false);
+    if (rc) return rc;
+    b.size = bytes;
+    if (bytes) rc = rkf_hip_check(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st), "upload");
+    return rc;
+}
+
+uint64_t used_bytes(const rkfm_ctx *c) {
+    uint64_t s = 0;
+    for (const rkfm_sparse_scene *x : c->scenes) s += x->info.device_bytes;
+    return s;
+}
+
+// Builds free_idx / free_bits from l2/l1 already on the device.
+int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32_t> &block_cells) {
+    const uint32_t *cells = s->info.cells;
+    uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
+    // Cells whose {C..C+1}^3 neighbourhood holds a block: C in block - {0,1}^3.
+    std::vector<uint32_t> rows;
+    rows.reserve(block_cells.size() * 8);
+    for (uint32_t bc : block_cells) {
+        uint32_t qx = bc % cells[0], qy = (bc / cells[0]) % cells[1], qz = bc / (cells[0] * cells[1]);
+        for (uint32_t dz = 0; dz < 2; dz++)
+            for (uint32_t dy = 0; dy < 2; dy++)
+                for (uint32_t dx = 0; dx < 2; dx++) {
+                    if (qx < dx || qy < dy || qz < dz) continue;
+                    rows.push_back((qx - dx) + (qy - dy) * cells[0] + (qz - dz) * cells[0] * cells[1]);
+                }
+    }
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    TmpBufs tmp;
+    DevBuf *d_rows = tmp.make(), *d_any = tmp.make();
+    int rc = put(c->stream, *d_rows, rows.data(), rows.size() * 4);
+    if (!rc) rc = d_any->reserve(rows.size() * 4 + 4, false);
+    if (!rc) rc = s->free_bits.reserve(rows.size() * 64 + 64, false);
+    if (!rc) rc = s->free_idx.reserve(ncell * 4, false);
+    if (rc) return rc;
+    s->free_bits.size = rows.size() * 64;
+    s->free_idx.size = ncell * 4;
+    std::vector<uint32_t> any(rows.size());
+    if (!rows.empty()) {
+        rkf::SparseDev D = s->dev();
+        hipLaunchKernelGGL(rkf::free_mask_kernel, dim3((unsigned)rows.size()), dim3(512), 0, c->stream, D,
+                           (const uint32_t *)d_rows->ptr, (uint64_t *)s->free_bits.ptr, (uint32_t *)d_any->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "free_mask launch");
+        if (!rc)
+            rc = rkf_hip_check(hipMemcpyAsync(any.data(), d_any->ptr, rows.size() * 4, hipMemcpyDeviceToHost, c->stream),
+                               "free rows download");
+        if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "free rows sync");
+        if (rc) return rc;
+    }
+    std::vector<uint32_t> fidx(ncell, rkf::kFreeCell);
+    for (size_t r = 0; r < rows.size(); r++)
+        if (any[r]) fidx[rows[r]] = (uint32_t)r;
+    s->n_free_rows = rows.size();
+    rc = rkf_hip_check(hipMemcpyAsync(s->free_idx.ptr, fidx.data(), ncell * 4, hipMemcpyHostToDevice, c->stream),
+                       "free idx upload");
+    if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "free idx sync");
+    return rc;
+}
+
+void finish_info(rkfm_sparse_scene *s) {
+    s->info.device_bytes = s->l2.cap + s->l1.cap + s->dist.cap + s->mat.cap + s->aux.cap + s->free_idx.cap +
+                           s->free_bits.cap;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkfm_sparse_params *prm,
+                      rkfm_sparse_scene **out) {
+    if (!c || !prims || !prm || !out || n == 0) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: argument");
+    *out = nullptr;
+    const float vs = prm->voxel_size, k = prm->smin_k;
+    if (!(vs > 0.0f) || k < 0.0f) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: voxel size / k");
+    const float bws = vs * 8.0f, cws = bws * 8.0f;
+    uint32_t cells[3];
+    for (int a = 0; a < 3; a++) {
+        float ext = prm->bounds_max[a] - prm->bounds_min[a];
+        if (!(ext > 0.0f)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: empty bounds");
+        double cc = std::ceil((double)ext / (double)cws);
+        if (cc > 32768.0) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: grid too large");
+        cells[a] = std::max(1u, (uint32_t)cc);
+    }
+    uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
+    if (ncell > 0xFFFFFFF0ull) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: too many cells");
+    // Groups: ids compacted in ascending order, prims in original order.
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) {
+        if (prims[i].type > 2) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: prim %u type", i);
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return prims[a].group < prims[b].group; });
+    std::vector<uint32_t> gstart, gcount, gid_of(n);
+    for (uint32_t i = 0; i < n; i++) {
+        if (i == 0 || prims[order[i]].group != prims[order[i - 1]].group) {
+            gstart.push_back(i);
+            gcount.push_back(0);
+        }
+        gcount.back()++;
+        gid_of[order[i]] = (uint32_t)gstart.size() - 1;
+    }
+    // Bin groups into cells.
+    const float M = 3.0f * bws + 2.0f * k;
+    std::vector<uint64_t> pairs;  // (cell << 32) | group
+    for (uint32_t i = 0; i < n; i++) {
+        const rkfm_prim &p = prims[i];
+        float R = p.type == 0 ? p.params[0]
+                  : p.type == 1 ? std::sqrt(p.params[0] * p.params[0] + p.params[1] * p.params[1] + p.params[2] * p.params[2])
+                                : p.params[0] + p.params[1];
+        double E = (double)R + M + 1e-3;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)std::floor(((double)p.centre[a] - E - prm->bounds_min[a]) / cws);
+            hi[a] = (int64_t)std::floor(((double)p.centre[a] + E - prm->bounds_min[a]) / cws);
+            lo[a] = std::max<int64_t>(lo[a], 0);
+            hi[a] = std::min<int64_t>(hi[a], (int64_t)cells[a] - 1);
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    double d2 = 0;
+                    int64_t q[3] = {x, y, z};
+                    for (int a = 0; a < 3; a++) {
+                        double c0 = prm->bounds_min[a] + (double)q[a] * cws, c1 = c0 + cws;
+                        double v = std::max(std::max(c0 - p.centre[a], (double)p.centre[a] - c1), 0.0);
+                        d2 += v * v;
+                    }
+                    if (d2 > E * E) continue;
+                    uint64_t cell = (uint64_t)x + (uint64_t)y * cells[0] + (uint64_t)z * cells[0] * cells[1];
+                    pairs.push_back((cell << 32) | gid_of[i]);
+                }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    std::vector<uint32_t> cand_cell, cand_off, cand_groups;
+    for (size_t i = 0; i < pairs.size(); i++) {
+        uint32_t cell = (uint32_t)(pairs[i] >> 32);
+        if (cand_cell.empty() || cand_cell.back() != cell) {
+            cand_cell.push_back(cell);
+            cand_off.push_back((uint32_t)cand_groups.size());
+        }
+        cand_groups.push_back((uint32_t)(pairs[i] & 0xFFFFFFFFu));
+    }
+    cand_off.push_back((uint32_t)cand_groups.size());
+    const uint32_t ncand = (uint32_t)cand_cell.size();
+
+    rkfm_sparse_scene *s = new rkfm_sparse_scene();
+    s->owner = c;
+    for (int a = 0; a < 3; a++) {
+        s->info.origin[a] = prm->bounds_min[a];
+        s->info.cells[a] = cells[a];
+    }
+    s->info.voxel_size = vs;
+    TmpBufs tmp;
+    DevBuf *d_prims = tmp.make(), *d_gp = tmp.make(), *d_gs = tmp.make(), *d_gc = tmp.make();
+    DevBuf *d_cc = tmp.make(), *d_co = tmp.make(), *d_cg = tmp.make(), *d_flags = tmp.make(), *d_cnt = tmp.make();
+    DevBuf *d_blk = tmp.make(), *d_base = tmp.make(), *d_scand = tmp.make(), *d_sbrick = tmp.make();
+    auto fail = [&](int rc) {
+        release_scene(s);
+        delete s;
+        return rc;
+    };
+    int rc = put(c->stream, *d_prims, prims, (uint64_t)n * sizeof(rkfm_prim));
+    if (!rc) rc = put(c->stream, *d_gp, order.data(), (uint64_t)n * 4);
+    if (!rc) rc = put(c->stream, *d_gs, gstart.data(), gstart.size() * 4);
+    if (!rc) rc = put(c->stream, *d_gc, gcount.data(), gcount.size() * 4);
+    if (!rc) rc = put(c->stream, *d_cc, cand_cell.data(), cand_cell.size() * 4);
+    if (!rc) rc = put(c->stream, *d_co, cand_off.data(), cand_off.size() * 4);
+    if (!rc) rc = put(c->stream, *d_cg, cand_groups.data(), cand_groups.size() * 4);
+    if (!rc) rc = d_flags->reserve((uint64_t)ncand * 512 + 4, false);
+    if (!rc) rc = d_cnt->reserve((uint64_t)ncand * 4 + 4, false);
+    if (rc) return fail(rc);
+    rkf::BuildDev B;
+    memset(&B, 0, sizeof(B));
+    B.prims = (const rkfm_prim *)d_prims->ptr;
+    B.group_prims = (const uint32_t *)d_gp->ptr;
+    B.group_start = (const uint32_t *)d_gs->ptr;
+    B.group_count = (const uint32_t *)d_gc->ptr;
+    B.cand_cell = (const uint32_t *)d_cc->ptr;
+    B.cand_off = (const uint32_t *)d_co->ptr;
+    B.cand_groups = (const uint32_t *)d_cg->ptr;
+    for (int a = 0; a < 3; a++) {
+        B.origin[a] = prm->bounds_min[a];
+        B.cells[a] = cells[a];
+    }
+    B.vs = vs;
+    B.k = k;
+    B.mat_seed = prm->mat_seed;
+    std::vector<uint32_t> counts(ncand);
+    if (ncand) {
+        hipLaunchKernelGGL(rkf::alloc_flags_kernel, dim3(ncand), dim3(512), 0, c->stream, B, (uint8_t *)d_flags->ptr,
+                           (uint32_t *)d_cnt->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "alloc_flags launch");
+        if (!rc)
+            rc = rkf_hip_check(hipMemcpyAsync(counts.data(), d_cnt->ptr, (uint64_t)ncand * 4, hipMemcpyDeviceToHost,
+                                              c->stream),
+                               "counts download");
+        if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "counts sync");
+        if (rc) return fail(rc);
+    }
+    std::vector<uint32_t> blk_of(ncand, RKF_EMPTY_SLOT), base(ncand, 0), block_cells;
+    uint64_t nbricks = 0;
+    uint32_t nblocks = 0;
+    for (uint32_t i = 0; i < ncand; i++) {
+        if (!counts[i]) continue;
+        blk_of[i] = nblocks++;
+        block_cells.push_back(cand_cell[i]);
+        base[i] = (uint32_t)nbricks;
+        nbricks += counts[i];
+    }
+    if (nbricks >= 0xFFFFFFF0ull) return fail(rkf_set_error(RKFM_ERR_OOM, "sparse_build: %llu bricks exceed u32 slots",
+                                                            (unsigned long long)nbricks));
+    if (prm->max_bricks && nbricks > prm->max_bricks)
+        return fail(rkf_set_error(RKFM_ERR_OOM, "sparse_build: %llu bricks > cap %llu", (unsigned long long)nbricks,
+                                  (unsigned long long)prm->max_bricks));
+    uint64_t need = ncell * 8 + (uint64_t)nblocks * 2048 + nbricks * 1536;
+    if (c->memory_budget && used_bytes(c) + need > c->memory_budget)
+        return fail(rkf_set_error(RKFM_ERR_OOM, "sparse_build: %llu bytes over the context budget",
+                                  (unsigned long long)need));
+    rc = s->l2.reserve(ncell * 4, false);
+    if (!rc) rc = s->l1.reserve((uint64_t)nblocks * 2048 + 4, false);
+    if (!rc) rc = s->dist.reserve(nbricks * 1024 + 4, false);
+    if (!rc) rc = s->mat.reserve(nbricks * 512 + 4, false);
+    if (!rc) rc = put(c->stream, *d_blk, blk_of.data(), (uint64_t)ncand * 4);
+    if (!rc) rc = put(c->stream, *d_base, base.data(), (uint64_t)ncand * 4);
+    if (!rc) rc = d_scand->reserve(nbricks * 4 + 4, false);
+    if (!rc) rc = d_sbrick->reserve(nbricks * 4 + 4, false);
+    if (rc) return fail(rc);
+    s->l2.size = ncell * 4;
+    s->l1.size = (uint64_t)nblocks * 2048;
+    s->dist.size = nbricks * 1024;
+    s->mat.size = nbricks * 512;
+    rc = rkf_hip_check(hipMemsetAsync(s->l2.ptr, 0xFF, ncell * 4, c->stream), "l2 clear");
+    if (rc) return fail(rc);
+    if (ncand) {
+        hipLaunchKernelGGL(rkf::assign_kernel, dim3(ncand), dim3(512), 0, c->stream, B, (const uint8_t *)d_flags->ptr,
+                           (const uint32_t *)d_blk->ptr, (const uint32_t *)d_base->ptr, (uint32_t *)s->l2.ptr,
+                           (uint32_t *)s->l1.ptr, (uint32_t *)d_scand->ptr, (uint32_t *)d_sbrick->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "assign launch");
+        if (rc) return fail(rc);
+    }
+    if (nbricks) {
+        hipLaunchKernelGGL(rkf::fill_kernel, dim3((unsigned)nbricks), dim3(512), 0, c->stream, B,
+                           (const uint32_t *)d_scand->ptr, (const uint32_t *)d_sbrick->ptr, (uint16_t *)s->dist.ptr,
+                           (uint8_t *)s->mat.ptr);
+        rc = rkf_hip_check(hipGetLastError(), "fill launch");
+        if (rc) return fail(rc);
+    }
+    rc = rkf_hip_check(hipStreamSynchronize(c->stream), "sparse_build sync");
+    if (rc) return fail(rc);
+    s->info.n_blocks = nblocks;
+    s->info.n_bricks = nbricks;
+    rc = build_free_masks(c, s, block_cells);
+    if (rc) return fail(rc);
+    finish_info(s);
+    c->scenes.push_back(s);
+    *out = s;
+    return RKFM_OK;
+}
+
+int rkfm_sparse_from_dense(rkfm_ctx *c, const uint32_t *map, const uint32_t dims[3], const rkf_voxel_sample *bricks,
+                           uint64_t n_bricks, float vs, const float origin[3], rkfm_sparse_scene **out) {
+    if (!c || !map || !dims || !origin || !out || (!bricks && n_bricks))
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_from_dense: null argument");
+    *out = nullptr;
+    if (!(vs > 0.0f) || !dims[0] || !dims[1] || !dims[2])
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_from_dense: voxel size / dims");
+    uint32_t cells[3] = {(dims[0] + 7u) / 8u, (dims[1] + 7u) / 8u, (dims[2] + 7u) / 8u};
+    uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
+    std::vector<uint32_t> l2(ncell, RKF_EMPTY_SLOT), l1, block_cells;
+    for (uint32_t bz = 0; bz < dims[2]; bz++)
+        for (uint32_t by = 0; by < dims[1]; by++)
+            for (uint32_t bx = 0; bx < dims[0]; bx++) {
+                uint32_t e = map[bx + (uint64_t)by * dims[0] + (uint64_t)bz * dims[0] * dims[1]];
+                if (e == RKF_EMPTY_SLOT) continue;
+                if (e != RKF_INTERIOR_SLOT && e >= n_bricks)
+                    return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "sparse_from_dense: slot %u >= %llu bricks", e,
+                                         (unsigned long long)n_bricks);
+                uint64_t cell = (bx >> 3) + (uint64_t)(by >> 3) * cells[0] + (uint64_t)(bz >> 3) * cells[0] * cells[1];
+                if (l2[cell] == RKF_EMPTY_SLOT) {
+                    l2[cell] = (uint32_t)(l1.size() / 512);
+                    l1.resize(l1.size() + 512, RKF_EMPTY_SLOT);
+                    block_cells.push_back((uint32_t)cell);
+                }
+                l1[(uint64_t)l2[cell] * 512 + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u] = e;
+            }
+    std::vector<uint16_t> dist(n_bricks * 512);
+    std::vector<uint8_t> mat(n_bricks * 512);
+    std::vector<uint16_t> aux(n_bricks * 512);
+    bool has_aux = false;
+    for (uint64_t i = 0; i < n_bricks * 512; i++) {
+        uint32_t w0 = bricks[i].word0, w1 = bricks[i].word1;
+        dist[i] = (uint16_t)(w0 & 0xFFFFu);
+        mat[i] = (uint8_t)((w0 >> 16) & 0x3Fu);
+        aux[i] = (uint16_t)(((w0 >> 22) & 0x3Fu) | ((w1 >> 24) << 8));
+        has_aux |= aux[i] != 0;
+    }
+    rkfm_sparse_scene *s = new rkfm_sparse_scene();
+    s->owner = c;
+    for (int a = 0; a < 3; a++) {
+        s->info.origin[a] = origin[a];
+        s->info.cells[a] = cells[a];
+    }
+    s->info.voxel_size = vs;
+    s->info.n_blocks = (uint32_t)(l1.size() / 512);
+    s->info.n_bricks = n_bricks;
+    int rc = put(c->stream, s->l2, l2.data(), ncell * 4);
+    if (!rc) rc = put(c->stream, s->l1, l1.data(), l1.size() * 4);
+    if (!rc) rc = put(c->stream, s->dist, dist.data(), dist.size() * 2);
+    if (!rc) rc = put(c->stream, s->mat, mat.data(), mat.size());
+    if (!rc && has_aux) rc = put(c->stream, s->aux, aux.data(), aux.size() * 2);
+    if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "from_dense sync");
+    if (!rc) rc = build_free_masks(c, s, block_cells);
+    if (rc) {
+        release_scene(s);
+        delete s;
+        return rc;
+    }
+    finish_info(s);
+    c->scenes.push_back(s);
+    *out = s;
+    return RKFM_OK;
+}
+
+int rkfm_sparse_info_get(const rkfm_sparse_scene *s, rkfm_sparse_info *info) {
+    if (!s || !info) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_info: null argument");
+    *info = s->info;
+    return RKFM_OK;
+}
+
+int rkfm_sparse_destroy(rkfm_ctx *c, rkfm_sparse_scene *s) {
+    if (!c || !s) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_destroy: null argument");
+    auto it = std::find(c->scenes.begin(), c->scenes.end(), s);
+    if (it == c->scenes.end()) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_destroy: scene not owned by ctx");
+    (void)hipStreamSynchronize(c->stream);
+    c->scenes.erase(it);
+    release_scene(s);
+    delete s;
+    return RKFM_OK;
+}
+
+int rkfm_sparse_download(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *l2, uint32_t *l1, uint16_t *dist,
+                         uint8_t *mat) {
+    if (!c || !s) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_download: null argument");
+    int rc = rkf_hip_check(hipStreamSynchronize(c->stream), "download sync");
+    if (!rc && l2) rc = rkf_hip_check(hipMemcpy(l2, s->l2.ptr, s->l2.size, hipMemcpyDeviceToHost), "download l2");
+    if (!rc && l1 && s->l1.size) rc = rkf_hip_check(hipMemcpy(l1, s->l1.ptr, s->l1.size, hipMemcpyDeviceToHost), "download l1");
+    if (!rc && dist && s->dist.size)
+        rc = rkf_hip_check(hipMemcpy(dist, s->dist.ptr, s->dist.size, hipMemcpyDeviceToHost), "download dist");
+    if (!rc && mat && s->mat.size) rc = rkf_hip_check(hipMemcpy(mat, s->mat.ptr, s->mat.size, hipMemcpyDeviceToHost), "download mat");
+    return rc;
+}
+
+int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *free_idx, uint64_t *free_bits,
+                              uint64_t *n_rows) {
+    if (!c || !s || !n_rows) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_download_free: null argument");
+    *n_rows = s->n_free_rows;
+    int rc = rkf_hip_check(hipStreamSynchronize(c->stream), "download sync");
+    if (!rc && free_idx)
+        rc = rkf_hip_check(hipMemcpy(free_idx, s->free_idx.ptr, s->free_idx.size, hipMemcpyDeviceToHost), "free idx");
+    if (!rc && free_bits && s->free_bits.size)
+        rc = rkf_hip_check(hipMemcpy(free_bits, s->free_bits.ptr, s->free_bits.size, hipMemcpyDeviceToHost), "free bits");
+    return rc;
+}
+
+int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                      const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, const rkfm_gbuffer *out,
+                      uint32_t variant, void *stream) {
+    if (!c || !s || !cam || !su) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: null argument");
+    if (variant > 1) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: variant %u", variant);
+    if (ext && ext->hit_eps_mode > RKFM_EPS_RELATIVE)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: hit_eps_mode %u", ext->hit_eps_mode);
+    rkf::SparseMarchParams P;
+    memset(&P, 0, sizeof(P));
+    int rc = rkf_fill_gbuf(out, P.g);
+    if (rc) return rc;
+    P.S = s->dev();
+    P.cam = *cam;
+    P.su = *su;
+    if (ext) P.ext = *ext;
+    P.object_id = 0;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    return rkf_hip_check(rkf_launch_sparse_march(&P, variant, st), "sparse_march launch");
+}
+
+int rkfm_sparse_query_points(rkfm_ctx *c, const rkfm_sparse_scene *s, const float *xyz, uint32_t n,
+                             rkfm_point_result *out, void *stream) {
+    if (!c || !s || (n && (!xyz || !out))) return rkf_set_error(RKFM_ERR_INVALID_ARG, "query_points: null argument");
+    rkf::SparseDev D = s->dev();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    return rkf_hip_check(rkf_launch_sparse_query(&D, xyz, n, out, st), "query launch");
+}
+
+}  // extern "C"

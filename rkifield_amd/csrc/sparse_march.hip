@@ -1,0 +1,352 @@
+// sparse_march.hip — marching resident sparse scenes (two-level index + SoA pool).
+//
+// Field: the reference's sample_voxelized (ray_march.wgsl:353-413) for one
+// world-space object whose dense brick map is the expansion of the two-level
+// index: 8 corner voxels, EMPTY -> +8 vs, INTERIOR -> -2 vs (stepping) or
+// -8 vs (gradient, :300-315), trilinear mix x->y->z, plus the outside-grid
+// distance. Step rules: ray_march_tiled (:905-982): near-step budget
+// (d < 0.02), MIN_STEP, 8-step bisection on overstep, hit threshold
+// (absolute or relative). Normals: 6-tap central difference at 2 vs
+// (:504-513, :1003-1020). G-buffer: :1083-1111. Shadows: soft_shadow
+// (shade_common_shading.wgsl:9-33) from shade_pbr.wgsl:40's origin.
+//
+// Multi-level free-space DDA (no reference counterpart): with
+// g = (p - origin)/vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
+// in bricks {b0, b0+1}^3. If all of them are EMPTY the field is the constant
+// 8 vs over the region floor(g/8) == b0 (or, when the whole L2 cell is free,
+// over floor(g/64) == cell); the ray jumps to the region's exit t_x, takes
+// prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip counts
+// as one loop iteration.
+#include "kernels.h"
+
+namespace rkf {
+
+struct SpCtx {
+    const SparseDev *S;
+    uint32_t evals;
+};
+
+RKF_DEV uint32_t sp_slot(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz) {
+    uint32_t bx = (uint32_t)cx >> 3, by = (uint32_t)cy >> 3, bz = (uint32_t)cz >> 3;
+    uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
+                    (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
+    uint32_t blk = S.l2[cell];
+    if (blk == RKF_EMPTY_SLOT) return RKF_EMPTY_SLOT;
+    return S.l1[(uint64_t)blk * 512u + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u];
+}
+
+RKF_DEV float sp_voxel(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz, float vs, bool grad) {
+    cx = clampi(cx, 0, S.tot[0] - 1);
+    cy = clampi(cy, 0, S.tot[1] - 1);
+    cz = clampi(cz, 0, S.tot[2] - 1);
+    uint32_t slot = sp_slot(S, cx, cy, cz);
+    if (slot == RKF_EMPTY_SLOT) return vs * 8.0f;
+    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * 8.0f) : -(vs * 2.0f);
+    uint64_t idx = (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
+    return h2f(S.dist[idx]);
+}
+
+RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count) {
+    const SparseDev &S = *c.S;
+    float vs = S.vs;
+    float be = vs * 8.0f;
+    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
+    f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
+    float lo = vs * 0.01f;
+    f3 cl = mk(clampf(gp.x, lo, gs.x - lo), clampf(gp.y, lo, gs.y - lo), clampf(gp.z, lo, gs.z - lo));
+    float outside = length(sub(gp, cl));
+    if (outside > be * 2.0f) return outside;
+    f3 vc = mk(cl.x / vs - 0.5f, cl.y / vs - 0.5f, cl.z / vs - 0.5f);
+    int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
+    float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
+    if (count) c.evals++;
+    float c000 = sp_voxel(S, x0, y0, z0, vs, grad), c100 = sp_voxel(S, x0 + 1, y0, z0, vs, grad);
+    float c010 = sp_voxel(S, x0, y0 + 1, z0, vs, grad), c110 = sp_voxel(S, x0 + 1, y0 + 1, z0, vs, grad);
+    float c001 = sp_voxel(S, x0, y0, z0 + 1, vs, grad), c101 = sp_voxel(S, x0 + 1, y0, z0 + 1, vs, grad);
+    float c011 = sp_voxel(S, x0, y0 + 1, z0 + 1, vs, grad), c111 = sp_voxel(S, x0 + 1, y0 + 1, z0 + 1, vs, grad);
+    float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
+    float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
+    float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
+    return mix(c0, c1, tz) + outside;
+}
+
+// Nearest voxel floor(grid_pos / vs) as ray_march.wgsl:540-596. Returns the
+// pool index or ~0.
+RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o) {
+    float vs = S.vs;
+    float be = vs * 8.0f;
+    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
+    f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
+    slot_o = RKF_EMPTY_SLOT;
+    vox_o = 0xFFFFFFFFu;
+    cell_o = 0xFFFFFFFFu;
+    if (gp.x < 0.0f || gp.y < 0.0f || gp.z < 0.0f || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) return ~0ull;
+    int32_t cx = clampi((int32_t)floorf(gp.x / vs), 0, S.tot[0] - 1);
+    int32_t cy = clampi((int32_t)floorf(gp.y / vs), 0, S.tot[1] - 1);
+    int32_t cz = clampi((int32_t)floorf(gp.z / vs), 0, S.tot[2] - 1);
+    uint32_t slot = sp_slot(S, cx, cy, cz);
+    cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * S.cells[0] + ((uint32_t)cz >> 6) * S.cells[0] * S.cells[1];
+    slot_o = slot;
+    uint32_t lv = (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
+    if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) return ~0ull;
+    vox_o = lv;
+    return (uint64_t)slot * 512u + lv;
+}
+
+RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
+    uint32_t a, b, d;
+    uint64_t i = sp_nearest(S, p, a, b, d);
+    return i == ~0ull ? 0u : (uint32_t)(S.mat[i] & 0x3Fu);
+}
+
+// Free-space test + exit parameter (see file header).
+RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, float t, float &t_exit) {
+    float vs = S.vs;
+    float be = vs * 8.0f;
+    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
+    f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
+    float lo = vs * 0.01f;
+    if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
+    float g[3] = {gp.x / vs - 0.5f, gp.y / vs - 0.5f, gp.z / vs - 0.5f};
+    int32_t v0[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        v0[a] = (int32_t)floorf(g[a]);
+        if (v0[a] < 0 || v0[a] > S.tot[a] - 2) return false;
+    }
+    uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
+    uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
+    uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
+    uint32_t row = S.free_idx[cell];
+    float lo_b[3], hi_b[3];
+    if (row == kFreeCell) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = (float)(q[a] * 64u);
+            hi_b[a] = (float)(q[a] * 64u + 64u);
+        }
+    } else {
+        uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
+        uint64_t w = S.free_bits[(uint64_t)row * 8u + (lb >> 6)];
+        if ((w >> (lb & 63u)) & 1ull) return false;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = (float)(b[a] * 8u);
+            hi_b[a] = (float)(b[a] * 8u + 8u);
+        }
+    }
+    float dv[3] = {sd.x, sd.y, sd.z};
+    float dt = kMaxFloat;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float bound = dv[a] > 0.0f ? hi_b[a] : lo_b[a];
+        float da = (bound - g[a]) * vs / dv[a];
+        dt = fminf(dt, da);
+    }
+    t_exit = t + fmaxf(dt, 0.0f);
+    return true;
+}
+
+struct SpRes {
+    float t, mat_d;
+    f3 mat_p;
+    bool hit;
+    uint32_t steps;
+};
+
+RKF_DEV void sp_march(SpCtx &c, f3 o, f3 dir, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &r) {
+    const SparseDev &S = *c.S;
+    float vs = S.vs;
+    r.hit = false;
+    r.t = kMaxFloat;
+    r.steps = 0;
+    r.mat_d = 0.0f;
+    r.mat_p = mk(0.0f, 0.0f, 0.0f);
+    f3 sd = safe_dir(dir);
+    float t = 0.0f, prev_t = 0.0f, prev_dist = kMaxFloat;
+    const float nudge = vs * 0.00390625f;
+    uint32_t near = 0;
+    for (uint32_t step = 0; step < kHardMax; step++) {
+        if (t > su.max_distance || near >= su.max_steps) break;
+        r.steps++;
+        f3 pos = add(o, muls(sd, t));
+        float tx;
+        if (sp_free(S, pos, sd, t, tx)) {
+            prev_t = tx;
+            prev_dist = vs * 8.0f;
+            t = tx + nudge;
+            continue;
+        }
+        float d = sp_sample(c, pos, false, true);
+        if (d < 0.0f && prev_dist > hit_eps(eps_mode, su.hit_threshold, prev_t)) {
+            float lo = prev_t, hi = t, mt = t, md = d;
+            for (uint32_t b = 0; b < kBisect; b++) {
+                float mid = (lo + hi) * 0.5f;
+                float mdd = sp_sample(c, add(o, muls(sd, mid)), false, true);
+                if (mdd < 0.0f) {
+                    hi = mid;
+                } else {
+                    lo = mid;
+                    mt = mid;
+                    md = mdd;
+                }
+            }
+            r.t = (lo + hi) * 0.5f;
+            r.hit = true;
+            r.mat_p = add(o, muls(sd, mt));
+            r.mat_d = md;
+            return;
+        }
+        if (d < hit_eps(eps_mode, su.hit_threshold, t)) {
+            r.t = t;
+            r.hit = true;
+            r.mat_p = pos;
+            r.mat_d = d;
+            return;
+        }
+        if (d < kNear) near++;
+        prev_t = t;
+        prev_dist = d;
+        t += fmaxf(d, kMinStep);
+    }
+}
+
+RKF_DEV float sp_shadow(SpCtx &c, f3 o, f3 L, float maxd, float k, uint32_t &steps) {
+    const float nudge = c.S->vs * 0.00390625f;
+    float shadow = 1.0f, t = 0.005f;
+    uint32_t near = 0;
+    steps = 0;
+    for (uint32_t i = 0; i < 256u; i++) {
+        if (near >= 32u || t > maxd) break;
+        steps++;
+        f3 p = add(o, muls(L, t));
+        float tx;
+        if (sp_free(*c.S, p, L, t, tx)) {
+            t = tx + nudge;
+            continue;
+        }
+        float d = sp_sample(c, p, false, true);
+        if (d < 0.005f) return 0.0f;
+        if (d < 0.02f) {
+            shadow = fminf(shadow, k * d / t);
+            near++;
+        }
+        t += fmaxf(d, 0.005f);
+    }
+    return clampf(shadow, 0.0f, 1.0f);
+}
+
+// Per-pixel work shared by the plain and tuned kernels.
+RKF_DEV void sp_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
+    const uint32_t W = P.g.width, H = P.g.height;
+    uint64_t p = (uint64_t)py * W + px;
+    SpCtx c{&P.S, 0u};
+    f3 o, dir;
+    camera_ray(P.cam, W, H, px, py, o, dir);
+    SpRes r;
+    sp_march(c, o, dir, P.su, P.ext.hit_eps_mode, r);
+    if (!r.hit) {
+        store_miss(P.g, p);
+        store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
+        if (P.g.shadow) P.g.shadow[p] = 0.0f;
+        return;
+    }
+    const SparseDev &S = P.S;
+    uint32_t evals_march = c.evals;
+    f3 hp = add(o, muls(dir, r.t));
+    uint32_t mat = 0;
+    if (r.mat_d < S.vs * 16.0f) mat = sp_material(S, r.mat_p);
+    float e = S.vs * 2.0f;
+    float gx = sp_sample(c, add(hp, mk(e, 0.0f, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(e, 0.0f, 0.0f)), true, false);
+    float gy = sp_sample(c, add(hp, mk(0.0f, e, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(0.0f, e, 0.0f)), true, false);
+    float gz = sp_sample(c, add(hp, mk(0.0f, 0.0f, e)), true, false) - sp_sample(c, sub(hp, mk(0.0f, 0.0f, e)), true, false);
+    f3 gv = mk(gx, gy, gz);
+    float len = length(gv);
+    f3 n = len < 1e-10f ? mk(0.0f, 1.0f, 0.0f) : mk(gv.x / len, gv.y / len, gv.z / len);
+    uint32_t slot, vox, cell;
+    uint64_t vi = sp_nearest(S, hp, slot, vox, cell);
+    uint32_t sec = 0, blend = 0;
+    if (S.aux && vi != ~0ull) {
+        uint32_t a = S.aux[vi];
+        sec = a & 0x3Fu;
+        blend = (a >> 8) & 0xFFu;
+    }
+    float sdf_at = sp_sample(c, hp, false, false);
+    uint32_t packed = (mat & 0x3Fu) | (sec << 6) | (blend << 12) | (P.object_id << 24);
+    float mx, my;
+    motion_vec(P.cam, W, H, px, py, hp, mx, my);
+    store_hit(P.g, p, hp, r.t, n, packed, mx, my, sdf_at);
+    uint32_t sh_steps = 0, sh_evals = 0;
+    if (P.g.shadow) {
+        float vis = 0.0f;
+        if (P.ext.shadow_enable) {
+            // The shading pass reads the normal back from the rgba16f target.
+            f3 nq = mk(h2f(f2h(n.x)), h2f(f2h(n.y)), h2f(f2h(n.z)));
+            f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
+            const float bias = 0.08f;
+            f3 so = add(add(hp, muls(nq, bias)), muls(L, bias * 0.5f));
+            float maxd = P.ext.shadow_max_dist > 0.0f ? P.ext.shadow_max_dist : 25.0f;
+            float k = P.ext.shadow_k > 0.0f ? P.ext.shadow_k : 16.0f;
+            SpCtx cs{&P.S, 0u};
+            vis = sp_shadow(cs, so, L, maxd, k, sh_steps);
+            sh_evals = cs.evals;
+        }
+        P.g.shadow[p] = vis;
+    }
+    store_debug(P.g, p, n, slot, vox, cell, r.steps, evals_march, sh_steps, sh_evals);
+}
+
+// Plain variant: one lane per pixel, 8x8 pixels per wavefront.
+__global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= P.g.width || py >= P.g.height) return;
+    sp_pixel(P, px, py);
+}
+
+// Point queries (sdf_collision.rs:38-56): distance, nearest material,
+// inside flag, central-difference gradient at eps 0.01 with
+// normalize_or_zero (v * (1/len) when that reciprocal is finite and > 0).
+__global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const float *xyz, uint32_t n,
+                                                           rkfm_point_result *out) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    SpCtx c{&S, 0u};
+    f3 p = mk(xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]);
+    float d = sp_sample(c, p, false, false);
+    const float e = 0.01f;
+    float dx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), false, false) - sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), false, false);
+    float dy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), false, false) - sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), false, false);
+    float dz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), false, false) - sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), false, false);
+    float len = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    float rcp = 1.0f / len;
+    rkfm_point_result r;
+    r.distance = d;
+    r.material = sp_material(S, p);
+    r.inside = d < 0.0f ? 1u : 0u;
+    if (isfinite(rcp) && rcp > 0.0f) {
+        r.gradient[0] = dx * rcp;
+        r.gradient[1] = dy * rcp;
+        r.gradient[2] = dz * rcp;
+    } else {
+        r.gradient[0] = r.gradient[1] = r.gradient[2] = 0.0f;
+    }
+    out[i] = r;
+}
+
+}  // namespace rkf
+
+extern "C" hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, hipStream_t stream) {
+    (void)variant;
+    dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
+    hipLaunchKernelGGL(rkf::sparse_march_plain, grid, dim3(64, 1, 1), 0, stream, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n,
+                                              rkfm_point_result *out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rkf::sparse_query_kernel, dim3((n + 255u) / 256u, 1, 1), dim3(256, 1, 1), 0, stream, *S, xyz,
+                       n, out);
+    return hipGetLastError();
+}

@@ -1,0 +1,170 @@
+"""numpy views of the reference's byte layouts (mirrors include/rkf_march.h).
+
+Each dtype is byte-identical to the Rust ``#[repr(C)]`` struct it names:
+
+* ``VOXEL_SAMPLE``   crates/rkf-core/src/voxel.rs:15-20            (8 B)
+* ``GPU_OBJECT``     crates/rkf-render/src/gpu_object.rs:41-95     (256 B)
+* ``CAMERA_UNIFORMS`` crates/rkf-render/src/camera.rs:194-213      (144 B)
+* ``SCENE_UNIFORMS`` crates/rkf-render/src/gpu_scene.rs:21-43      (16 B)
+* ``COARSE_FIELD_UNIFORMS`` crates/rkf-render/src/coarse_field.rs:31-42 (48 B)
+* ``BVH_NODE``       crates/rkf-render/src/gpu_bvh.rs:24-41        (32 B)
+"""
+import numpy as np
+
+EMPTY_SLOT = 0xFFFFFFFF
+INTERIOR_SLOT = 0xFFFFFFFE
+BVH_INVALID = 0xFFFFFFFF
+BRICK_DIM = 8
+TILE_SIZE = 16
+TILE_MAX_OBJECTS = 32
+SDF_TYPE_NONE, SDF_TYPE_ANALYTICAL, SDF_TYPE_VOXELIZED = 0, 1, 2
+PRIM_SPHERE, PRIM_BOX, PRIM_CAPSULE, PRIM_TORUS, PRIM_CYLINDER, PRIM_PLANE = range(6)
+EPS_ABSOLUTE, EPS_RELATIVE = 0, 1
+STRATEGY_AUTO, STRATEGY_TILED, STRATEGY_BVH, STRATEGY_BRUTE = 0, 1, 2, 3
+MAX_FLOAT = np.float32(3.402823e38)
+
+VOXEL_SAMPLE = np.dtype([("word0", "<u4"), ("word1", "<u4")])
+
+GPU_OBJECT = np.dtype([
+    ("inverse_world", "<f4", (16,)),
+    ("aabb_min", "<f4", (4,)),
+    ("aabb_max", "<f4", (4,)),
+    ("brick_map_offset", "<u4"),
+    ("brick_map_dims", "<u4", (3,)),
+    ("voxel_size", "<f4"),
+    ("material_id", "<u4"),
+    ("sdf_type", "<u4"),
+    ("blend_mode", "<u4"),
+    ("blend_radius", "<f4"),
+    ("sdf_params", "<f4", (4,)),
+    ("accumulated_scale", "<f4", (3,)),
+    ("lod_level", "<u4"),
+    ("object_id", "<u4"),
+    ("primitive_type", "<u4"),
+    ("geometry_aabb_min", "<f4", (3,)),
+    ("geometry_aabb_max", "<f4", (3,)),
+    ("_padding", "<f4", (15,)),
+])
+
+CAMERA_UNIFORMS = np.dtype([
+    ("position", "<f4", (4,)),
+    ("forward", "<f4", (4,)),
+    ("right", "<f4", (4,)),
+    ("up", "<f4", (4,)),
+    ("resolution", "<f4", (2,)),
+    ("jitter", "<f4", (2,)),
+    ("prev_vp", "<f4", (16,)),
+])
+
+SCENE_UNIFORMS = np.dtype([
+    ("num_objects", "<u4"),
+    ("max_steps", "<u4"),
+    ("max_distance", "<f4"),
+    ("hit_threshold", "<f4"),
+])
+
+COARSE_FIELD_UNIFORMS = np.dtype([
+    ("origin_cam_rel", "<f4", (4,)),
+    ("dims", "<u4", (4,)),
+    ("voxel_size", "<f4"),
+    ("inv_voxel_size", "<f4"),
+    ("_pad", "<f4", (2,)),
+])
+
+BVH_NODE = np.dtype([
+    ("aabb_min", "<f4", (3,)),
+    ("left", "<u4"),
+    ("aabb_max", "<f4", (3,)),
+    ("right_or_object", "<u4"),
+])
+
+SCENE_EXT = np.dtype([
+    ("hit_eps_mode", "<u4"),
+    ("strategy", "<u4"),
+    ("shadow_enable", "<u4"),
+    ("_reserved0", "<u4"),
+    ("light_dir", "<f4", (3,)),
+    ("shadow_max_dist", "<f4"),
+    ("shadow_k", "<f4"),
+    ("_reserved1", "<f4", (3,)),
+])
+
+DEBUG_PX = np.dtype([
+    ("normal", "<f4", (3,)),
+    ("brick_slot", "<u4"),
+    ("voxel_index", "<u4"),
+    ("cell_index", "<u4"),
+    ("steps", "<u4"),
+    ("evals", "<u4"),
+    ("shadow_steps", "<u4"),
+    ("shadow_evals", "<u4"),
+    ("_pad", "<u4", (2,)),
+])
+
+PRIM = np.dtype([
+    ("centre", "<f4", (3,)),
+    ("type", "<u4"),
+    ("params", "<f4", (4,)),
+    ("rot", "<f4", (9,)),
+    ("group", "<u4"),
+    ("material", "<u4"),
+    ("_pad", "<u4"),
+])
+
+SPARSE_PARAMS = np.dtype([
+    ("voxel_size", "<f4"),
+    ("smin_k", "<f4"),
+    ("mat_seed", "<u8"),
+    ("bounds_min", "<f4", (3,)),
+    ("bounds_max", "<f4", (3,)),
+    ("max_bricks", "<u8"),
+])
+
+SPARSE_INFO = np.dtype([
+    ("origin", "<f4", (3,)),
+    ("voxel_size", "<f4"),
+    ("cells", "<u4", (3,)),
+    ("n_blocks", "<u4"),
+    ("n_bricks", "<u8"),
+    ("device_bytes", "<u8"),
+])
+
+POINT_RESULT = np.dtype([
+    ("distance", "<f4"),
+    ("material", "<u4"),
+    ("inside", "<u4"),
+    ("gradient", "<f4", (3,)),
+])
+
+assert VOXEL_SAMPLE.itemsize == 8
+assert GPU_OBJECT.itemsize == 256
+assert CAMERA_UNIFORMS.itemsize == 144
+assert SCENE_UNIFORMS.itemsize == 16
+assert COARSE_FIELD_UNIFORMS.itemsize == 48
+assert BVH_NODE.itemsize == 32
+assert SCENE_EXT.itemsize == 48
+assert DEBUG_PX.itemsize == 48
+assert PRIM.itemsize == 80
+assert POINT_RESULT.itemsize == 24
+
+
+def scene_uniforms(num_objects=0, max_steps=128, max_distance=100.0, hit_threshold=0.001):
+    """SceneUniforms::default (gpu_scene.rs:34-43) with overrides."""
+    s = np.zeros((), SCENE_UNIFORMS)
+    s["num_objects"] = num_objects
+    s["max_steps"] = max_steps
+    s["max_distance"] = max_distance
+    s["hit_threshold"] = hit_threshold
+    return s
+
+
+def scene_ext(hit_eps_mode=EPS_ABSOLUTE, strategy=STRATEGY_AUTO, shadow_enable=0,
+              light_dir=(0.0, 1.0, 0.0), shadow_max_dist=25.0, shadow_k=16.0):
+    e = np.zeros((), SCENE_EXT)
+    e["hit_eps_mode"] = hit_eps_mode
+    e["strategy"] = strategy
+    e["shadow_enable"] = shadow_enable
+    e["light_dir"] = light_dir
+    e["shadow_max_dist"] = shadow_max_dist
+    e["shadow_k"] = shadow_k
+    return e

@@ -1,0 +1,464 @@
+"""ctypes binding of librkf_march.so and a Python mirror of rkf-render's pass API.
+
+The classes keep the reference's names and argument meaning:
+
+* ``GpuScene``      crates/rkf-render/src/gpu_scene.rs:52-195 (brick pool, brick
+                    maps, objects, BVH, camera and scene uniforms)
+* ``GBuffer``       crates/rkf-render/src/gbuffer.rs:14-58 (4 targets in the
+                    reference formats, device-resident)
+* ``TileObjectCullPass`` crates/rkf-render/src/tile_object_cull.rs:23-43,196
+* ``CoarseField``   crates/rkf-render/src/coarse_field.rs:45-307
+* ``RayMarchPass``  crates/rkf-render/src/ray_march.rs:29-118 (new / dispatch /
+                    recreate_pipeline)
+
+Errors: every C call returns a status; a non-zero status raises
+``RkfMarchError`` carrying the library's message. The reference's dispatch
+cannot fail from the caller's view (wgpu validation errors are logged);
+here invalid inputs are reported instead of rendered.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import layouts as L
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librkf_march.so")
+_LIB = None
+
+
+class RkfMarchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rkf_march error {code}: {msg}")
+        self.code = code
+
+
+ERR = dict(OK=0, INVALID_ARG=-1, OUT_OF_RANGE=-2, OOM=-3, DEVICE=-4, LAYOUT=-5, NOT_READY=-6, BUSY=-7)
+
+_VP = C.c_void_p
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+
+_SIGS = {
+    "rkfm_last_error_string": (C.c_char_p, []),
+    "rkfm_version": (C.c_char_p, []),
+    "rkfm_create": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "rkfm_destroy": (C.c_int, [_VP]),
+    "rkfm_sync": (C.c_int, [_VP]),
+    "rkfm_stream": (_VP, [_VP]),
+    "rkfm_upload_brick_pool": (C.c_int, [_VP, _VP, _U64, _U64]),
+    "rkfm_upload_brick_maps": (C.c_int, [_VP, _VP, _U64, _U64]),
+    "rkfm_upload_objects": (C.c_int, [_VP, _VP, _U32]),
+    "rkfm_upload_bvh": (C.c_int, [_VP, _VP, _U32]),
+    "rkfm_set_camera": (C.c_int, [_VP, _VP]),
+    "rkfm_set_scene": (C.c_int, [_VP, _VP, _VP]),
+    "rkfm_set_tiles": (C.c_int, [_VP, _VP, _VP, _U32, _U32]),
+    "rkfm_set_coarse_field": (C.c_int, [_VP, _VP, _VP]),
+    "rkfm_build_tiles": (C.c_int, [_VP, _U32, _U32]),
+    "rkfm_build_coarse_field": (C.c_int, [_VP, _VP, _VP, _F]),
+    "rkfm_read_tiles": (C.c_int, [_VP, _VP, _VP, _U64]),
+    "rkfm_read_coarse_field": (C.c_int, [_VP, _VP, _U64, _VP]),
+    "rkfm_dispatch": (C.c_int, [_VP, _VP, _VP]),
+    "rkfm_voxelize_sphere": (C.c_int, [_F, _F, _F, _U32, _U64, _U32, _U32, _VP, _VP, _VP, _VP]),
+    "rkfm_voxelize_test_cross": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "rkfm_camera_look_at": (C.c_int, [_VP, _VP, _F, _U32, _U32, _VP]),
+    "rkfm_build_bvh": (C.c_int, [_VP, _U32, _VP, _VP]),
+    "rkfm_sparse_build": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
+    "rkfm_sparse_from_dense": (C.c_int, [_VP, _VP, _VP, _VP, _U64, _F, _VP, C.POINTER(_VP)]),
+    "rkfm_sparse_info_get": (C.c_int, [_VP, _VP]),
+    "rkfm_sparse_destroy": (C.c_int, [_VP, _VP]),
+    "rkfm_sparse_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
+    "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
+    "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
+    "rkfm_free": (C.c_int, [_VP, _VP]),
+    "rkfm_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, _U64]),
+    "rkfm_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, _U64]),
+    "rkfm_memset": (C.c_int, [_VP, _VP, C.c_int, _U64]),
+    "rkfm_timer_begin": (C.c_int, [_VP]),
+    "rkfm_timer_end": (C.c_int, [_VP, C.POINTER(_F)]),
+}
+
+
+def lib():
+    """Load librkf_march.so; raise if it is not built (there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        lib_ = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lib_, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = lib_
+    return _LIB
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RkfMarchError(rc, lib().rkfm_last_error_string().decode())
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return a.ctypes.data
+
+
+class Context:
+    """rkfm_ctx: owns device state on one HIP device."""
+
+    def __init__(self, device=0, memory_budget=0):
+        cfg = np.zeros(1, np.dtype([("device", "<i4"), ("flags", "<u4"), ("memory_budget", "<u8")]))
+        cfg["device"] = device
+        cfg["memory_budget"] = memory_budget
+        h = _VP()
+        _check(lib().rkfm_create(_p(cfg), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rkfm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().rkfm_sync(self.h))
+
+    def timer_begin(self):
+        _check(lib().rkfm_timer_begin(self.h))
+
+    def timer_end(self):
+        ms = _F()
+        _check(lib().rkfm_timer_end(self.h, C.byref(ms)))
+        return ms.value
+
+    def array(self, shape, dtype):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, host):
+        host = np.ascontiguousarray(host)
+        d = DeviceArray(self, host.shape, host.dtype)
+        d.upload(host)
+        return d
+
+
+class DeviceArray:
+    """A device allocation made through the library (no framework needed)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = _VP()
+        _check(lib().rkfm_alloc(ctx.h, max(self.nbytes, 1), C.byref(h)))
+        self.ptr = h.value
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, self.dtype)
+        assert host.nbytes == self.nbytes
+        _check(lib().rkfm_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().rkfm_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def fill(self, byte):
+        _check(lib().rkfm_memset(self.ctx.h, self.ptr, byte, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().rkfm_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class GBuffer:
+    """gbuffer.rs:14-58: position rgba32f, normal rgba16f, material r32u,
+    motion rgba32f (+ optional debug and shadow planes), device-resident."""
+
+    def __init__(self, ctx, width, height, debug=False, shadow=False):
+        self.ctx, self.width, self.height = ctx, width, height
+        n = width * height
+        self.position = ctx.array((n, 4), np.float32)
+        self.normal = ctx.array((n, 4), np.uint16)
+        self.material = ctx.array((n,), np.uint32)
+        self.motion = ctx.array((n, 4), np.float32)
+        self.debug = ctx.array((n,), L.DEBUG_PX) if debug else None
+        self.shadow = ctx.array((n,), np.float32) if shadow else None
+        desc = np.zeros(1, np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<u8"),
+                                     ("motion", "<u8"), ("debug", "<u8"), ("shadow", "<u8"),
+                                     ("width", "<u4"), ("height", "<u4")]))
+        desc["position"] = self.position.ptr
+        desc["normal"] = self.normal.ptr
+        desc["material"] = self.material.ptr
+        desc["motion"] = self.motion.ptr
+        desc["debug"] = self.debug.ptr if debug else 0
+        desc["shadow"] = self.shadow.ptr if shadow else 0
+        desc["width"] = width
+        desc["height"] = height
+        self.desc = desc
+
+    def read(self):
+        out = dict(position=self.position.download(), normal=self.normal.download(),
+                   material=self.material.download(), motion=self.motion.download())
+        if self.debug is not None:
+            out["debug"] = self.debug.download()
+        if self.shadow is not None:
+            out["shadow"] = self.shadow.download()
+        return out
+
+
+class GpuScene:
+    """gpu_scene.rs:52-195: the group-0 inputs of the pass."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.num_objects = 0
+
+    def upload_brick_pool(self, bricks, first_slot=0):
+        b = np.ascontiguousarray(bricks, L.VOXEL_SAMPLE).reshape(-1)
+        assert b.size % 512 == 0
+        _check(lib().rkfm_upload_brick_pool(self.ctx.h, _p(b), b.size // 512, first_slot))
+
+    def upload_brick_maps(self, entries, offset=0):
+        e = np.ascontiguousarray(entries, np.uint32)
+        _check(lib().rkfm_upload_brick_maps(self.ctx.h, _p(e), e.size, offset))
+
+    def upload_objects(self, objects):
+        o = np.ascontiguousarray(objects, L.GPU_OBJECT)
+        _check(lib().rkfm_upload_objects(self.ctx.h, _p(o), o.size))
+        self.num_objects = o.size
+
+    def upload_bvh(self, nodes):
+        n = np.ascontiguousarray(nodes, L.BVH_NODE) if nodes is not None else np.zeros(0, L.BVH_NODE)
+        _check(lib().rkfm_upload_bvh(self.ctx.h, _p(n) if n.size else None, n.size))
+
+    def update_camera(self, cam):
+        c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+        _check(lib().rkfm_set_camera(self.ctx.h, _p(c)))
+
+    def update_scene(self, su, ext=None):
+        s = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+        e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
+        _check(lib().rkfm_set_scene(self.ctx.h, _p(s), _p(e)))
+
+
+class TileObjectCullPass:
+    """tile_object_cull.rs:23-205 (16x16 tiles, <= 32 objects per tile)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, width, height
+        self.num_tiles_x = (width + 15) // 16
+        self.num_tiles_y = (height + 15) // 16
+
+    def dispatch(self):
+        _check(lib().rkfm_build_tiles(self.ctx.h, self.width, self.height))
+
+    def set_lists(self, idx, counts):
+        i = np.ascontiguousarray(idx, np.uint32)
+        c = np.ascontiguousarray(counts, np.uint32)
+        _check(lib().rkfm_set_tiles(self.ctx.h, _p(i), _p(c), self.num_tiles_x, self.num_tiles_y))
+
+    def clear(self):
+        _check(lib().rkfm_set_tiles(self.ctx.h, None, None, 0, 0))
+
+    def read(self):
+        nt = self.num_tiles_x * self.num_tiles_y
+        idx = np.zeros(nt * 32, np.uint32)
+        counts = np.zeros(nt, np.uint32)
+        _check(lib().rkfm_read_tiles(self.ctx.h, _p(idx), _p(counts), nt))
+        return idx, counts
+
+
+COARSE_VOXEL_SIZE = 0.32  # coarse_field.rs:20
+MAX_COARSE_DIM = 128      # coarse_field.rs:23
+
+
+class CoarseField:
+    """coarse_field.rs:45-307: f16 unsigned distance to object AABBs."""
+
+    def __init__(self, ctx, origin, extent, voxel_size=COARSE_VOXEL_SIZE):
+        self.ctx = ctx
+        self.origin = np.asarray(origin, np.float32)
+        ext = np.asarray(extent, np.float32)
+        self.dims = np.array([min(max(int(np.ceil(np.float32(e) / np.float32(voxel_size))), 1), MAX_COARSE_DIM)
+                              for e in ext], np.uint32)
+        self.voxel_size = np.float32(voxel_size)
+
+    @classmethod
+    def from_scene_aabbs(cls, ctx, aabbs, voxel_size=COARSE_VOXEL_SIZE, margin=1.0):
+        """coarse_field.rs:211-237."""
+        if len(aabbs) == 0:
+            lo, hi = np.zeros(3, np.float32), np.full(3, voxel_size, np.float32)
+        else:
+            lo = np.min(np.array([a[0] for a in aabbs], np.float32), axis=0)
+            hi = np.max(np.array([a[1] for a in aabbs], np.float32), axis=0)
+        origin = lo - np.float32(margin)
+        extent = (hi + np.float32(margin)) - origin
+        return cls(ctx, origin, extent, voxel_size)
+
+    def populate_and_upload(self):
+        """populate (coarse_field.rs:240-266) on the GPU + upload uniforms
+        relative to the scene camera (coarse_field.rs:279-307)."""
+        _check(lib().rkfm_build_coarse_field(self.ctx.h, _p(self.origin), _p(self.dims), self.voxel_size))
+
+    def read(self):
+        n = int(np.prod(self.dims))
+        cells = np.zeros(n, np.uint16)
+        u = np.zeros(1, L.COARSE_FIELD_UNIFORMS)
+        _check(lib().rkfm_read_coarse_field(self.ctx.h, _p(cells), n, _p(u)))
+        return cells, u[0]
+
+    def upload_cells(self, cells, uniforms):
+        c = np.ascontiguousarray(cells, np.uint16)
+        u = np.ascontiguousarray(uniforms, L.COARSE_FIELD_UNIFORMS)
+        _check(lib().rkfm_set_coarse_field(self.ctx.h, _p(c), _p(u)))
+
+    def clear(self):
+        _check(lib().rkfm_set_coarse_field(self.ctx.h, None, None))
+
+
+class RayMarchPass:
+    """ray_march.rs:29-118."""
+
+    def __init__(self, ctx, scene=None, gbuffer=None, tile_cull=None, coarse_field=None):
+        self.ctx = ctx
+
+    def recreate_pipeline(self, *_args):
+        """Hot reload has no meaning for ahead-of-time compiled kernels: no-op."""
+
+    def dispatch(self, scene, gbuffer, tile_cull=None, coarse_field=None, stream=None):
+        _check(lib().rkfm_dispatch(self.ctx.h, gbuffer.desc.ctypes.data, stream))
+
+
+def voxelize_sphere(radius, margin, voxel_size, object_index=0, mat_seed=0, material=0, first_slot=0):
+    """Host voxelize_sdf of |p| - r over +-(r + margin). Returns (dims, map, bricks)."""
+    dims = np.zeros(3, np.uint32)
+    nb = np.zeros(1, np.uint64)
+    _check(lib().rkfm_voxelize_sphere(radius, margin, voxel_size, object_index, mat_seed, material, first_slot,
+                                      _p(dims), None, None, _p(nb)))
+    m = np.empty(int(np.prod(dims)), np.uint32)
+    b = np.empty(int(nb[0]) * 512, L.VOXEL_SAMPLE)
+    _check(lib().rkfm_voxelize_sphere(radius, margin, voxel_size, object_index, mat_seed, material, first_slot,
+                                      _p(dims), _p(m), _p(b), _p(nb)))
+    return dims, m, b
+
+
+def voxelize_test_cross():
+    dims = np.zeros(3, np.uint32)
+    nb = np.zeros(1, np.uint64)
+    _check(lib().rkfm_voxelize_test_cross(_p(dims), None, None, _p(nb)))
+    m = np.empty(int(np.prod(dims)), np.uint32)
+    b = np.empty(int(nb[0]) * 512, L.VOXEL_SAMPLE)
+    _check(lib().rkfm_voxelize_test_cross(_p(dims), _p(m), _p(b), _p(nb)))
+    return dims, m, b
+
+
+def camera_look_at(position, target, fov_degrees, width, height):
+    out = np.zeros(1, L.CAMERA_UNIFORMS)
+    pos = np.asarray(position, np.float32)
+    tgt = np.asarray(target, np.float32)
+    _check(lib().rkfm_camera_look_at(_p(pos), _p(tgt), fov_degrees, width, height, _p(out)))
+    return out[0]
+
+
+def build_bvh(objects):
+    o = np.ascontiguousarray(objects, L.GPU_OBJECT)
+    nodes = np.zeros(max(2 * o.size - 1, 1), L.BVH_NODE)
+    n = np.zeros(1, np.uint32)
+    _check(lib().rkfm_build_bvh(_p(o), o.size, _p(nodes), _p(n)))
+    return nodes[: int(n[0])]
+
+
+class SparseScene:
+    """A resident two-level sparse scene (see include/rkf_march.h)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = np.zeros(1, L.SPARSE_INFO)
+        _check(lib().rkfm_sparse_info_get(self.h, _p(info)))
+        self.info = info[0]
+
+    @classmethod
+    def build(cls, ctx, prims, voxel_size, bounds_min, bounds_max, smin_k=0.0, mat_seed=0, max_bricks=0):
+        p = np.ascontiguousarray(prims, L.PRIM)
+        prm = np.zeros(1, L.SPARSE_PARAMS)
+        prm["voxel_size"] = voxel_size
+        prm["smin_k"] = smin_k
+        prm["mat_seed"] = mat_seed
+        prm["bounds_min"] = bounds_min
+        prm["bounds_max"] = bounds_max
+        prm["max_bricks"] = max_bricks
+        h = _VP()
+        _check(lib().rkfm_sparse_build(ctx.h, _p(p), p.size, _p(prm), C.byref(h)))
+        return cls(ctx, h.value)
+
+    @classmethod
+    def from_dense(cls, ctx, dims, brick_map, bricks, voxel_size, origin):
+        d = np.asarray(dims, np.uint32)
+        m = np.ascontiguousarray(brick_map, np.uint32)
+        b = np.ascontiguousarray(bricks, L.VOXEL_SAMPLE).reshape(-1)
+        o = np.asarray(origin, np.float32)
+        h = _VP()
+        _check(lib().rkfm_sparse_from_dense(ctx.h, _p(m), _p(d), _p(b), b.size // 512, voxel_size, _p(o),
+                                            C.byref(h)))
+        return cls(ctx, h.value)
+
+    def destroy(self):
+        if self.h:
+            _check(lib().rkfm_sparse_destroy(self.ctx.h, self.h))
+            self.h = None
+
+    def download(self):
+        i = self.info
+        nc = int(np.prod(i["cells"]))
+        l2 = np.zeros(nc, np.uint32)
+        l1 = np.zeros(int(i["n_blocks"]) * 512, np.uint32)
+        dist = np.zeros(int(i["n_bricks"]) * 512, np.uint16)
+        mat = np.zeros(int(i["n_bricks"]) * 512, np.uint8)
+        _check(lib().rkfm_sparse_download(self.ctx.h, self.h, _p(l2), _p(l1), _p(dist), _p(mat)))
+        return l2, l1, dist, mat
+
+    def download_free(self):
+        nc = int(np.prod(self.info["cells"]))
+        rows = np.zeros(1, np.uint64)
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, None, None, _p(rows)))
+        fidx = np.zeros(nc, np.uint32)
+        bits = np.zeros(max(int(rows[0]) * 8, 1), np.uint64)
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, _p(fidx), _p(bits), _p(rows)))
+        return fidx, bits
+
+    def march(self, cam, su, ext, gbuffer, variant=0, stream=None):
+        c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+        s = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+        e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
+        _check(lib().rkfm_sparse_march(self.ctx.h, self.h, _p(c), _p(s), _p(e), gbuffer.desc.ctypes.data,
+                                       variant, stream))
+
+    def query(self, xyz_dev, out_dev, n, stream=None):
+        _check(lib().rkfm_sparse_query_points(self.ctx.h, self.h, _p(xyz_dev), n, _p(out_dev), stream))

@@ -1,0 +1,177 @@
+"""GPU parity of the drop-in kernel (reference layouts) against the oracle.
+
+Bit-exact: hit mask, packed material (material | secondary | blend | object id),
+brick slot, voxel index, object index, step and evaluation counts, and (same
+operation order) every float of the G-buffer vs the FP32 twin.
+Tolerances vs the FP64 oracle (rays whose discrete path is the same):
+depth <= 1e-4 relative, normal <= 1e-3 rad (f32 debug normal).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from rkifield_amd import layouts as L
+from rkifield_amd import march as M
+from rkifield_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DEPTH_REL_TOL = 1e-4
+NORMAL_RAD_TOL = 1e-3
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _upload(ctx, sc, strategy=L.STRATEGY_AUTO, tiles=True, coarse=True):
+    gs = M.GpuScene(ctx)
+    gs.upload_brick_pool(sc.pool)
+    gs.upload_brick_maps(sc.maps)
+    gs.upload_objects(sc.objects)
+    gs.upload_bvh(M.build_bvh(sc.objects))
+    gs.update_camera(sc.cam)
+    ext = sc.ext.copy()
+    ext["strategy"] = strategy
+    gs.update_scene(sc.su, ext)
+    tc = M.TileObjectCullPass(ctx, sc.width, sc.height)
+    cf = M.CoarseField.from_scene_aabbs(ctx, sc.aabbs())
+    if tiles:
+        tc.dispatch()
+    else:
+        tc.clear()
+    if coarse:
+        cf.populate_and_upload()
+    else:
+        cf.clear()
+    return gs, tc, cf, ext
+
+
+def _compare(gpu, res, width, height):
+    pos, nrm, mat, mot = O.pack_gbuffer(res)
+    dbg = gpu["debug"]
+    hit = res["hit"] == 1
+    ghit = gpu["position"][:, 3] != L.MAX_FLOAT
+    assert np.array_equal(ghit, hit), f"hit mask differs on {np.sum(ghit != hit)} rays"
+    assert np.array_equal(gpu["material"], mat)
+    assert np.array_equal(dbg["brick_slot"], res["brick_slot"])
+    assert np.array_equal(dbg["voxel_index"], res["voxel_index"])
+    assert np.array_equal(dbg["cell_index"], res["cell_index"])
+    assert np.array_equal(dbg["steps"], res["steps"])
+    assert np.array_equal(dbg["evals"], res["evals"])
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    dn = np.asarray(res["normal"], np.float32)
+    assert np.array_equal(dbg["normal"][hit].view(np.uint32), dn[hit].view(np.uint32))
+
+
+def _tolerance(gpu, res64, res32):
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    hit = same & (res32["hit"] == 1)
+    t = gpu["position"][:, 3].astype(np.float64)
+    rel = np.abs(t[hit] - res64["t"][hit]) / res64["t"][hit]
+    n = gpu["debug"]["normal"].astype(np.float64)
+    cosang = np.clip(np.sum(n[hit] * res64["normal"][hit], axis=1), -1.0, 1.0)
+    ang = np.arccos(cosang)
+    return same.mean(), rel.max(initial=0.0), ang.max(initial=0.0)
+
+
+@pytest.fixture(scope="module")
+def spheres():
+    return scenes.spheres_scene(seed=0)
+
+
+def test_producers_match_oracle(ctx, spheres):
+    sc = spheres
+    _say("case: producers, spheres seed 0, 640x360")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    idx, counts = tc.read()
+    oidx, ocounts = O.tile_cull(sc.cam, sc.su, sc.objects, sc.width, sc.height)
+    assert np.array_equal(counts, ocounts)
+    for t in range(counts.size):
+        c = counts[t]
+        assert np.array_equal(idx[t * 32:t * 32 + c], oidx[t * 32:t * 32 + c])
+    cells, u = cf.read()
+    ocells = O.coarse_populate(cf.origin, cf.dims, cf.voxel_size, sc.objects)
+    assert np.array_equal(cells, ocells)
+    assert np.allclose(u["origin_cam_rel"][:3], cf.origin - sc.cam["position"][:3])
+
+
+@pytest.mark.parametrize("strategy", [L.STRATEGY_TILED])
+def test_config0_full_parity(ctx, spheres, strategy):
+    sc = spheres
+    _say(f"case: config0 640x360 strategy {strategy}")
+    gs, tc, cf, ext = _upload(ctx, sc, strategy)
+    gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+    M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    gpu = gb.read()
+    idx, counts = tc.read()
+    cells, u = cf.read()
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=u)
+    res32, _ = O.ref_trace_parallel(inp)
+    assert res32["hit"].sum() > 1000
+    _compare(gpu, res32, sc.width, sc.height)
+    res64 = O.ref_trace(inp, prec=64)
+    same, rel, ang = _tolerance(gpu, res64, res32)
+    _say(f"config0: same-path {same:.5f} max rel depth {rel:.3e} max normal angle {ang:.3e} rad")
+    assert same > 0.995
+    assert rel <= DEPTH_REL_TOL
+    assert ang <= NORMAL_RAD_TOL
+
+
+@pytest.mark.parametrize("strategy", [L.STRATEGY_BVH, L.STRATEGY_BRUTE])
+def test_fallback_strategies_parity(ctx, strategy):
+    sc = scenes.spheres_scene(seed=3, n=12, width=96, height=64, half_cube=3.0, cam_dist=9.0)
+    _say(f"case: fallback strategy {strategy} 96x64")
+    gs, tc, cf, ext = _upload(ctx, sc, strategy, tiles=False, coarse=False)
+    gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+    M.RayMarchPass(ctx).dispatch(gs, gb)
+    gpu = gb.read()
+    nodes = M.build_bvh(sc.objects)
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      bvh=nodes)
+    res32 = O.ref_trace(inp, prec=32)
+    assert res32["hit"].sum() > 50
+    _compare(gpu, res32, sc.width, sc.height)
+
+
+def test_ragged_sizes_and_absolute_eps(ctx):
+    """Image sizes that are not multiples of 8 or 16; absolute hit threshold."""
+    sc = scenes.spheres_scene(seed=5, n=10, width=37, height=23, half_cube=2.0, cam_dist=7.0, rel_eps=False)
+    _say("case: ragged 37x23 abs eps")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+    M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    gpu = gb.read()
+    idx, counts = tc.read()
+    cells, u = cf.read()
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=u)
+    _compare(gpu, O.ref_trace(inp, prec=32), sc.width, sc.height)
+
+
+def test_empty_scene_and_errors(ctx):
+    sc = scenes.spheres_scene(seed=1, n=4, width=16, height=16, half_cube=2.0, cam_dist=6.0)
+    _say("case: errors")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    su = sc.su.copy()
+    su["num_objects"] = 99
+    gs.update_scene(su, ext)
+    gb = M.GBuffer(ctx, 16, 16)
+    with pytest.raises(M.RkfMarchError) as e:
+        M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    assert e.value.code == M.ERR["OUT_OF_RANGE"]
+    su["num_objects"] = 0
+    gs.update_scene(su, ext)
+    tc.dispatch()
+    M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    out = gb.read()
+    assert np.all(out["position"][:, 3] == L.MAX_FLOAT) and np.all(out["material"] == 0)
+    gb2 = M.GBuffer(ctx, 32, 16)
+    su["num_objects"] = 4
+    gs.update_scene(su, ext)
+    with pytest.raises(M.RkfMarchError) as e:
+        M.RayMarchPass(ctx).dispatch(gs, gb2, tc, cf)  # tile grid built for 16x16
+    assert e.value.code == M.ERR["LAYOUT"]

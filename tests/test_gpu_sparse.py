@@ -1,0 +1,214 @@
+"""GPU parity of the sparse two-level scene path against the oracle.
+
+* GPU voxeliser: every stored voxel distance and every allocation decision
+  equals the brute-force oracle field (all primitives, all groups), bit for bit.
+* Free-space masks: equal to the oracle's dense masks.
+* March (plain and tuned variants), shadow rays and point queries: bit-exact
+  vs the FP32 twin; depth/normal within tolerance of the FP64 oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from rkifield_amd import layouts as L
+from rkifield_amd import march as M
+from rkifield_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DEPTH_REL_TOL = 1e-4
+NORMAL_RAD_TOL = 1e-3
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _mix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _small_prims(seed=7, n=48, half=3.0):
+    p = scenes.primitive_field(seed=seed, n=n, half_cube=half, size_range=(0.3, 1.5), group=4)
+    return p
+
+
+def _oracle_inputs(sc):
+    i = sc.info
+    l2, l1, dist, mat = sc.download()
+    return O.SparseInputs(i["origin"], float(i["voxel_size"]), i["cells"], l2, l1, dist, mat), (l2, l1, dist, mat)
+
+
+@pytest.fixture(scope="module")
+def small_scene(ctx):
+    prims = _small_prims()
+    lo, hi = scenes.prim_bounds(prims, 0.5)
+    _say("case: sparse build, 48 prims, vs 0.05")
+    with np.errstate(over="ignore"):
+        sc = M.SparseScene.build(ctx, prims, 0.05, lo, hi, smin_k=0.1, mat_seed=scenes.material_seed(7))
+    yield prims, sc, lo
+    sc.destroy()
+
+
+def test_voxeliser_matches_brute_force_field(small_scene):
+    prims, sc, lo = small_scene
+    sp, (l2, l1, dist, mat) = _oracle_inputs(sc)
+    i = sc.info
+    assert i["n_bricks"] > 100
+    vs = np.float32(i["voxel_size"])
+    bws = vs * np.float32(8)
+    cells = [int(c) for c in i["cells"]]
+    lib = O.lib()
+    lib.orc_scene_field.restype = C.c_float
+    lib.orc_scene_field.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]
+    pr = np.ascontiguousarray(prims)
+    field = lambda p: lib.orc_scene_field(pr.ctypes.data, pr.size, np.float32(0.1), np.asarray(p, np.float32).ctypes.data)
+    origin = np.asarray(i["origin"], np.float32)
+    band = bws * np.float32(1.8)
+    nb = [c * 8 for c in cells]
+    rng = np.random.default_rng(0)
+    # allocation decisions: every brick of every occupied cell plus random bricks
+    checked = 0
+    for gbz in range(nb[2]):
+        for gby in range(nb[1]):
+            for gbx in range(nb[0]):
+                cell = (gbx >> 3) + (gby >> 3) * cells[0] + (gbz >> 3) * cells[0] * cells[1]
+                blk = l2[cell]
+                slot = L.EMPTY_SLOT if blk == L.EMPTY_SLOT else l1[blk * 512 + (gbx & 7) + (gby & 7) * 8 + (gbz & 7) * 64]
+                if blk == L.EMPTY_SLOT and rng.random() > 0.02:
+                    continue
+                hb = bws * np.float32(0.5)
+                c = [(origin[0] + np.float32(gbx) * bws) + hb, (origin[1] + np.float32(gby) * bws) + hb,
+                     (origin[2] + np.float32(gbz) * bws) + hb]
+                need = abs(field(c)) < band
+                assert need == (slot != L.EMPTY_SLOT), (gbx, gby, gbz)
+                checked += 1
+                if slot != L.EMPTY_SLOT and rng.random() < 0.05:
+                    half = vs * np.float32(0.5)
+                    for v in rng.integers(0, 512, 16):
+                        vx, vy, vz = v & 7, (v >> 3) & 7, v >> 6
+                        p = [(origin[0] + np.float32(gbx) * bws) + (np.float32(vx) * vs + half),
+                             (origin[1] + np.float32(gby) * bws) + (np.float32(vy) * vs + half),
+                             (origin[2] + np.float32(gbz) * bws) + (np.float32(vz) * vs + half)]
+                        want = O.f32_to_f16_bits([field(p)])[0]
+                        assert dist[slot * 512 + v] == want
+                        key = np.uint64((gbx * 8 + vx) | ((gby * 8 + vy) << 21) | ((gbz * 8 + vz) << 42))
+                        seed = np.uint64(scenes.material_seed(7))
+                        with np.errstate(over="ignore"):
+                            m = int(_mix64(seed ^ _mix64(key)) & np.uint64(15))
+                        assert mat[slot * 512 + v] == m
+    assert checked > 500
+
+
+def test_free_masks_match_oracle(small_scene):
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    fidx, bits = sc.download_free()
+    cells = [int(c) for c in sc.info["cells"]]
+    nb = [c * 8 for c in cells]
+    for cz in range(cells[2]):
+        for cy in range(cells[1]):
+            for cx in range(cells[0]):
+                cell = cx + cy * cells[0] + cz * cells[0] * cells[1]
+                assert (fidx[cell] == 0xFFFFFFFF) == bool(sp.free_l2[cell])
+                if fidx[cell] == 0xFFFFFFFF:
+                    continue
+                row = bits[fidx[cell] * 8: fidx[cell] * 8 + 8]
+                for lb in range(512):
+                    bx, by, bz = cx * 8 + (lb & 7), cy * 8 + ((lb >> 3) & 7), cz * 8 + (lb >> 6)
+                    notfree = (int(row[lb >> 6]) >> (lb & 63)) & 1
+                    assert notfree == (0 if sp.brick_free(bx, by, bz) else 1)
+
+
+@pytest.mark.parametrize("variant", [1, 0])
+def test_sparse_march_parity(ctx, small_scene, variant):
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    w, h = 96, 64
+    cam = M.camera_look_at((6.0, 3.0, 7.0), (0, 0, 0), 60.0, w, h)
+    su = L.scene_uniforms(num_objects=1)
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1,
+                      light_dir=(np.array([0.5, 1.0, 0.3]) / np.linalg.norm([0.5, 1.0, 0.3])))
+    _say(f"case: sparse march variant {variant} {w}x{h}")
+    gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
+    sc.march(cam, su, ext, gb, variant=variant)
+    gpu = gb.read()
+    res32 = O.sparse_trace(sp, cam, su, ext, w, h, prec=32)
+    assert res32["hit"].sum() > 300
+    pos, nrm, mat, mot = O.pack_gbuffer(res32)
+    hit = res32["hit"] == 1
+    assert np.array_equal(gpu["position"][:, 3] != L.MAX_FLOAT, hit)
+    assert np.array_equal(gpu["material"], mat)
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals", "shadow_steps", "shadow_evals"):
+        assert np.array_equal(gpu["debug"][f], res32[f]), f
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    assert np.array_equal(gpu["shadow"].view(np.uint32), res32["shadow"].astype(np.float32).view(np.uint32))
+    res64 = O.sparse_trace(sp, cam, su, ext, w, h, prec=64)
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    both = same & hit
+    rel = np.abs(gpu["position"][both, 3] - res64["t"][both]) / res64["t"][both]
+    n = gpu["debug"]["normal"][both].astype(np.float64)
+    ang = np.arccos(np.clip(np.sum(n * res64["normal"][both], axis=1), -1, 1))
+    _say(f"sparse: same-path {same.mean():.4f} max rel depth {rel.max():.3e} max normal angle {ang.max():.3e}")
+    assert same.mean() > 0.99
+    assert rel.max() <= DEPTH_REL_TOL
+    assert ang.max() <= NORMAL_RAD_TOL
+
+
+def test_point_queries_parity(ctx, small_scene):
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    rng = np.random.default_rng(3)
+    n = 5000
+    hi = lo + np.asarray(sc.info["cells"], np.float32) * np.float32(sc.info["voxel_size"] * 64)
+    pts = rng.uniform(lo - 1, hi + 1, (n, 3)).astype(np.float32)
+    _say("case: point queries 5000")
+    d_in = ctx.to_device(pts)
+    d_out = ctx.array((n,), L.POINT_RESULT)
+    sc.query(d_in, d_out, n)
+    got = d_out.download()
+    want = O.sparse_query(sp, pts, prec=32)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+def test_from_dense_matches_reference_layout(ctx):
+    """test_cross.rkf streamed into a resident sparse scene at the origin."""
+    import os
+    data = open(os.path.join(os.path.dirname(__file__), "golden", "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    vs = np.float32(h["voxel_size"])
+    half = h["dims"].astype(np.float32) * vs * np.float32(8) * np.float32(0.5)
+    _say("case: from_dense test_cross")
+    sc = M.SparseScene.from_dense(ctx, h["dims"], m, b, float(vs), -half)
+    sp, _ = _oracle_inputs(sc)
+    w, hh = 64, 48
+    cam = M.camera_look_at((0.3, 1.0, 1.6), (0, 0, 0), 60.0, w, hh)
+    su = L.scene_uniforms(num_objects=1)
+    ext = L.scene_ext()
+    gb = M.GBuffer(ctx, w, hh, debug=True)
+    sc.march(cam, su, ext, gb, variant=1)
+    gpu = gb.read()
+    res = O.sparse_trace(sp, cam, su, ext, w, hh, prec=32)
+    pos, nrm, mat, mot = O.pack_gbuffer(res)
+    assert res["hit"].sum() > 200
+    assert np.array_equal(gpu["material"], mat)
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["debug"]["brick_slot"], res["brick_slot"])
+    sc.destroy()
+
+
+def test_sparse_errors(ctx):
+    prims = _small_prims(n=4)
+    lo, hi = scenes.prim_bounds(prims, 0.5)
+    with pytest.raises(M.RkfMarchError) as e:
+        M.SparseScene.build(ctx, prims, 0.05, lo, hi, max_bricks=1)
+    assert e.value.code == M.ERR["OOM"]
+    with pytest.raises(M.RkfMarchError):
+        M.SparseScene.build(ctx, prims, -1.0, lo, hi)

@@ -1,0 +1,329 @@
+"""Pins for the CPU oracle (oracle/) — the reference's own vectors, the golden
+fixture, and closed forms. CPU only.
+
+Reference vectors used (file:line in the reference):
+  voxel.rs:120-277, brick.rs:189-338, brick_map.rs:322-333, brick_pool.rs:198-540,
+  sampling.rs:88-292, sdf.rs:58-167, scenes/test_cross.rkf + examples/gen_test_rkf.rs.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from rkifield_amd import layouts as L
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------- binary16 ----------------
+
+def test_f16_matches_ieee_binary16_exhaustively_on_a_sweep():
+    rng = np.random.default_rng(0)
+    bits = np.concatenate([
+        rng.integers(0, 2**32, 2_000_000, dtype=np.uint64).astype(np.uint32),
+        np.arange(0x33000000, 0x33800000, 97, dtype=np.uint32),  # subnormal boundary
+        np.arange(0x477FE000, 0x47800100, 1, dtype=np.uint32),   # overflow boundary
+        np.array([0, 0x80000000, 0x7F800000, 0xFF800000, 0x3F800000, 0x38800000, 0x33800000, 0x33000000],
+                 np.uint32),
+    ])
+    f = bits.view(np.float32)
+    finite = np.isfinite(f)
+    ours = O.f32_to_f16_bits(f)
+    ref = f.astype(np.float16).view(np.uint16)
+    assert np.array_equal(ours[finite], ref[finite])
+    nan = np.isnan(f)
+    assert np.all(np.isnan(O.f16_bits_to_f32(ours[nan])))
+
+
+def test_f16_decode_all_values():
+    h = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    ours = O.f16_bits_to_f32(h)
+    ref = h.view(np.float16).astype(np.float32)
+    both_nan = np.isnan(ours) & np.isnan(ref)
+    assert np.array_equal(ours[~both_nan].view(np.uint32), ref[~both_nan].view(np.uint32))
+
+
+# ---------------- VoxelSample (voxel.rs tests) ----------------
+
+def test_voxel_roundtrip_vectors():
+    w0, w1 = O.voxel_new(1.5, 42, 0)
+    assert O.f16_bits_to_f32([w0 & 0xFFFF])[0] == 1.5 and (w0 >> 16) & 0x3F == 42 and w1 == 0
+    w0, _ = O.voxel_new(-0.5, 1, 0)
+    assert O.f16_bits_to_f32([w0 & 0xFFFF])[0] == -0.5
+    w0, w1 = O.voxel_new(1.0, 5, 200)
+    assert (w0 >> 16) & 0x3F == 5 and w1 >> 24 == 200
+    w0, _ = O.voxel_new(0.0, 0xFF, 0)
+    assert (w0 >> 16) & 0x3F == 63 and (w0 >> 22) & 0x3F == 0 and w0 & 0xFFFF == 0
+    w0, _ = O.voxel_new(0.0, 0x7F, 0)
+    assert (w0 >> 16) & 0x3F == 63 and (w0 >> 22) & 0x3F == 0
+    w0, _ = O.voxel_new(65504.0, 0, 0)
+    assert w0 & 0xFFFF == 0x7BFF  # f16::MAX
+    w0, w1 = O.voxel_new(float("inf"), 0, 0)  # VoxelSample::default
+    assert w0 == 0x7C00 and w1 == 0
+    assert (w0 >> 28) == 0
+
+
+# ---------------- index math ----------------
+
+def test_brick_index_corners():
+    assert O.brick_index(0, 0, 0) == 0
+    assert O.brick_index(7, 0, 0) == 7
+    assert O.brick_index(0, 7, 0) == 56
+    assert O.brick_index(0, 0, 7) == 448
+    assert O.brick_index(7, 7, 7) == 511
+    assert O.brick_index(8, 0, 0) == -1
+
+
+def test_map_flatten_vectors():
+    d = (3, 4, 5)
+    assert O.map_flatten(d, 0, 0, 0) == 0
+    assert O.map_flatten(d, 2, 0, 0) == 2
+    assert O.map_flatten(d, 0, 1, 0) == 3
+    assert O.map_flatten(d, 0, 0, 1) == 12
+    assert O.map_flatten(d, 2, 3, 4) == 59
+    assert O.map_flatten(d, 3, 0, 0) == -1
+
+
+def test_pool_allocates_low_slots_first():
+    assert O.pool_allocate_sequence(8, 4) == [0, 1, 2, 3]
+    assert O.pool_allocate_sequence(2, 3) == [0, 1, -1]
+
+
+# ---------------- CPU sampling (sampling.rs tests) ----------------
+
+def _brick(fn):
+    b = np.zeros(512, L.VOXEL_SAMPLE)
+    for z in range(8):
+        for y in range(8):
+            for x in range(8):
+                d, m = fn(x, y, z)
+                w0, w1 = O.voxel_new(d, m, 0)
+                b[x + 8 * y + 64 * z] = (w0, w1)
+    return b
+
+
+def _centre(x, y, z):
+    return [(x + 0.5) / 8, (y + 0.5) / 8, (z + 0.5) / 8]
+
+
+def test_trilinear_at_voxel_centres():
+    b = _brick(lambda x, y, z: (x + y * 0.1 + z * 0.01, (x + 8 * y + 64 * z) % 256))
+    for z in range(8):
+        for y in range(8):
+            for x in range(8):
+                stored = O.f16_bits_to_f32([b[x + 8 * y + 64 * z]["word0"] & 0xFFFF])[0]
+                assert abs(O.sample_brick_trilinear(b, _centre(x, y, z)) - stored) < 0.02
+                assert O.sample_brick_nearest_material(b, _centre(x, y, z)) == ((x + 8 * y + 64 * z) % 256) & 0x3F
+
+
+def test_trilinear_uniform_midpoint_monotone_clamp():
+    b = _brick(lambda x, y, z: (3.5, 1))
+    for p in ([0, 0, 0], [0.5, 0.5, 0.5], [1, 1, 1], [0.25, 0.75, 0.1]):
+        assert abs(O.sample_brick_trilinear(b, p) - 3.5) < 0.1
+    b = _brick(lambda x, y, z: (0.0 if x < 4 else 2.0, 1))
+    assert abs(O.sample_brick_trilinear(b, [(3.5 + 0.5) / 8, 0.5, 0.5]) - 1.0) < 0.15
+    b = _brick(lambda x, y, z: (float(x), 1))
+    prev = O.sample_brick_trilinear(b, [0.0, 0.5, 0.5])
+    for i in range(1, 21):
+        cur = O.sample_brick_trilinear(b, [i / 20, 0.5, 0.5])
+        assert cur >= prev - 0.01
+        prev = cur
+    g = _brick(lambda x, y, z: (x + y * 0.1 + z * 0.01, 0))
+    for p in ([-0.5, 0.5, 0.5], [1.5, 0.5, 0.5], [0.5, -0.1, 1.5]):
+        assert np.isfinite(O.sample_brick_trilinear(g, p))
+
+
+def test_nearest_material_snap_and_clamp():
+    b = _brick(lambda x, y, z: (0.0, 10 if (x, y, z) == (3, 3, 3) else 20))
+    assert O.sample_brick_nearest_material(b, _centre(3, 3, 3)) == 10
+    assert O.sample_brick_nearest_material(b, [3.6 / 8, 3.5 / 8, 3.5 / 8]) == 10
+    b = _brick(lambda x, y, z: (0.0, 42))
+    assert O.sample_brick_nearest_material(b, [-1, -1, -1]) == 42
+    assert O.sample_brick_nearest_material(b, [2, 2, 2]) == 42
+
+
+# ---------------- analytic SDFs (sdf.rs tests) ----------------
+
+def test_sdf_closed_forms():
+    import ctypes as C
+    lib = O.lib()
+    f3 = C.c_float * 3
+    lib.orc_sphere_sdf.restype = C.c_float
+    lib.orc_box_sdf.restype = C.c_float
+    lib.orc_capsule_sdf.restype = C.c_float
+    lib.orc_capsule_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    lib.orc_sphere_sdf.argtypes = [C.c_void_p, C.c_float, C.c_void_p]
+    sph = lambda c, r, p: lib.orc_sphere_sdf(f3(*c), r, f3(*p))
+    box = lambda h, p: lib.orc_box_sdf(f3(*h), f3(*p))
+    cap = lambda a, b, r, p: lib.orc_capsule_sdf(f3(*a), f3(*b), r, f3(*p))
+    assert abs(sph([0, 0, 0], 1, [0, 0, 0]) + 1) < 1e-6
+    assert abs(sph([0, 0, 0], 1, [1, 0, 0])) < 1e-6
+    assert abs(sph([0, 0, 0], 1, [2, 0, 0]) - 1) < 1e-6
+    assert abs(sph([1, 2, 3], 0.5, [1, 2, 3]) + 0.5) < 1e-6
+    assert abs(cap([0, 0, 0], [0, 1, 0], 0.5, [0, 0.5, 0]) + 0.5) < 1e-6
+    assert abs(cap([0, 0, 0], [0, 1, 0], 0.5, [1.5, 0.5, 0]) - 1.0) < 1e-6
+    assert abs(box([1, 1, 1], [0, 0, 0]) + 1) < 1e-6
+    assert abs(box([1, 1, 1], [2, 2, 2]) - math.sqrt(3)) < 1e-5
+    assert abs(box([1, 1, 1], [0.5, 0, 0]) + 0.5) < 1e-6
+    assert abs(O.lib().orc_smin(1.0, 5.0, 0.1) - 1.0) < 1e-7  # |a-b| >= k -> min
+
+
+# ---------------- golden fixture ----------------
+
+def test_rkf_fixture_header():
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    assert h["version"] == 2 and h["lod_count"] == 1 and h["material_ids"] == [6]
+    assert list(h["dims"]) == [8, 4, 8] and h["brick_count"] == 224
+    assert h["uncompressed_size"] == 256 * 4 + 224 * 4096
+    assert np.allclose(h["aabb_max"], [0.96, 0.48, 0.96])
+
+
+def test_voxelize_sdf_reproduces_test_cross_bit_exactly():
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    dims, m2, b2 = O.voxelize_test_cross()
+    assert list(dims) == list(h["dims"])
+    assert np.array_equal(m, m2)
+    assert np.array_equal(b["word0"], b2["word0"])
+    assert np.array_equal(b["word1"], b2["word1"])
+    # materials: 6 everywhere; slots 0..223 in z-y-x order
+    assert np.all(((b["word0"] >> 16) & 0x3F) == 6)
+    occ = m[m != L.EMPTY_SLOT]
+    assert np.array_equal(occ, np.arange(224))
+
+
+def test_lz4_rejects_corruption():
+    data = bytearray(open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read())
+    data[0:4] = b"XXXX"
+    with pytest.raises(ValueError):
+        O.rkf_load(bytes(data))
+
+
+# ---------------- march: closed forms and invariants ----------------
+
+def _single_sphere_scene(r=1.0, vs=0.05, cam=(0.0, 0.0, 3.0), w=48, h=32, rel=False):
+    from rkifield_amd import march as M
+    dims, m, b = M.voxelize_sphere(r, vs * 4, vs, 0, 0, 7, 0)
+    obj = np.zeros(1, L.GPU_OBJECT)
+    obj["inverse_world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    be = np.float32(vs * 8)
+    half = dims.astype(np.float32) * be * np.float32(0.5)
+    obj["aabb_min"][0, :3] = -half
+    obj["aabb_max"][0, :3] = half
+    obj["brick_map_dims"] = dims
+    obj["voxel_size"] = vs
+    obj["sdf_type"] = L.SDF_TYPE_VOXELIZED
+    obj["accumulated_scale"] = (1, 1, 1)
+    obj["object_id"] = 3
+    cu = M.camera_look_at(cam, (0, 0, 0), 60.0, w, h)
+    su = L.scene_uniforms(num_objects=1)
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE if rel else L.EPS_ABSOLUTE, strategy=L.STRATEGY_BRUTE)
+    return O.RefInputs(b, m, obj, cu, su, ext, w, h), cu
+
+
+def test_voxelized_sphere_depth_matches_ray_sphere_intersection():
+    inp, cam = _single_sphere_scene()
+    res = O.ref_trace(inp, prec=64)
+    o = np.array(cam["position"][:3], np.float64)
+    hits = res[res["hit"] == 1]
+    assert hits.size > 100
+    for r in hits[::7]:
+        d = (r["pos"] - o) / np.linalg.norm(r["pos"] - o)
+        bq = np.dot(o, d)
+        disc = bq * bq - (np.dot(o, o) - 1.0)
+        assert disc > -1e-3
+        t_exact = -bq - math.sqrt(max(disc, 0.0))
+        assert abs(r["t"] - t_exact) < 0.05 * 0.5 + 2e-3  # O(vs) voxelisation error
+        n = r["pos"] / np.linalg.norm(r["pos"])
+        assert np.dot(n, r["normal"]) > 0.97  # normal close to (p - c)/r
+        assert r["material_id"] == 7 and (r["packed_material"] >> 24) == 3
+
+
+def test_empty_map_misses_everything():
+    inp, _ = _single_sphere_scene()
+    inp.keep[1][:] = L.EMPTY_SLOT
+    res = O.ref_trace(inp, prec=32)
+    assert res["hit"].sum() == 0
+
+
+def test_fp32_twin_agrees_with_fp64_oracle():
+    inp, _ = _single_sphere_scene(rel=True)
+    a = O.ref_trace(inp, prec=32)
+    b = O.ref_trace(inp, prec=64)
+    same = (a["hit"] == b["hit"]) & (a["steps"] == b["steps"])
+    assert same.mean() > 0.98
+    hit = same & (a["hit"] == 1)
+    rel = np.abs(a["t"][hit] - b["t"][hit]) / b["t"][hit]
+    assert rel.max() < 1e-4
+    cosang = np.clip(np.sum(a["normal"][hit] * b["normal"][hit], axis=1), -1, 1)
+    assert np.arccos(cosang).max() < 1e-3
+
+
+def test_planted_mistake_in_oracle_input_is_caught():
+    """Flip one voxel's sign in the twin's input: the hit/depth output must
+    change for the rays that cross it (the checks above can fail)."""
+    inp, _ = _single_sphere_scene()
+    base = O.ref_trace(inp, prec=32)
+    pool = inp.keep[0]
+    near = np.where(np.abs(O.f16_bits_to_f32(pool["word0"] & 0xFFFF)) < 0.02)[0]
+    pool["word0"][near] ^= 0x8000  # negate near-surface distances
+    moved = O.ref_trace(inp, prec=32)
+    assert np.any(moved["t"] != base["t"])
+
+
+# ---------------- sparse path equals the reference field ----------------
+
+def _sparse_from_dense_host(dims, m, b, vs, origin):
+    dims = [int(x) for x in dims]
+    cells = [(d + 7) // 8 for d in dims]
+    l2 = np.full(cells[0] * cells[1] * cells[2], L.EMPTY_SLOT, np.uint32)
+    l1 = []
+    for bz in range(dims[2]):
+        for by in range(dims[1]):
+            for bx in range(dims[0]):
+                e = m[bx + by * dims[0] + bz * dims[0] * dims[1]]
+                if e == L.EMPTY_SLOT:
+                    continue
+                c = (bx >> 3) + (by >> 3) * cells[0] + (bz >> 3) * cells[0] * cells[1]
+                if l2[c] == L.EMPTY_SLOT:
+                    l2[c] = len(l1)
+                    l1.append(np.full(512, L.EMPTY_SLOT, np.uint32))
+                l1[l2[c]][(bx & 7) + (by & 7) * 8 + (bz & 7) * 64] = e
+    l1 = np.concatenate(l1) if l1 else np.zeros(0, np.uint32)
+    dist = (b["word0"] & 0xFFFF).astype(np.uint16)
+    mat = ((b["word0"] >> 16) & 0x3F).astype(np.uint8)
+    return O.SparseInputs(origin, vs, cells, l2, l1, dist, mat)
+
+
+def test_sparse_field_equals_reference_field_at_origin():
+    """test_cross.rkf placed at the origin: the sparse field p - origin with
+    origin = -grid/2 is bit-identical to sample_voxelized's local + grid/2."""
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    vs = np.float32(h["voxel_size"])
+    be = vs * np.float32(8.0)
+    half = h["dims"].astype(np.float32) * be * np.float32(0.5)
+    sp = _sparse_from_dense_host(h["dims"], m, b, vs, -half)
+    obj = np.zeros(1, L.GPU_OBJECT)
+    obj["inverse_world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    obj["brick_map_dims"] = h["dims"]
+    obj["voxel_size"] = vs
+    obj["sdf_type"] = L.SDF_TYPE_VOXELIZED
+    obj["accumulated_scale"] = (1, 1, 1)
+    su = L.scene_uniforms(num_objects=1)
+    # the full trace of a small view through both layouts must agree bit for bit
+    from rkifield_amd import march as M
+    cu = M.camera_look_at((0.3, 1.0, 1.6), (0, 0, 0), 60.0, 40, 30)
+    inp = O.RefInputs(b, m, obj, cu, su, L.scene_ext(strategy=L.STRATEGY_BRUTE), 40, 30)
+    ref = O.ref_trace(inp, prec=32)
+    ext = L.scene_ext()
+    spr = O.sparse_trace(sp, cu, su, ext, 40, 30, prec=32)
+    assert ref["hit"].sum() > 100
+    assert np.array_equal(ref["hit"], spr["hit"])
+    hit = ref["hit"] == 1
+    # DDA changes only free-space steps: depths agree within the hit threshold
+    assert np.abs(ref["t"][hit] - spr["t"][hit]).max() < 2e-3
+    assert np.array_equal(ref["material_id"][hit], spr["material_id"][hit])
