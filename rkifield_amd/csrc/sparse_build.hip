@@ -29,6 +29,16 @@
 
 namespace rkf {
 
+// Grids of one block per item: 2-D so that blocks x 512 lanes stay below the
+// 2^32 work-items-per-dimension limit.
+constexpr uint32_t kGridX = 32768u;
+inline dim3 item_grid(uint64_t n) {
+    uint32_t x = n < kGridX ? (uint32_t)n : kGridX;
+    uint32_t y = (uint32_t)((n + x - 1) / x);
+    return dim3(x, y, 1);
+}
+RKF_DEV uint64_t item_index() { return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x; }
+
 struct BuildDev {
     const rkfm_prim *prims;
     const uint32_t *group_prims;   // prim indices grouped, in original order
@@ -106,9 +116,12 @@ RKF_DEV float scene_field(const BuildDev &B, uint32_t cand, float x, float y, fl
 }
 
 // One block per candidate cell, one lane per brick (x + 8y + 64z in the cell).
-__global__ __launch_bounds__(512) void alloc_flags_kernel(BuildDev B, uint8_t *flags, uint32_t *counts) {
+__global__ __launch_bounds__(512) void alloc_flags_kernel(BuildDev B, uint32_t ncand, uint8_t *flags,
+                                                          uint32_t *counts) {
     __shared__ uint32_t wsum[8];
-    uint32_t cand = blockIdx.x, t = threadIdx.x;
+    uint64_t item = item_index();
+    if (item >= ncand) return;
+    uint32_t cand = (uint32_t)item, t = threadIdx.x;
     uint32_t cell = B.cand_cell[cand];
     uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
     uint32_t gbx = qx * 8u + (t & 7u), gby = qy * 8u + ((t >> 3) & 7u), gbz = qz * 8u + (t >> 6);
@@ -131,11 +144,13 @@ __global__ __launch_bounds__(512) void alloc_flags_kernel(BuildDev B, uint8_t *f
 }
 
 // Assign slots (base + rank in z-y-x order) and write L1/L2 entries.
-__global__ __launch_bounds__(512) void assign_kernel(BuildDev B, const uint8_t *flags, const uint32_t *blk_of,
-                                                     const uint32_t *slot_base, uint32_t *l2, uint32_t *l1,
-                                                     uint32_t *slot_cand, uint32_t *slot_brick) {
+__global__ __launch_bounds__(512) void assign_kernel(BuildDev B, uint32_t ncand, const uint8_t *flags,
+                                                     const uint32_t *blk_of, const uint32_t *slot_base, uint32_t *l2,
+                                                     uint32_t *l1, uint32_t *slot_cand, uint32_t *slot_brick) {
     __shared__ uint32_t wsum[8];
-    uint32_t cand = blockIdx.x, t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint64_t item = item_index();
+    if (item >= ncand) return;
+    uint32_t cand = (uint32_t)item, t = threadIdx.x, lane = t & 63u, w = t >> 6;
     uint32_t blk = blk_of[cand];
     if (blk == RKF_EMPTY_SLOT) return;  // uniform per block
     bool need = flags[(uint64_t)cand * 512u + t] != 0;
@@ -155,9 +170,11 @@ __global__ __launch_bounds__(512) void assign_kernel(BuildDev B, const uint8_t *
 }
 
 // One block per allocated brick, one lane per voxel (populate_brick).
-__global__ __launch_bounds__(512) void fill_kernel(BuildDev B, const uint32_t *slot_cand, const uint32_t *slot_brick,
-                                                   uint16_t *dist, uint8_t *mat) {
-    uint32_t slot = blockIdx.x, v = threadIdx.x;
+__global__ __launch_bounds__(512) void fill_kernel(BuildDev B, uint64_t nbricks, const uint32_t *slot_cand,
+                                                   const uint32_t *slot_brick, uint16_t *dist, uint8_t *mat) {
+    uint64_t item = item_index();
+    if (item >= nbricks) return;
+    uint32_t slot = (uint32_t)item, v = threadIdx.x;
     uint32_t cand = slot_cand[slot], lb = slot_brick[slot];
     uint32_t cell = B.cand_cell[cand];
     uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
@@ -180,10 +197,12 @@ __global__ __launch_bounds__(512) void fill_kernel(BuildDev B, const uint32_t *s
 
 // Free masks: one block per row cell, one lane per brick b of the cell;
 // bit set = some brick of {b..b+1}^3 (clamped to the grid) is allocated.
-__global__ __launch_bounds__(512) void free_mask_kernel(SparseDev S, const uint32_t *row_cell, uint64_t *bits,
-                                                        uint32_t *row_any) {
+__global__ __launch_bounds__(512) void free_mask_kernel(SparseDev S, uint64_t nrows, const uint32_t *row_cell,
+                                                        uint64_t *bits, uint32_t *row_any) {
     __shared__ uint32_t any;
-    uint32_t row = blockIdx.x, t = threadIdx.x;
+    uint64_t item = item_index();
+    if (item >= nrows) return;
+    uint32_t row = (uint32_t)item, t = threadIdx.x;
     if (t == 0) any = 0;
     __syncthreads();
     uint32_t cell = row_cell[row];
@@ -311,8 +330,8 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
     std::vector<uint32_t> any(rows.size());
     if (!rows.empty()) {
         rkf::SparseDev D = s->dev();
-        hipLaunchKernelGGL(rkf::free_mask_kernel, dim3((unsigned)rows.size()), dim3(512), 0, c->stream, D,
-                           (const uint32_t *)d_rows->ptr, (uint64_t *)s->free_bits.ptr, (uint32_t *)d_any->ptr);
+        hipLaunchKernelGGL(rkf::free_mask_kernel, rkf::item_grid(rows.size()), dim3(512), 0, c->stream, D,
+                           (uint64_t)rows.size(), (const uint32_t *)d_rows->ptr, (uint64_t *)s->free_bits.ptr, (uint32_t *)d_any->ptr);
         rc = rkf_hip_check(hipGetLastError(), "free_mask launch");
         if (!rc)
             rc = rkf_hip_check(hipMemcpyAsync(any.data(), d_any->ptr, rows.size() * 4, hipMemcpyDeviceToHost, c->stream),
@@ -461,7 +480,8 @@ int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkf
     B.mat_seed = prm->mat_seed;
     std::vector<uint32_t> counts(ncand);
     if (ncand) {
-        hipLaunchKernelGGL(rkf::alloc_flags_kernel, dim3(ncand), dim3(512), 0, c->stream, B, (uint8_t *)d_flags->ptr,
+        hipLaunchKernelGGL(rkf::alloc_flags_kernel, rkf::item_grid(ncand), dim3(512), 0, c->stream, B, ncand,
+                           (uint8_t *)d_flags->ptr,
                            (uint32_t *)d_cnt->ptr);
         rc = rkf_hip_check(hipGetLastError(), "alloc_flags launch");
         if (!rc)
@@ -506,14 +526,15 @@ int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkf
     rc = rkf_hip_check(hipMemsetAsync(s->l2.ptr, 0xFF, ncell * 4, c->stream), "l2 clear");
     if (rc) return fail(rc);
     if (ncand) {
-        hipLaunchKernelGGL(rkf::assign_kernel, dim3(ncand), dim3(512), 0, c->stream, B, (const uint8_t *)d_flags->ptr,
+        hipLaunchKernelGGL(rkf::assign_kernel, rkf::item_grid(ncand), dim3(512), 0, c->stream, B, ncand,
+                           (const uint8_t *)d_flags->ptr,
                            (const uint32_t *)d_blk->ptr, (const uint32_t *)d_base->ptr, (uint32_t *)s->l2.ptr,
                            (uint32_t *)s->l1.ptr, (uint32_t *)d_scand->ptr, (uint32_t *)d_sbrick->ptr);
         rc = rkf_hip_check(hipGetLastError(), "assign launch");
         if (rc) return fail(rc);
     }
     if (nbricks) {
-        hipLaunchKernelGGL(rkf::fill_kernel, dim3((unsigned)nbricks), dim3(512), 0, c->stream, B,
+        hipLaunchKernelGGL(rkf::fill_kernel, rkf::item_grid(nbricks), dim3(512), 0, c->stream, B, nbricks,
                            (const uint32_t *)d_scand->ptr, (const uint32_t *)d_sbrick->ptr, (uint16_t *)s->dist.ptr,
                            (uint8_t *)s->mat.ptr);
         rc = rkf_hip_check(hipGetLastError(), "fill launch");
