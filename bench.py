@@ -109,9 +109,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config1", choices=["config1", "config0"])
-    ap.add_argument("--variant", type=int, default=0, help="sparse kernel: 0 tuned, 1 plain")
+    ap.add_argument("--variant", type=int, default=0, help="sparse kernel: 0 per-pixel, 1 persistent refill")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also launch the plain copy kernel on this many MiB (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path shards rays across GPUs with no exchange, "
@@ -129,7 +131,7 @@ def main():
         launch = lambda gb: sc.march(cam, su, ext, gb, variant=args.variant)
         workload = ("config1: 4096 sphere/box/capsule prims, size U[0.1,4] m, smin k=0.1 groups of 8, "
                     "256 m cube, vs 0.02 m two-level sparse index; 1920x1080 primary + 1 shadow ray per hit")
-        kernel = "sparse_march_tuned" if args.variant == 0 else "sparse_march_plain"
+        kernel = "sparse_march_plain" if args.variant == 0 else "sparse_march_tuned"
     else:
         sc = scenes.spheres_scene(seed=0)
         width, height = sc.width, sc.height
@@ -156,8 +158,12 @@ def main():
     nbytes = algorithmic_bytes(out["debug"], hit)
     n_primary = width * height
     n_shadow = int(hit.sum()) if args.workload == "config1" else 0
-    log(f"hits {int(hit.sum())} / {n_primary}; evals {int(out['debug']['evals'].sum())}, "
-        f"shadow evals {int(out['debug']['shadow_evals'].sum())}; algorithmic bytes {nbytes / 1e9:.3f} GB")
+    dbg = out["debug"]
+    log(f"hits {int(hit.sum())} / {n_primary}; evals {int(dbg['evals'].sum())}, "
+        f"shadow evals {int(dbg['shadow_evals'].sum())}; algorithmic bytes {nbytes / 1e9:.3f} GB")
+    log(f"steps/ray mean {dbg['steps'].mean():.2f} p99 {np.percentile(dbg['steps'], 99):.0f} max {dbg['steps'].max()}; "
+        f"evals/ray mean {dbg['evals'].mean():.2f}; hit rays: steps mean {dbg['steps'][hit].mean() if hit.any() else 0:.1f}; "
+        f"shadow steps mean {dbg['shadow_steps'][hit].mean() if hit.any() else 0:.1f}")
     del gbd
 
     gb = M.GBuffer(ctx, width, height, shadow=args.workload == "config1")
@@ -175,8 +181,17 @@ def main():
     achieved = nbytes / (ms / 1e3) / 1e9
     log(f"{kernel}: {ms:.3f} ms/launch, {mrays:.1f} Mrays/s, {achieved:.1f} GB/s algorithmic")
 
+    if args.copy_mib:
+        nb = args.copy_mib << 20
+        a = ctx.array((nb,), np.uint8)
+        b = ctx.array((nb,), np.uint8)
+        for _ in range(3):
+            ctx.copy(b, a, nb)
+        ctx.sync()
+        log(f"copy kernel: 3 x {args.copy_mib} MiB")
+
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}_traffic.json")
+    tpath =os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}_traffic.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")

@@ -341,8 +341,9 @@ int rkfm_sparse_download(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
 
 /* March a sparse scene: ray generation, sphere tracing with the reference
  * step rules, multi-level DDA over free space, normals, G-buffer write and
- * optional shadow rays. variant 0 = tuned kernel, 1 = plain one-lane-per-ray
- * kernel (same results). */
+ * optional shadow rays. variant 0 = one lane per pixel, 8x8 pixels per
+ * wavefront, hardware-scheduled; variant 1 = persistent wavefronts that
+ * refill finished lanes (same results, bit for bit). */
 int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                       const rkf_camera_uniforms *camera,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
@@ -361,13 +362,15 @@ typedef struct rkfm_point_result {
 int rkfm_sparse_query_points(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                              const float *xyz_dev, uint32_t n,
                              rkfm_point_result *out_dev, void *hip_stream);
-/* Free-space masks used by the DDA: free_idx has one u32 per L2 cell
+/* Free-space structures used by the DDA: free_idx has one u32 per L2 cell
  * (0xFFFFFFFF = every brick of the cell is free, else a row of free_bits);
  * free_bits holds 8 u64 per row, bit (x + 8y + 64z) set = that brick is not
- * free. *n_rows receives the row count (free_bits needs 8 * n_rows words). */
+ * free; prox has one u8 per L2 cell, the Chebyshev distance in cells (capped
+ * at 16) to the nearest cell that is not entirely free (0 = not free).
+ * *n_rows receives the row count (free_bits needs 8 * n_rows words). */
 int rkfm_sparse_download_free(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                               uint32_t *free_idx, uint64_t *free_bits,
-                              uint64_t *n_rows);
+                              uint8_t *prox, uint64_t *n_rows);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and timing helpers (the C ABI takes device pointers)        */
@@ -379,6 +382,10 @@ int rkfm_memcpy_h2d(rkfm_ctx *ctx, void *dst_dev, const void *src_host,
 int rkfm_memcpy_d2h(rkfm_ctx *ctx, void *dst_host, const void *src_dev,
                     uint64_t bytes);
 int rkfm_memset(rkfm_ctx *ctx, void *dst_dev, int value, uint64_t bytes);
+/* Plain copy kernel (bytes a multiple of 16) on the context stream; its
+ * measured HBM traffic calibrates profiler byte counters. */
+int rkfm_device_copy(rkfm_ctx *ctx, void *dst_dev, const void *src_dev,
+                     uint64_t bytes);
 /* HIP events on the context stream around the work enqueued in between. */
 int rkfm_timer_begin(rkfm_ctx *ctx);
 int rkfm_timer_end(rkfm_ctx *ctx, float *ms);

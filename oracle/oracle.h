@@ -156,6 +156,8 @@ typedef struct orc_sparse_scene {
     const uint16_t *dist;
     const uint8_t *mat;
     const uint8_t *free_l2;    /* per L2 cell: 1 = every brick of it is free */
+    const uint8_t *prox;       /* per L2 cell: Chebyshev distance (cells, <= 16)
+                                  to the nearest non-free cell; 0 = not free */
 } orc_sparse_scene;
 
 /* Unpacked per-ray result. For the FP32 twin all values are exactly the
@@ -190,6 +192,9 @@ void orc_ref_trace(const orc_ref_scene *s, int prec, uint64_t p0, uint64_t p1,
  * EMPTY; that test is evaluated on the fly). */
 uint64_t orc_sparse_n_bricks_global(const orc_sparse_scene *s);
 void orc_sparse_free_l2(const orc_sparse_scene *s, uint8_t *free_l2);
+/* prox from free_l2 (cap 16), by 16 relaxation passes of the 26-neighbour
+ * minimum (d = min(16, 1 + min over the 3x3x3 box)); out-of-grid = free. */
+void orc_sparse_prox(const orc_sparse_scene *s, const uint8_t *free_l2, uint8_t *prox);
 int orc_sparse_brick_free(const orc_sparse_scene *s, uint32_t bx, uint32_t by, uint32_t bz);
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,

@@ -59,7 +59,7 @@ class SparseScene(C.Structure):
         ("origin", C.c_float * 3), ("voxel_size", C.c_float),
         ("cells", C.c_uint32 * 3),
         ("l2", C.c_void_p), ("l1", C.c_void_p), ("dist", C.c_void_p), ("mat", C.c_void_p),
-        ("free_l2", C.c_void_p),
+        ("free_l2", C.c_void_p), ("prox", C.c_void_p),
     ]
 
 
@@ -272,6 +272,7 @@ class SparseInputs:
         cells = [int(c) for c in cells]
         nc = cells[0] * cells[1] * cells[2]
         self.free_l2 = np.zeros(nc, np.uint8)
+        self.prox = np.zeros(nc, np.uint8)
         s = SparseScene()
         s.origin[:] = [float(x) for x in origin]
         s.voxel_size = float(voxel_size)
@@ -281,8 +282,10 @@ class SparseInputs:
         s.dist = _p(self.dist).value
         s.mat = _p(self.mat).value
         s.free_l2 = _p(self.free_l2).value
+        s.prox = _p(self.prox).value
         self.s = s
         lib().orc_sparse_free_l2(C.byref(s), _p(self.free_l2))
+        lib().orc_sparse_prox(C.byref(s), _p(self.free_l2), _p(self.prox))
 
     def brick_free(self, bx, by, bz):
         return bool(lib().orc_sparse_brick_free(C.byref(self.s), C.c_uint32(bx), C.c_uint32(by), C.c_uint32(bz)))

@@ -122,6 +122,37 @@ void orc_sparse_free_l2(const orc_sparse_scene *s, uint8_t *free_l2) {
             }
 }
 
+void orc_sparse_prox(const orc_sparse_scene *s, const uint8_t *free_l2, uint8_t *prox) {
+    const uint32_t cx = s->cells[0], cy = s->cells[1], cz = s->cells[2];
+    const uint64_t n = (uint64_t)cx * cy * cz;
+    const uint8_t cap = 16;
+    uint8_t *cur = (uint8_t *)malloc(n), *nxt = (uint8_t *)malloc(n);
+    for (uint64_t i = 0; i < n; i++) cur[i] = free_l2[i] ? cap : 0;
+    for (int pass = 0; pass < cap; pass++) {
+        for (uint32_t z = 0; z < cz; z++)
+            for (uint32_t y = 0; y < cy; y++)
+                for (uint32_t x = 0; x < cx; x++) {
+                    uint64_t i = (uint64_t)x + (uint64_t)y * cx + (uint64_t)z * cx * cy;
+                    if (!free_l2[i]) { nxt[i] = 0; continue; }
+                    uint32_t m = 255;
+                    for (int dz = -1; dz <= 1; dz++)
+                        for (int dy = -1; dy <= 1; dy++)
+                            for (int dx = -1; dx <= 1; dx++) {
+                                int64_t xx = (int64_t)x + dx, yy = (int64_t)y + dy, zz = (int64_t)z + dz;
+                                if (xx < 0 || yy < 0 || zz < 0 || xx >= cx || yy >= cy || zz >= cz) continue;
+                                uint32_t v = cur[(uint64_t)xx + (uint64_t)yy * cx + (uint64_t)zz * cx * cy];
+                                if (v < m) m = v;
+                            }
+                    uint32_t d = m + 1u;
+                    nxt[i] = (uint8_t)(d < cap ? d : cap);
+                }
+        uint8_t *t = cur; cur = nxt; nxt = t;
+    }
+    memcpy(prox, cur, n);
+    free(cur);
+    free(nxt);
+}
+
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
                       const rkfm_scene_ext *ext, uint32_t w, uint32_t h, int prec, uint64_t p0, uint64_t p1,
                       orc_ray_result *out) {

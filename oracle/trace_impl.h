@@ -521,9 +521,12 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
  *
  * Free-space DDA: let g = (p - origin)/vs - 0.5 (the trilinear coordinate) and
  * b0 = floor(floor(g)/8). A sample's 8 corners lie in bricks {b0, b0+1}^3, so
- * if all of those are EMPTY (free_brick[b0]) the field is the EMPTY constant
- * 8 vs over the whole region floor(g/8) == b0, and if every brick of an L2
- * cell is free (free_l2) over floor(g/64) == cell. Sphere tracing through such
+ * if all of those are EMPTY (b0 "free") the field is the EMPTY constant
+ * 8 vs over the whole region floor(g/8) == b0; if every brick of an L2 cell is
+ * free, over floor(g/64) == cell; and prox[cell] = d >= 1 (the Chebyshev
+ * distance, in cells, to the nearest cell that is not entirely free, capped
+ * at 16; cells outside the grid count as free) extends that region to the
+ * cube of cells within distance d - 1. Sphere tracing through such
  * a region only takes positive, non-near steps; the march instead advances to
  * the region's exit t_x along the ray, sets prev_t = t_x, prev_dist = 8 vs
  * (the field value there) and resumes at t_x + vs/256. Each skip counts as one
@@ -613,7 +616,7 @@ static uint32_t F(sp_material)(const F(sctx) *c, V3 p) {
 
 /* DDA: returns 1 and the exit parameter when the sample at `pos` lies in a
  * free region. */
-static int F(sp_free)(const F(sctx) *c, V3 pos, V3 sd, R t, R *t_exit) {
+static int F(sp_free)(const F(sctx) *c, V3 pos, V3 sd, V3 inv, R t, R *t_exit) {
     const orc_sparse_scene *s = c->s;
     R vs = (R)s->voxel_size;
     R be = vs * (R)8.0f;
@@ -631,17 +634,23 @@ static int F(sp_free)(const F(sctx) *c, V3 pos, V3 sd, R t, R *t_exit) {
     uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * s->cells[0] + (uint64_t)q[2] * s->cells[0] * s->cells[1];
     R lo_b[3], hi_b[3];
-    if (s->free_l2[cell]) {
-        for (int a = 0; a < 3; a++) { lo_b[a] = (R)(q[a] * 64u); hi_b[a] = (R)(q[a] * 64u + 64u); }
+    uint32_t d = s->prox[cell];
+    if (d > 0) {
+        /* every cell within Chebyshev distance d-1 of this one is free */
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((R)q[a] - (R)(d - 1u)) * (R)64.0f;
+            hi_b[a] = ((R)q[a] + (R)d) * (R)64.0f;
+        }
     } else {
         if (!orc_sparse_brick_free(s, b[0], b[1], b[2])) return 0;
         for (int a = 0; a < 3; a++) { lo_b[a] = (R)(b[a] * 8u); hi_b[a] = (R)(b[a] * 8u + 8u); }
     }
     R dv[3] = {sd.x, sd.y, sd.z};
+    R iv[3] = {inv.x, inv.y, inv.z};
     R dt = K_MAX_FLOAT;
     for (int a = 0; a < 3; a++) {
         R bound = dv[a] > (R)0 ? hi_b[a] : lo_b[a];
-        R da = (bound - g[a]) * vs / dv[a];
+        R da = (bound - g[a]) * iv[a];
         dt = F(minr)(dt, da);
     }
     *t_exit = t + F(maxr)(dt, (R)0);
@@ -666,13 +675,14 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
                   FABS(dir.z) < (R)1e-10f ? (R)1e-10f : dir.z);
     R t = 0, prev_t = 0, prev_dist = K_MAX_FLOAT;
     R nudge = vs * (R)0.00390625f;
+    V3 inv = F(mk)(vs / sd.x, vs / sd.y, vs / sd.z);
     uint32_t near = 0;
     for (uint32_t step = 0; step < K_HARD_MAX; step++) {
         if (t > (R)su->max_distance || near >= su->max_steps) break;
         r->steps++;
         V3 pos = F(add)(o, F(muls)(sd, t));
         R tx;
-        if (F(sp_free)(c, pos, sd, t, &tx)) {
+        if (F(sp_free)(c, pos, sd, inv, t, &tx)) {
             prev_t = tx;
             prev_dist = vs * (R)8.0f;
             t = tx + nudge;
@@ -713,6 +723,7 @@ static R F(sp_shadow)(F(sctx) *c, V3 o, V3 L, R maxd, R k, uint32_t *steps) {
     R vs = (R)c->s->voxel_size;
     R nudge = vs * (R)0.00390625f;
     R shadow = 1, t = (R)0.005f;
+    V3 inv = F(mk)(vs / L.x, vs / L.y, vs / L.z);
     uint32_t near = 0;
     *steps = 0;
     for (uint32_t i = 0; i < 256u; i++) {
@@ -720,7 +731,7 @@ static R F(sp_shadow)(F(sctx) *c, V3 o, V3 L, R maxd, R k, uint32_t *steps) {
         (*steps)++;
         V3 p = F(add)(o, F(muls)(L, t));
         R tx;
-        if (F(sp_free)(c, p, L, t, &tx)) { t = tx + nudge; continue; }
+        if (F(sp_free)(c, p, L, inv, t, &tx)) { t = tx + nudge; continue; }
         R d = F(sp_sample)(c, p, 0, 1);
         if (d < (R)0.005f) return 0;
         if (d < (R)0.02f) {

@@ -400,6 +400,12 @@ int rkfm_memset(rkfm_ctx *c, void *dst, int value, uint64_t bytes) {
     return rkf_hip_check(hipMemsetAsync(dst, value, bytes, c->stream), "memset");
 }
 
+int rkfm_device_copy(rkfm_ctx *c, void *dst, const void *src, uint64_t bytes) {
+    if (!c || !dst || !src) return rkf_set_error(RKFM_ERR_INVALID_ARG, "device_copy: null argument");
+    if (bytes % 16u) return rkf_set_error(RKFM_ERR_INVALID_ARG, "device_copy: size must be a multiple of 16");
+    return rkf_hip_check(rkf_launch_copy(dst, src, bytes, c->stream), "copy launch");
+}
+
 // ---- timing on the context stream ----
 int rkfm_timer_begin(rkfm_ctx *c) {
     if (!c) return rkf_set_error(RKFM_ERR_INVALID_ARG, "timer: null ctx");

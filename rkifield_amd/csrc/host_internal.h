@@ -26,7 +26,7 @@ struct DevBuf {
 struct rkfm_sparse_scene {
     rkfm_ctx *owner = nullptr;
     rkfm_sparse_info info{};
-    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits;
+    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, prox;
     uint64_t n_free_rows = 0;
     rkf::SparseDev dev() const;
 };
@@ -47,6 +47,8 @@ struct rkfm_ctx {
     bool cam_set = false, scene_set = false, coarse_set = false;
     // Resident sparse scenes.
     std::vector<rkfm_sparse_scene *> scenes;
+    // Persistent-wave count of the tuned sparse kernel (occupancy x CUs).
+    uint32_t tuned_waves = 0;
 };
 
 void rkf_sparse_release_all(rkfm_ctx *c);

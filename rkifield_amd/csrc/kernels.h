@@ -59,6 +59,8 @@ struct SparseDev {
     const uint16_t *aux;       // per slot: 512 (secondary | blend << 8), or null
     const uint32_t *free_idx;  // per L2 cell: kFreeCell or row of free_bits
     const uint64_t *free_bits; // 8 words per row: bit set = brick NOT free
+    const uint8_t *prox;       // per L2 cell: Chebyshev distance (cells, <= 16)
+                               // to the nearest non-free cell; 0 = not free
     uint64_t n_cells;
 };
 
@@ -75,9 +77,12 @@ struct SparseMarchParams {
 
 extern "C" {
 hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream);
+hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream);
 hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream);
 hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream);
-hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, hipStream_t stream);
+hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,
+                                   hipStream_t stream);
+hipError_t rkf_occupancy_waves(uint32_t which, uint32_t *waves);
 hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,
                                    hipStream_t stream);
 }

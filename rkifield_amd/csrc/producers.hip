@@ -109,7 +109,24 @@ __global__ __launch_bounds__(256) void coarse_kernel(CoarseParams P) {
     P.cells[i] = (uint16_t)f2h(md);
 }
 
+// Plain device-to-device copy (16 B per lane, grid-stride): the reference
+// kernel whose measured HBM traffic calibrates the profiler's byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16) {
+    uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace rkf
+
+extern "C" hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream) {
+    uint64_t n16 = bytes / 16u;
+    if (n16 == 0) return hipSuccess;
+    uint64_t blocks = (n16 + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    hipLaunchKernelGGL(rkf::copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (uint4 *)dst, (const uint4 *)src,
+                       n16);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream) {
     hipLaunchKernelGGL(rkf::tile_cull_kernel, dim3(P->tiles_x, P->tiles_y, 1), dim3(256, 1, 1), 0, stream, *P);

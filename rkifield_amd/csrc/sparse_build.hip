@@ -229,6 +229,37 @@ __global__ __launch_bounds__(512) void free_mask_kernel(SparseDev S, uint64_t nr
     if (t == 0) row_any[row] = any;
 }
 
+// Proximity field: d = 0 for cells that are not entirely free, else
+// min(16, 1 + min over the 3x3x3 box); out-of-grid cells are ignored (free).
+constexpr uint32_t kProxCap = 16u;
+__global__ __launch_bounds__(256) void prox_init_kernel(const uint32_t *free_idx, uint64_t n, uint8_t *prox) {
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) prox[i] = free_idx[i] == kFreeCell ? (uint8_t)kProxCap : (uint8_t)0;
+}
+__global__ __launch_bounds__(256) void prox_relax_kernel(uint32_t cx, uint32_t cy, uint32_t cz, const uint8_t *cur,
+                                                         uint8_t *nxt) {
+    uint64_t n = (uint64_t)cx * cy * cz;
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    uint8_t self = cur[i];
+    if (self == 0) {
+        nxt[i] = 0;
+        return;
+    }
+    uint32_t x = (uint32_t)(i % cx), y = (uint32_t)((i / cx) % cy), z = (uint32_t)(i / ((uint64_t)cx * cy));
+    uint32_t m = 255;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int64_t xx = (int64_t)x + dx, yy = (int64_t)y + dy, zz = (int64_t)z + dz;
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= cx || yy >= cy || zz >= cz) continue;
+                uint32_t v = cur[(uint64_t)xx + (uint64_t)yy * cx + (uint64_t)zz * cx * cy];
+                m = v < m ? v : m;
+            }
+    uint32_t d = m + 1u;
+    nxt[i] = (uint8_t)(d < kProxCap ? d : kProxCap);
+}
+
 }  // namespace rkf
 
 rkf::SparseDev rkfm_sparse_scene::dev() const {
@@ -248,6 +279,7 @@ rkf::SparseDev rkfm_sparse_scene::dev() const {
     d.aux = (const uint16_t *)aux.ptr;
     d.free_idx = (const uint32_t *)free_idx.ptr;
     d.free_bits = (const uint64_t *)free_bits.ptr;
+    d.prox = (const uint8_t *)prox.ptr;
     d.n_cells = (uint64_t)info.cells[0] * info.cells[1] * info.cells[2];
     return d;
 }
@@ -260,6 +292,7 @@ static void release_scene(rkfm_sparse_scene *s) {
     s->aux.release();
     s->free_idx.release();
     s->free_bits.release();
+    s->prox.release();
 }
 
 void rkf_sparse_release_all(rkfm_ctx *c) {
@@ -345,13 +378,30 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
     s->n_free_rows = rows.size();
     rc = rkf_hip_check(hipMemcpyAsync(s->free_idx.ptr, fidx.data(), ncell * 4, hipMemcpyHostToDevice, c->stream),
                        "free idx upload");
+    if (rc) return rc;
+    // proximity field
+    DevBuf *tmpb = tmp.make();
+    rc = s->prox.reserve(ncell, false);
+    if (!rc) rc = tmpb->reserve(ncell, false);
+    if (rc) return rc;
+    s->prox.size = ncell;
+    dim3 g((unsigned)((ncell + 255) / 256)), b(256);
+    hipLaunchKernelGGL(rkf::prox_init_kernel, g, b, 0, c->stream, (const uint32_t *)s->free_idx.ptr, ncell,
+                       (uint8_t *)s->prox.ptr);
+    uint8_t *cur = (uint8_t *)s->prox.ptr, *nxt = (uint8_t *)tmpb->ptr;
+    for (uint32_t pass = 0; pass < rkf::kProxCap; pass++) {
+        hipLaunchKernelGGL(rkf::prox_relax_kernel, g, b, 0, c->stream, cells[0], cells[1], cells[2], cur, nxt);
+        std::swap(cur, nxt);
+    }
+    static_assert(rkf::kProxCap % 2u == 0u, "an even pass count leaves the result in s->prox");
+    rc = rkf_hip_check(hipGetLastError(), "prox launch");
     if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "free idx sync");
     return rc;
 }
 
 void finish_info(rkfm_sparse_scene *s) {
     s->info.device_bytes = s->l2.cap + s->l1.cap + s->dist.cap + s->mat.cap + s->aux.cap + s->free_idx.cap +
-                           s->free_bits.cap;
+                           s->free_bits.cap + s->prox.cap;
 }
 
 }  // namespace
@@ -646,7 +696,7 @@ int rkfm_sparse_download(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *l2, 
 }
 
 int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *free_idx, uint64_t *free_bits,
-                              uint64_t *n_rows) {
+                              uint8_t *prox, uint64_t *n_rows) {
     if (!c || !s || !n_rows) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_download_free: null argument");
     *n_rows = s->n_free_rows;
     int rc = rkf_hip_check(hipStreamSynchronize(c->stream), "download sync");
@@ -654,6 +704,7 @@ int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t 
         rc = rkf_hip_check(hipMemcpy(free_idx, s->free_idx.ptr, s->free_idx.size, hipMemcpyDeviceToHost), "free idx");
     if (!rc && free_bits && s->free_bits.size)
         rc = rkf_hip_check(hipMemcpy(free_bits, s->free_bits.ptr, s->free_bits.size, hipMemcpyDeviceToHost), "free bits");
+    if (!rc && prox) rc = rkf_hip_check(hipMemcpy(prox, s->prox.ptr, s->prox.size, hipMemcpyDeviceToHost), "prox");
     return rc;
 }
 
@@ -674,7 +725,12 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
     if (ext) P.ext = *ext;
     P.object_id = 0;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    return rkf_hip_check(rkf_launch_sparse_march(&P, variant, st), "sparse_march launch");
+    if (variant == 1 && !c->tuned_waves) {
+        rc = rkf_hip_check(rkf_occupancy_waves(0, &c->tuned_waves), "occupancy");
+        if (rc) return rc;
+    }
+    return rkf_hip_check(rkf_launch_sparse_march(&P, variant, c->tuned_waves, st),
+                         "sparse_march launch");
 }
 
 int rkfm_sparse_query_points(rkfm_ctx *c, const rkfm_sparse_scene *s, const float *xyz, uint32_t n,

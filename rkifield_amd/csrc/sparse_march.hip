@@ -14,7 +14,9 @@
 // g = (p - origin)/vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
 // in bricks {b0, b0+1}^3. If all of them are EMPTY the field is the constant
 // 8 vs over the region floor(g/8) == b0 (or, when the whole L2 cell is free,
-// over floor(g/64) == cell); the ray jumps to the region's exit t_x, takes
+// over floor(g/64) == cell, and over the cube of cells within Chebyshev
+// distance prox-1 of it, prox being the distance to the nearest cell that is
+// not entirely free, capped at 16); the ray jumps to the region's exit t_x, takes
 // prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip counts
 // as one loop iteration.
 #include "kernels.h"
@@ -60,10 +62,38 @@ RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count) {
     int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
     float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
     if (count) c.evals++;
-    float c000 = sp_voxel(S, x0, y0, z0, vs, grad), c100 = sp_voxel(S, x0 + 1, y0, z0, vs, grad);
-    float c010 = sp_voxel(S, x0, y0 + 1, z0, vs, grad), c110 = sp_voxel(S, x0 + 1, y0 + 1, z0, vs, grad);
-    float c001 = sp_voxel(S, x0, y0, z0 + 1, vs, grad), c101 = sp_voxel(S, x0 + 1, y0, z0 + 1, vs, grad);
-    float c011 = sp_voxel(S, x0, y0 + 1, z0 + 1, vs, grad), c111 = sp_voxel(S, x0 + 1, y0 + 1, z0 + 1, vs, grad);
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    // Fast path: all 8 corners inside one brick (no clamping, no brick
+    // crossing) -> one index walk and 8 loads from one 1 KiB distance page.
+    bool one_brick = x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + 1 < S.tot[0] && y0 + 1 < S.tot[1] &&
+                     z0 + 1 < S.tot[2] && (x0 & 7) != 7 && (y0 & 7) != 7 && (z0 & 7) != 7;
+    if (one_brick) {
+        uint32_t slot = sp_slot(S, x0, y0, z0);
+        if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) {
+            float v = slot == RKF_EMPTY_SLOT ? vs * 8.0f : (grad ? -(vs * 8.0f) : -(vs * 2.0f));
+            c000 = c100 = c010 = c110 = c001 = c101 = c011 = c111 = v;
+        } else {
+            const uint16_t *pg = S.dist + (uint64_t)slot * 512u + (uint32_t)(x0 & 7) + (uint32_t)(y0 & 7) * 8u +
+                                 (uint32_t)(z0 & 7) * 64u;
+            c000 = h2f(pg[0]);
+            c100 = h2f(pg[1]);
+            c010 = h2f(pg[8]);
+            c110 = h2f(pg[9]);
+            c001 = h2f(pg[64]);
+            c101 = h2f(pg[65]);
+            c011 = h2f(pg[72]);
+            c111 = h2f(pg[73]);
+        }
+    } else {
+        c000 = sp_voxel(S, x0, y0, z0, vs, grad);
+        c100 = sp_voxel(S, x0 + 1, y0, z0, vs, grad);
+        c010 = sp_voxel(S, x0, y0 + 1, z0, vs, grad);
+        c110 = sp_voxel(S, x0 + 1, y0 + 1, z0, vs, grad);
+        c001 = sp_voxel(S, x0, y0, z0 + 1, vs, grad);
+        c101 = sp_voxel(S, x0 + 1, y0, z0 + 1, vs, grad);
+        c011 = sp_voxel(S, x0, y0 + 1, z0 + 1, vs, grad);
+        c111 = sp_voxel(S, x0 + 1, y0 + 1, z0 + 1, vs, grad);
+    }
     float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
     float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
     float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
@@ -100,7 +130,7 @@ RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
 }
 
 // Free-space test + exit parameter (see file header).
-RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, float t, float &t_exit) {
+RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit) {
     float vs = S.vs;
     float be = vs * 8.0f;
     f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
@@ -117,17 +147,19 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, float t, float &t_exit) 
     uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
     uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
-    uint32_t row = S.free_idx[cell];
     float lo_b[3], hi_b[3];
-    if (row == kFreeCell) {
+    uint32_t d = S.prox[cell];
+    if (d > 0) {
+        // every cell within Chebyshev distance d-1 of this one is free
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            lo_b[a] = (float)(q[a] * 64u);
-            hi_b[a] = (float)(q[a] * 64u + 64u);
+            lo_b[a] = ((float)q[a] - (float)(d - 1u)) * 64.0f;
+            hi_b[a] = ((float)q[a] + (float)d) * 64.0f;
         }
     } else {
+        uint32_t row = S.free_idx[cell];
         uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
-        uint64_t w = S.free_bits[(uint64_t)row * 8u + (lb >> 6)];
+        uint64_t w = row == kFreeCell ? 0ull : S.free_bits[(uint64_t)row * 8u + (lb >> 6)];
         if ((w >> (lb & 63u)) & 1ull) return false;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -136,11 +168,12 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, float t, float &t_exit) 
         }
     }
     float dv[3] = {sd.x, sd.y, sd.z};
+    float iv[3] = {inv.x, inv.y, inv.z};
     float dt = kMaxFloat;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float bound = dv[a] > 0.0f ? hi_b[a] : lo_b[a];
-        float da = (bound - g[a]) * vs / dv[a];
+        float da = (bound - g[a]) * iv[a];
         dt = fminf(dt, da);
     }
     t_exit = t + fmaxf(dt, 0.0f);
@@ -154,108 +187,145 @@ struct SpRes {
     uint32_t steps;
 };
 
-RKF_DEV void sp_march(SpCtx &c, f3 o, f3 dir, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &r) {
-    const SparseDev &S = *c.S;
-    float vs = S.vs;
-    r.hit = false;
-    r.t = kMaxFloat;
+// Resumable primary-ray state: one iteration of ray_march_tiled's loop
+// (:905-982) per prim_step call, so the plain kernel (loop to completion) and
+// the tuned kernel (interleaved with lane refills) run identical arithmetic.
+struct PrimRay {
+    f3 o, sd;  // origin, safe direction (shadow rays: light direction)
+    f3 inv;    // vs / sd per axis: free-space exits in voxel units -> t
+    float t, prev_t, prev_dist;  // shadow rays keep their visibility in prev_dist
+    uint32_t near, steps, evals;
+};
+enum : int { kRayContinue = 0, kRayHit = 1, kRayMiss = 2 };
+
+RKF_DEV void prim_init(PrimRay &r, f3 o, f3 dir, float vs) {
+    r.o = o;
+    r.sd = safe_dir(dir);
+    r.inv = mk(vs / r.sd.x, vs / r.sd.y, vs / r.sd.z);
+    r.t = 0.0f;
+    r.prev_t = 0.0f;
+    r.prev_dist = kMaxFloat;
+    r.near = 0;
     r.steps = 0;
-    r.mat_d = 0.0f;
-    r.mat_p = mk(0.0f, 0.0f, 0.0f);
-    f3 sd = safe_dir(dir);
-    float t = 0.0f, prev_t = 0.0f, prev_dist = kMaxFloat;
-    const float nudge = vs * 0.00390625f;
-    uint32_t near = 0;
-    for (uint32_t step = 0; step < kHardMax; step++) {
-        if (t > su.max_distance || near >= su.max_steps) break;
-        r.steps++;
-        f3 pos = add(o, muls(sd, t));
-        float tx;
-        if (sp_free(S, pos, sd, t, tx)) {
-            prev_t = tx;
-            prev_dist = vs * 8.0f;
-            t = tx + nudge;
-            continue;
-        }
-        float d = sp_sample(c, pos, false, true);
-        if (d < 0.0f && prev_dist > hit_eps(eps_mode, su.hit_threshold, prev_t)) {
-            float lo = prev_t, hi = t, mt = t, md = d;
-            for (uint32_t b = 0; b < kBisect; b++) {
-                float mid = (lo + hi) * 0.5f;
-                float mdd = sp_sample(c, add(o, muls(sd, mid)), false, true);
-                if (mdd < 0.0f) {
-                    hi = mid;
-                } else {
-                    lo = mid;
-                    mt = mid;
-                    md = mdd;
-                }
+    r.evals = 0;
+}
+
+RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &out) {
+    if (r.steps >= kHardMax || r.t > su.max_distance || r.near >= su.max_steps) return kRayMiss;
+    const float vs = S.vs;
+    r.steps++;
+    f3 pos = add(r.o, muls(r.sd, r.t));
+    float tx;
+    if (sp_free(S, pos, r.sd, r.inv, r.t, tx)) {
+        r.prev_t = tx;
+        r.prev_dist = vs * 8.0f;
+        r.t = tx + vs * 0.00390625f;
+        return kRayContinue;
+    }
+    SpCtx c{&S, r.evals};
+    float d = sp_sample(c, pos, false, true);
+    if (d < 0.0f && r.prev_dist > hit_eps(eps_mode, su.hit_threshold, r.prev_t)) {
+        float lo = r.prev_t, hi = r.t, mt = r.t, md = d;
+        for (uint32_t b = 0; b < kBisect; b++) {
+            float mid = (lo + hi) * 0.5f;
+            float mdd = sp_sample(c, add(r.o, muls(r.sd, mid)), false, true);
+            if (mdd < 0.0f) {
+                hi = mid;
+            } else {
+                lo = mid;
+                mt = mid;
+                md = mdd;
             }
-            r.t = (lo + hi) * 0.5f;
-            r.hit = true;
-            r.mat_p = add(o, muls(sd, mt));
-            r.mat_d = md;
-            return;
         }
-        if (d < hit_eps(eps_mode, su.hit_threshold, t)) {
-            r.t = t;
-            r.hit = true;
-            r.mat_p = pos;
-            r.mat_d = d;
-            return;
-        }
-        if (d < kNear) near++;
-        prev_t = t;
-        prev_dist = d;
-        t += fmaxf(d, kMinStep);
+        r.evals = c.evals;
+        out.t = (lo + hi) * 0.5f;
+        out.hit = true;
+        out.mat_p = add(r.o, muls(r.sd, mt));
+        out.mat_d = md;
+        return kRayHit;
     }
+    r.evals = c.evals;
+    if (d < hit_eps(eps_mode, su.hit_threshold, r.t)) {
+        out.t = r.t;
+        out.hit = true;
+        out.mat_p = pos;
+        out.mat_d = d;
+        return kRayHit;
+    }
+    if (d < kNear) r.near++;
+    r.prev_t = r.t;
+    r.prev_dist = d;
+    r.t += fmaxf(d, kMinStep);
+    return kRayContinue;
 }
 
-RKF_DEV float sp_shadow(SpCtx &c, f3 o, f3 L, float maxd, float k, uint32_t &steps) {
-    const float nudge = c.S->vs * 0.00390625f;
-    float shadow = 1.0f, t = 0.005f;
-    uint32_t near = 0;
-    steps = 0;
-    for (uint32_t i = 0; i < 256u; i++) {
-        if (near >= 32u || t > maxd) break;
-        steps++;
-        f3 p = add(o, muls(L, t));
-        float tx;
-        if (sp_free(*c.S, p, L, t, tx)) {
-            t = tx + nudge;
-            continue;
-        }
-        float d = sp_sample(c, p, false, true);
-        if (d < 0.005f) return 0.0f;
-        if (d < 0.02f) {
-            shadow = fminf(shadow, k * d / t);
-            near++;
-        }
-        t += fmaxf(d, 0.005f);
-    }
-    return clampf(shadow, 0.0f, 1.0f);
+// Resumable soft shadow (shade_common_shading.wgsl:9-33) in the same state
+// record: sd = light direction, prev_dist = running visibility.
+using ShadowRay = PrimRay;
+
+RKF_DEV void shadow_init(ShadowRay &r, f3 o, f3 L, float vs) {
+    r.o = o;
+    r.sd = L;
+    r.inv = mk(vs / L.x, vs / L.y, vs / L.z);
+    r.t = 0.005f;
+    r.prev_t = 0.0f;
+    r.prev_dist = 1.0f;
+    r.near = 0;
+    r.steps = 0;
+    r.evals = 0;
 }
 
-// Per-pixel work shared by the plain and tuned kernels.
-RKF_DEV void sp_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
+// Returns true when finished; r.prev_dist then holds the visibility.
+RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) {
+    if (r.steps >= 256u || r.near >= 32u || r.t > maxd) {
+        r.prev_dist = clampf(r.prev_dist, 0.0f, 1.0f);
+        return true;
+    }
+    r.steps++;
+    f3 p = add(r.o, muls(r.sd, r.t));
+    float tx;
+    if (sp_free(S, p, r.sd, r.inv, r.t, tx)) {
+        r.t = tx + S.vs * 0.00390625f;
+        return false;
+    }
+    SpCtx c{&S, r.evals};
+    float d = sp_sample(c, p, false, true);
+    r.evals = c.evals;
+    if (d < 0.005f) {
+        r.prev_dist = 0.0f;
+        return true;
+    }
+    if (d < 0.02f) {
+        r.prev_dist = fminf(r.prev_dist, k * d / r.t);
+        r.near++;
+    }
+    r.t += fmaxf(d, 0.005f);
+    return false;
+}
+
+RKF_DEV float shadow_maxd(const SparseMarchParams &P) { return P.ext.shadow_max_dist > 0.0f ? P.ext.shadow_max_dist : 25.0f; }
+RKF_DEV float shadow_k(const SparseMarchParams &P) { return P.ext.shadow_k > 0.0f ? P.ext.shadow_k : 16.0f; }
+
+// Writes the G-buffer (and debug record without shadow counters) for a
+// finished primary ray; returns true and the shadow-ray origin when a shadow
+// ray is due.
+RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, const PrimRay &r, int status,
+                         const SpRes &res, f3 &shadow_o) {
     const uint32_t W = P.g.width, H = P.g.height;
     uint64_t p = (uint64_t)py * W + px;
-    SpCtx c{&P.S, 0u};
-    f3 o, dir;
-    camera_ray(P.cam, W, H, px, py, o, dir);
-    SpRes r;
-    sp_march(c, o, dir, P.su, P.ext.hit_eps_mode, r);
-    if (!r.hit) {
+    if (status != kRayHit) {
         store_miss(P.g, p);
-        store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
+        store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, r.evals);
         if (P.g.shadow) P.g.shadow[p] = 0.0f;
-        return;
+        return false;
     }
     const SparseDev &S = P.S;
-    uint32_t evals_march = c.evals;
-    f3 hp = add(o, muls(dir, r.t));
+    SpCtx c{&S, 0u};
+    f3 o, dir;
+    camera_ray(P.cam, W, H, px, py, o, dir);
+    f3 hp = add(o, muls(dir, res.t));
     uint32_t mat = 0;
-    if (r.mat_d < S.vs * 16.0f) mat = sp_material(S, r.mat_p);
+    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p);
     float e = S.vs * 2.0f;
     float gx = sp_sample(c, add(hp, mk(e, 0.0f, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(e, 0.0f, 0.0f)), true, false);
     float gy = sp_sample(c, add(hp, mk(0.0f, e, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(0.0f, e, 0.0f)), true, false);
@@ -275,33 +345,131 @@ RKF_DEV void sp_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
     uint32_t packed = (mat & 0x3Fu) | (sec << 6) | (blend << 12) | (P.object_id << 24);
     float mx, my;
     motion_vec(P.cam, W, H, px, py, hp, mx, my);
-    store_hit(P.g, p, hp, r.t, n, packed, mx, my, sdf_at);
-    uint32_t sh_steps = 0, sh_evals = 0;
-    if (P.g.shadow) {
-        float vis = 0.0f;
-        if (P.ext.shadow_enable) {
-            // The shading pass reads the normal back from the rgba16f target.
-            f3 nq = mk(h2f(f2h(n.x)), h2f(f2h(n.y)), h2f(f2h(n.z)));
-            f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
-            const float bias = 0.08f;
-            f3 so = add(add(hp, muls(nq, bias)), muls(L, bias * 0.5f));
-            float maxd = P.ext.shadow_max_dist > 0.0f ? P.ext.shadow_max_dist : 25.0f;
-            float k = P.ext.shadow_k > 0.0f ? P.ext.shadow_k : 16.0f;
-            SpCtx cs{&P.S, 0u};
-            vis = sp_shadow(cs, so, L, maxd, k, sh_steps);
-            sh_evals = cs.evals;
-        }
-        P.g.shadow[p] = vis;
+    store_hit(P.g, p, hp, res.t, n, packed, mx, my, sdf_at);
+    store_debug(P.g, p, n, slot, vox, cell, r.steps, r.evals);
+    if (!P.g.shadow) return false;
+    if (!P.ext.shadow_enable) {
+        P.g.shadow[p] = 0.0f;
+        return false;
     }
-    store_debug(P.g, p, n, slot, vox, cell, r.steps, evals_march, sh_steps, sh_evals);
+    // The shading pass reads the normal back from the rgba16f target.
+    f3 nq = mk(h2f(f2h(n.x)), h2f(f2h(n.y)), h2f(f2h(n.z)));
+    f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
+    const float bias = 0.08f;
+    shadow_o = add(add(hp, muls(nq, bias)), muls(L, bias * 0.5f));
+    return true;
 }
 
-// Plain variant: one lane per pixel, 8x8 pixels per wavefront.
+RKF_DEV void shadow_store(const SparseMarchParams &P, uint64_t p, const ShadowRay &s) {
+    P.g.shadow[p] = s.prev_dist;
+    if (P.g.debug) {
+        P.g.debug[p].shadow_steps = s.steps;
+        P.g.debug[p].shadow_evals = s.evals;
+    }
+}
+
+// Plain variant: one lane per pixel, 8x8 pixels per wavefront, each ray
+// marched to completion, then its shadow ray.
 __global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= P.g.width || py >= P.g.height) return;
-    sp_pixel(P, px, py);
+    f3 o, dir;
+    camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
+    PrimRay r;
+    prim_init(r, o, dir, P.S.vs);
+    SpRes res;
+    int st;
+    do {
+        st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+    } while (st == kRayContinue);
+    f3 so;
+    if (prim_finish(P, px, py, r, st, res, so)) {
+        ShadowRay sr;
+        shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
+        float maxd = shadow_maxd(P), k = shadow_k(P);
+        while (!shadow_step(P.S, sr, maxd, k)) {
+        }
+        shadow_store(P, (uint64_t)py * P.g.width + px, sr);
+    }
+}
+
+// Tuned variant: persistent wavefronts with per-lane ray refill and no
+// atomics. Wave w of G owns the 8x8 tiles w, w+G, w+2G, ... (tile-major ray
+// ids, so a refill takes the next rays of the same tile) and hands them to
+// its lanes from a wave-uniform cursor: a lane whose ray finished takes the
+// next ray as soon as a quarter of the wave is idle. A lane whose primary ray
+// hits turns into that pixel's shadow ray, so shadow marching fills lanes
+// instead of stalling the wave. Both ray kinds share one step: position,
+// free-space test, trilinear sample; only the bookkeeping differs. Each ray
+// runs exactly the arithmetic of prim_step / shadow_step.
+constexpr uint32_t kRefillMin = 16u;
+constexpr int kInnerSteps = 4;
+
+__global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, uint32_t n_waves) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t W = P.g.width, H = P.g.height;
+    const uint32_t tiles_x = (W + 7u) / 8u;
+    const uint32_t n_tiles = tiles_x * ((H + 7u) / 8u);
+    const uint32_t wave = blockIdx.x;
+    const bool shadows = P.g.shadow && P.ext.shadow_enable;
+    const float maxd = shadow_maxd(P), ksh = shadow_k(P);
+    const f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
+    // per-wave stream: ray s of this wave is tile (wave + (s / 64) * n_waves), slot s % 64
+    uint32_t cursor = 0;
+    bool exhausted = wave >= n_tiles;
+    // lane state: 0 idle, 1 primary, 2 shadow
+    uint32_t kind = 0;
+    PrimRay r;
+    uint32_t px = 0, py = 0;
+    while (true) {
+        uint64_t idle = __ballot(kind == 0);
+        if (!exhausted && (uint32_t)__popcll(idle) >= kRefillMin) {
+            uint32_t n = (uint32_t)__popcll(idle);
+            if (kind == 0) {
+                uint32_t sidx = cursor + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                uint32_t tile = wave + (sidx >> 6) * n_waves;
+                if (tile < n_tiles) {
+                    uint32_t k = sidx & 63u;
+                    px = (tile % tiles_x) * 8u + (k & 7u);
+                    py = (tile / tiles_x) * 8u + (k >> 3);
+                    if (px < W && py < H) {
+                        f3 o, dir;
+                        camera_ray(P.cam, W, H, px, py, o, dir);
+                        prim_init(r, o, dir, P.S.vs);
+                        kind = 1;
+                    }
+                }
+            }
+            cursor += n;
+            if (wave + (cursor >> 6) * n_waves >= n_tiles) exhausted = true;
+        }
+        if (__ballot(kind != 0) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        for (int it = 0; it < kInnerSteps && kind != 0; it++) {
+            if (kind == 1) {
+                SpRes res;
+                int st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+                if (st != kRayContinue) {
+                    f3 so;
+                    bool sh = prim_finish(P, px, py, r, st, res, so);
+                    if (sh && shadows) {
+                        shadow_init(r, so, L, P.S.vs);
+                        kind = 2;
+                    } else {
+                        kind = 0;
+                    }
+                }
+            } else {
+                if (shadow_step(P.S, r, maxd, ksh)) {
+                    shadow_store(P, (uint64_t)py * W + px, r);
+                    kind = 0;
+                }
+            }
+        }
+    }
 }
 
 // Point queries (sdf_collision.rs:38-56): distance, nearest material,
@@ -336,11 +504,32 @@ __global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const fl
 
 }  // namespace rkf
 
-extern "C" hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, hipStream_t stream) {
-    (void)variant;
-    dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
-    hipLaunchKernelGGL(rkf::sparse_march_plain, grid, dim3(64, 1, 1), 0, stream, *P);
+extern "C" hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,
+                                              hipStream_t stream) {
+    if (variant == 0) {
+        dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
+        hipLaunchKernelGGL(rkf::sparse_march_plain, grid, dim3(64, 1, 1), 0, stream, *P);
+        return hipGetLastError();
+    }
+    uint32_t tiles = ((P->g.width + 7u) / 8u) * ((P->g.height + 7u) / 8u);
+    uint32_t n = waves < tiles ? waves : tiles;
+    if (n == 0) n = 1;
+    hipLaunchKernelGGL(rkf::sparse_march_tuned, dim3(n), dim3(64), 0, stream, *P, n);
     return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_occupancy_waves(uint32_t which, uint32_t *waves) {
+    (void)which;
+    int dev = 0, blocks = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rkf::sparse_march_tuned, 64, 0);
+    if (e != hipSuccess) return e;
+    *waves = (uint32_t)(blocks > 0 ? blocks : 1) * (uint32_t)prop.multiProcessorCount;
+    return hipSuccess;
 }
 
 extern "C" hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n,

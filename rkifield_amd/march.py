@@ -70,7 +70,7 @@ _SIGS = {
     "rkfm_sparse_info_get": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_destroy": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
-    "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
@@ -78,6 +78,7 @@ _SIGS = {
     "rkfm_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, _U64]),
     "rkfm_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, _U64]),
     "rkfm_memset": (C.c_int, [_VP, _VP, C.c_int, _U64]),
+    "rkfm_device_copy": (C.c_int, [_VP, _VP, _VP, _U64]),
     "rkfm_timer_begin": (C.c_int, [_VP]),
     "rkfm_timer_end": (C.c_int, [_VP, C.POINTER(_F)]),
 }
@@ -147,6 +148,9 @@ class Context:
         ms = _F()
         _check(lib().rkfm_timer_end(self.h, C.byref(ms)))
         return ms.value
+
+    def copy(self, dst, src, nbytes):
+        _check(lib().rkfm_device_copy(self.h, dst.ptr, src.ptr, nbytes))
 
     def array(self, shape, dtype):
         return DeviceArray(self, shape, dtype)
@@ -447,11 +451,12 @@ class SparseScene:
     def download_free(self):
         nc = int(np.prod(self.info["cells"]))
         rows = np.zeros(1, np.uint64)
-        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, None, None, _p(rows)))
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, None, None, None, _p(rows)))
         fidx = np.zeros(nc, np.uint32)
         bits = np.zeros(max(int(rows[0]) * 8, 1), np.uint64)
-        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, _p(fidx), _p(bits), _p(rows)))
-        return fidx, bits
+        prox = np.zeros(nc, np.uint8)
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, _p(fidx), _p(bits), _p(prox), _p(rows)))
+        return fidx, bits, prox
 
     def march(self, cam, su, ext, gbuffer, variant=0, stream=None):
         c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)

@@ -108,7 +108,8 @@ def test_voxeliser_matches_brute_force_field(small_scene):
 def test_free_masks_match_oracle(small_scene):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)
-    fidx, bits = sc.download_free()
+    fidx, bits, prox = sc.download_free()
+    assert np.array_equal(prox, sp.prox)
     cells = [int(c) for c in sc.info["cells"]]
     nb = [c * 8 for c in cells]
     for cz in range(cells[2]):
@@ -125,7 +126,7 @@ def test_free_masks_match_oracle(small_scene):
                     assert notfree == (0 if sp.brick_free(bx, by, bz) else 1)
 
 
-@pytest.mark.parametrize("variant", [1, 0])
+@pytest.mark.parametrize("variant", [0, 1])
 def test_sparse_march_parity(ctx, small_scene, variant):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)
