@@ -103,12 +103,78 @@ def cpu_baseline_ref(sc, ext, idx, counts, cells, u, seconds):
                 sample=f"FP32 oracle twin (C, OpenMP) on the full 640x360 frame in {dt:.2f} s; host {cpu_model()}")
 
 
+def run_config2(args, ctx, M, L, scenes):
+    """64 uneven batches over the config1 scene in one graph-captured launch
+    (autotuned shape) vs the same batches launched one by one."""
+    sc, _, su, ext = build_config1(ctx, M, L, scenes, 1920, 1080)
+    views, sh_orig, pts = scenes.mixed_batches(seed=2)
+    b = M.Batch(ctx)
+    singles = []
+    for (w, h, pos, tgt) in views:
+        cam = M.camera_look_at(pos, tgt, 60.0, w, h)
+        gb = M.GBuffer(ctx, w, h, shadow=True)
+        b.add_view(sc, cam, su, ext, gb)
+        singles.append(("view", cam, gb))
+    for o in sh_orig:
+        od, vd = ctx.to_device(o), ctx.array((o.shape[0],), np.float32)
+        b.add_shadow(sc, od, vd, o.shape[0], ext)
+        sb = M.Batch(ctx)
+        sb.add_shadow(sc, od, vd, o.shape[0], ext)
+        singles.append(("shadow", sb, None))
+    for x in pts:
+        xd, rd = ctx.to_device(x), ctx.array((x.shape[0],), L.POINT_RESULT)
+        b.add_points(sc, xd, rd, x.shape[0])
+        singles.append(("points", (xd, rd, x.shape[0]), None))
+    work = b.work_items()
+    t0 = time.time()
+    best, best_ms = b.autotune(reps=3)
+    log(f"autotune: {time.time() - t0:.1f} s -> schedule {int(best['schedule'])} tile {int(best['tile_w'])}x"
+        f"{int(best['tile_h'])} waves/CU {int(best['waves_per_cu'])} ({best_ms:.3f} ms)")
+    g = b.graph(tuning=best)
+    for _ in range(args.warmup):
+        g.launch()
+    ctx.sync()
+    ctx.timer_begin()
+    for _ in range(args.steps):
+        g.launch()
+    ms = ctx.timer_end() / args.steps
+    ctx.sync()
+
+    def one_by_one():
+        for kind, a, bb in singles:
+            if kind == "view":
+                sc.march(a, su, ext, bb, variant=0)
+            elif kind == "shadow":
+                a.run(tuning=(L.SCHEDULE_GRID, 8, 8, 0))
+            else:
+                sc.query(a[0], a[1], a[2])
+    one_by_one()
+    ctx.sync()
+    ctx.timer_begin()
+    for _ in range(args.steps):
+        one_by_one()
+    ms_single = ctx.timer_end() / args.steps
+    ctx.sync()
+    log(f"config2: {work} rays+queries per launch; batched graph {ms:.3f} ms, one launch per batch {ms_single:.3f} ms")
+    line = {
+        "metric": "Mrays/s (rays + queries) of 64 uneven batches in one graph-captured launch",
+        "value": round(work / (ms / 1e3) / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+        "config": {"workload": "config2: 16 views W~U{256..1920} H~U{256..1080} (+ shadow ray per hit), "
+                               "16 shadow batches and 32 point batches of U{1k..64k}, config1 scene",
+                   "tuning": {k: int(best[k]) for k in ("schedule", "tile_w", "tile_h", "waves_per_cu")},
+                   "work_items": work, "one_launch_per_batch_ms": round(ms_single, 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config1", choices=["config1", "config0"])
+    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2"])
     ap.add_argument("--variant", type=int, default=0, help="sparse kernel: 0 per-pixel, 1 persistent refill")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -125,6 +191,10 @@ def main():
     from rkifield_amd import scenes
 
     ctx = M.Context(0)
+    if args.workload == "config2":
+        run_config2(args, ctx, M, L, scenes)
+        ctx.close()
+        return
     if args.workload == "config1":
         width, height = 1920, 1080
         sc, cam, su, ext = build_config1(ctx, M, L, scenes, width, height)

@@ -373,6 +373,66 @@ int rkfm_sparse_download_free(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                               uint8_t *prox, uint64_t *n_rows);
 
 /* ------------------------------------------------------------------------ */
+/* Batched launch: many uneven ray/query batches, over any resident sparse   */
+/* scenes, in one kernel launch (capturable in a HIP graph). Each batch is   */
+/* computed exactly as its single-batch entry point computes it.            */
+/* ------------------------------------------------------------------------ */
+#define RKFM_BATCH_VIEW 0u   /* camera view: G-buffer (+ shadow plane)        */
+#define RKFM_BATCH_SHADOW 1u /* soft-shadow rays from given origins          */
+#define RKFM_BATCH_POINTS 2u /* point queries                                */
+
+typedef struct rkfm_batch_item {
+    uint32_t kind;                    /* RKFM_BATCH_*                         */
+    uint32_t n;                       /* shadow rays / points                 */
+    const rkfm_sparse_scene *scene;
+    /* view */
+    rkf_camera_uniforms camera;
+    rkf_scene_uniforms su;
+    rkfm_scene_ext ext;               /* ext.light_dir etc. also used by shadow batches */
+    rkfm_gbuffer out;
+    /* shadow: origins (3 floats each, device) -> visibility (device) */
+    const float *origins;
+    float *visibility;
+    /* points: xyz (3 floats each, device) -> results (device) */
+    const float *xyz;
+    rkfm_point_result *results;
+} rkfm_batch_item;
+
+#define RKFM_SCHEDULE_GRID 0u       /* one wavefront per work unit           */
+#define RKFM_SCHEDULE_PERSISTENT 1u /* persistent wavefronts pull work units */
+
+/* Launch shape: views are cut into tile_w x tile_h pixel packets
+ * (tile_w * tile_h == 64, one packet per wavefront); shadow and point batches
+ * into packets of 64. waves_per_cu sizes the persistent grid. */
+typedef struct rkfm_batch_tuning {
+    uint32_t schedule;
+    uint32_t tile_w, tile_h;
+    uint32_t waves_per_cu;
+} rkfm_batch_tuning;
+
+/* Runs the batches (NULL tuning = the autotuned or default shape). */
+int rkfm_batch_run(rkfm_ctx *ctx, const rkfm_batch_item *items, uint32_t n,
+                   const rkfm_batch_tuning *tuning, void *hip_stream);
+/* Times every candidate launch shape on these batches (reps launches each)
+ * and writes the fastest to *best; *best_ms receives its time per launch. */
+int rkfm_batch_autotune(rkfm_ctx *ctx, const rkfm_batch_item *items,
+                        uint32_t n, uint32_t reps, rkfm_batch_tuning *best,
+                        float *best_ms);
+
+/* A batch set captured once into a HIP graph (work-counter reset + launch);
+ * replays re-run the same batches (their input buffers may change between
+ * replays; outputs are rewritten). */
+typedef struct rkfm_batch_graph rkfm_batch_graph;
+int rkfm_batch_graph_create(rkfm_ctx *ctx, const rkfm_batch_item *items,
+                            uint32_t n, const rkfm_batch_tuning *tuning,
+                            rkfm_batch_graph **out);
+int rkfm_batch_graph_launch(rkfm_batch_graph *g, void *hip_stream);
+int rkfm_batch_graph_destroy(rkfm_batch_graph *g);
+/* Rays + queries one run of the batches processes (views count primary
+ * pixels; shadow rays of views are not known before the run). */
+uint64_t rkfm_batch_work_items(const rkfm_batch_item *items, uint32_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and timing helpers (the C ABI takes device pointers)        */
 /* ------------------------------------------------------------------------ */
 int rkfm_alloc(rkfm_ctx *ctx, uint64_t bytes, void **dev_ptr);

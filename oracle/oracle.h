@@ -202,6 +202,8 @@ void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       uint64_t p1, orc_ray_result *out);
 void orc_sparse_query(const orc_sparse_scene *s, const float *xyz, uint64_t n,
                       int prec, rkfm_point_result *out);
+void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t n, const float L[3],
+                       float max_dist, float k, int prec, float *visibility);
 /* Sparse field value at a world point (stepping semantics). */
 double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec);
 

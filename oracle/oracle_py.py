@@ -323,6 +323,15 @@ def sparse_query(sp: SparseInputs, xyz, prec=32):
     return out
 
 
+def sparse_shadow(sp: SparseInputs, origins, light, max_dist=25.0, k=16.0, prec=32):
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    L_ = np.asarray(light, np.float32)
+    out = np.zeros(o.shape[0], np.float32)
+    lib().orc_sparse_shadow(C.byref(sp.s), _p(o), C.c_uint64(o.shape[0]), _p(L_), C.c_float(max_dist),
+                            C.c_float(k), C.c_int(prec), _p(out))
+    return out
+
+
 def sparse_sample(sp: SparseInputs, p, prec=32):
     p = np.asarray(p, np.float32)
     return lib().orc_sparse_sample(C.byref(sp.s), _p(p), C.c_int(prec))

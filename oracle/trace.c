@@ -190,6 +190,25 @@ void orc_sparse_query(const orc_sparse_scene *s, const float *xyz, uint64_t n, i
     }
 }
 
+/* soft_shadow (shade_common_shading.wgsl:9-33) from given origins toward L. */
+void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t n, const float L[3], float maxd,
+                       float k, int prec, float *vis) {
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t steps;
+        if (prec == 64) {
+            sctx_f64 c;
+            sp_init_f64(&c, s);
+            vis[i] = (float)sp_shadow_f64(&c, mk_f64(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]),
+                                          mk_f64(L[0], L[1], L[2]), maxd, k, &steps);
+        } else {
+            sctx_f32 c;
+            sp_init_f32(&c, s);
+            vis[i] = sp_shadow_f32(&c, mk_f32(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]),
+                                   mk_f32(L[0], L[1], L[2]), maxd, k, &steps);
+        }
+    }
+}
+
 double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec) {
     if (prec == 64) {
         sctx_f64 c;

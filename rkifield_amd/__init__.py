@@ -6,6 +6,8 @@ falls back to a CPU path: a missing library raises when first used.
 """
 from . import layouts  # noqa: F401
 from .march import (  # noqa: F401
+    Batch,
+    BatchGraph,
     CoarseField,
     Context,
     DeviceArray,

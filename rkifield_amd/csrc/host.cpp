@@ -107,6 +107,9 @@ int rkfm_destroy(rkfm_ctx *c) {
     c->tile_idx.release();
     c->tile_counts.release();
     c->coarse.release();
+    c->batch_items.release();
+    c->batch_begins.release();
+    c->batch_counter.release();
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);

@@ -49,6 +49,13 @@ struct rkfm_ctx {
     std::vector<rkfm_sparse_scene *> scenes;
     // Persistent-wave count of the tuned sparse kernel (occupancy x CUs).
     uint32_t tuned_waves = 0;
+    // Batched launch: device descriptors (rewritten per call, stream-ordered),
+    // their host staging copies, the persistent work counter, the tuning.
+    DevBuf batch_items, batch_begins, batch_counter;
+    std::vector<unsigned char> batch_host_items;
+    std::vector<uint32_t> batch_host_begins;
+    rkfm_batch_tuning batch_tuning{};
+    bool batch_tuned = false;
 };
 
 void rkf_sparse_release_all(rkfm_ctx *c);

@@ -1,372 +1,8 @@
-// sparse_march.hip — marching resident sparse scenes (two-level index + SoA pool).
-//
-// Field: the reference's sample_voxelized (ray_march.wgsl:353-413) for one
-// world-space object whose dense brick map is the expansion of the two-level
-// index: 8 corner voxels, EMPTY -> +8 vs, INTERIOR -> -2 vs (stepping) or
-// -8 vs (gradient, :300-315), trilinear mix x->y->z, plus the outside-grid
-// distance. Step rules: ray_march_tiled (:905-982): near-step budget
-// (d < 0.02), MIN_STEP, 8-step bisection on overstep, hit threshold
-// (absolute or relative). Normals: 6-tap central difference at 2 vs
-// (:504-513, :1003-1020). G-buffer: :1083-1111. Shadows: soft_shadow
-// (shade_common_shading.wgsl:9-33) from shade_pbr.wgsl:40's origin.
-//
-// Multi-level free-space DDA (no reference counterpart): with
-// g = (p - origin)/vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
-// in bricks {b0, b0+1}^3. If all of them are EMPTY the field is the constant
-// 8 vs over the region floor(g/8) == b0 (or, when the whole L2 cell is free,
-// over floor(g/64) == cell, and over the cube of cells within Chebyshev
-// distance prox-1 of it, prox being the distance to the nearest cell that is
-// not entirely free, capped at 16); the ray jumps to the region's exit t_x, takes
-// prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip counts
-// as one loop iteration.
-#include "kernels.h"
+// sparse_march.hip — single-view kernels over a resident sparse scene and the
+// point-query kernel. The per-ray arithmetic lives in sparse_device.h.
+#include "sparse_device.h"
 
 namespace rkf {
-
-struct SpCtx {
-    const SparseDev *S;
-    uint32_t evals;
-};
-
-RKF_DEV uint32_t sp_slot(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz) {
-    uint32_t bx = (uint32_t)cx >> 3, by = (uint32_t)cy >> 3, bz = (uint32_t)cz >> 3;
-    uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
-                    (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
-    uint32_t blk = S.l2[cell];
-    if (blk == RKF_EMPTY_SLOT) return RKF_EMPTY_SLOT;
-    return S.l1[(uint64_t)blk * 512u + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u];
-}
-
-RKF_DEV float sp_voxel(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz, float vs, bool grad) {
-    cx = clampi(cx, 0, S.tot[0] - 1);
-    cy = clampi(cy, 0, S.tot[1] - 1);
-    cz = clampi(cz, 0, S.tot[2] - 1);
-    uint32_t slot = sp_slot(S, cx, cy, cz);
-    if (slot == RKF_EMPTY_SLOT) return vs * 8.0f;
-    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * 8.0f) : -(vs * 2.0f);
-    uint64_t idx = (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
-    return h2f(S.dist[idx]);
-}
-
-RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count) {
-    const SparseDev &S = *c.S;
-    float vs = S.vs;
-    float be = vs * 8.0f;
-    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
-    f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
-    float lo = vs * 0.01f;
-    f3 cl = mk(clampf(gp.x, lo, gs.x - lo), clampf(gp.y, lo, gs.y - lo), clampf(gp.z, lo, gs.z - lo));
-    float outside = length(sub(gp, cl));
-    if (outside > be * 2.0f) return outside;
-    f3 vc = mk(cl.x / vs - 0.5f, cl.y / vs - 0.5f, cl.z / vs - 0.5f);
-    int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
-    float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
-    if (count) c.evals++;
-    float c000, c100, c010, c110, c001, c101, c011, c111;
-    // Fast path: all 8 corners inside one brick (no clamping, no brick
-    // crossing) -> one index walk and 8 loads from one 1 KiB distance page.
-    bool one_brick = x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + 1 < S.tot[0] && y0 + 1 < S.tot[1] &&
-                     z0 + 1 < S.tot[2] && (x0 & 7) != 7 && (y0 & 7) != 7 && (z0 & 7) != 7;
-    if (one_brick) {
-        uint32_t slot = sp_slot(S, x0, y0, z0);
-        if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) {
-            float v = slot == RKF_EMPTY_SLOT ? vs * 8.0f : (grad ? -(vs * 8.0f) : -(vs * 2.0f));
-            c000 = c100 = c010 = c110 = c001 = c101 = c011 = c111 = v;
-        } else {
-            const uint16_t *pg = S.dist + (uint64_t)slot * 512u + (uint32_t)(x0 & 7) + (uint32_t)(y0 & 7) * 8u +
-                                 (uint32_t)(z0 & 7) * 64u;
-            c000 = h2f(pg[0]);
-            c100 = h2f(pg[1]);
-            c010 = h2f(pg[8]);
-            c110 = h2f(pg[9]);
-            c001 = h2f(pg[64]);
-            c101 = h2f(pg[65]);
-            c011 = h2f(pg[72]);
-            c111 = h2f(pg[73]);
-        }
-    } else {
-        c000 = sp_voxel(S, x0, y0, z0, vs, grad);
-        c100 = sp_voxel(S, x0 + 1, y0, z0, vs, grad);
-        c010 = sp_voxel(S, x0, y0 + 1, z0, vs, grad);
-        c110 = sp_voxel(S, x0 + 1, y0 + 1, z0, vs, grad);
-        c001 = sp_voxel(S, x0, y0, z0 + 1, vs, grad);
-        c101 = sp_voxel(S, x0 + 1, y0, z0 + 1, vs, grad);
-        c011 = sp_voxel(S, x0, y0 + 1, z0 + 1, vs, grad);
-        c111 = sp_voxel(S, x0 + 1, y0 + 1, z0 + 1, vs, grad);
-    }
-    float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
-    float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
-    float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
-    return mix(c0, c1, tz) + outside;
-}
-
-// Nearest voxel floor(grid_pos / vs) as ray_march.wgsl:540-596. Returns the
-// pool index or ~0.
-RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o) {
-    float vs = S.vs;
-    float be = vs * 8.0f;
-    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
-    f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
-    slot_o = RKF_EMPTY_SLOT;
-    vox_o = 0xFFFFFFFFu;
-    cell_o = 0xFFFFFFFFu;
-    if (gp.x < 0.0f || gp.y < 0.0f || gp.z < 0.0f || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) return ~0ull;
-    int32_t cx = clampi((int32_t)floorf(gp.x / vs), 0, S.tot[0] - 1);
-    int32_t cy = clampi((int32_t)floorf(gp.y / vs), 0, S.tot[1] - 1);
-    int32_t cz = clampi((int32_t)floorf(gp.z / vs), 0, S.tot[2] - 1);
-    uint32_t slot = sp_slot(S, cx, cy, cz);
-    cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * S.cells[0] + ((uint32_t)cz >> 6) * S.cells[0] * S.cells[1];
-    slot_o = slot;
-    uint32_t lv = (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
-    if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) return ~0ull;
-    vox_o = lv;
-    return (uint64_t)slot * 512u + lv;
-}
-
-RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
-    uint32_t a, b, d;
-    uint64_t i = sp_nearest(S, p, a, b, d);
-    return i == ~0ull ? 0u : (uint32_t)(S.mat[i] & 0x3Fu);
-}
-
-// Free-space test + exit parameter (see file header).
-RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit) {
-    float vs = S.vs;
-    float be = vs * 8.0f;
-    f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
-    f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
-    float lo = vs * 0.01f;
-    if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
-    float g[3] = {gp.x / vs - 0.5f, gp.y / vs - 0.5f, gp.z / vs - 0.5f};
-    int32_t v0[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        v0[a] = (int32_t)floorf(g[a]);
-        if (v0[a] < 0 || v0[a] > S.tot[a] - 2) return false;
-    }
-    uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
-    uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
-    uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
-    float lo_b[3], hi_b[3];
-    uint32_t d = S.prox[cell];
-    if (d > 0) {
-        // every cell within Chebyshev distance d-1 of this one is free
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            lo_b[a] = ((float)q[a] - (float)(d - 1u)) * 64.0f;
-            hi_b[a] = ((float)q[a] + (float)d) * 64.0f;
-        }
-    } else {
-        uint32_t row = S.free_idx[cell];
-        uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
-        uint64_t w = row == kFreeCell ? 0ull : S.free_bits[(uint64_t)row * 8u + (lb >> 6)];
-        if ((w >> (lb & 63u)) & 1ull) return false;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            lo_b[a] = (float)(b[a] * 8u);
-            hi_b[a] = (float)(b[a] * 8u + 8u);
-        }
-    }
-    float dv[3] = {sd.x, sd.y, sd.z};
-    float iv[3] = {inv.x, inv.y, inv.z};
-    float dt = kMaxFloat;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        float bound = dv[a] > 0.0f ? hi_b[a] : lo_b[a];
-        float da = (bound - g[a]) * iv[a];
-        dt = fminf(dt, da);
-    }
-    t_exit = t + fmaxf(dt, 0.0f);
-    return true;
-}
-
-struct SpRes {
-    float t, mat_d;
-    f3 mat_p;
-    bool hit;
-    uint32_t steps;
-};
-
-// Resumable primary-ray state: one iteration of ray_march_tiled's loop
-// (:905-982) per prim_step call, so the plain kernel (loop to completion) and
-// the tuned kernel (interleaved with lane refills) run identical arithmetic.
-struct PrimRay {
-    f3 o, sd;  // origin, safe direction (shadow rays: light direction)
-    f3 inv;    // vs / sd per axis: free-space exits in voxel units -> t
-    float t, prev_t, prev_dist;  // shadow rays keep their visibility in prev_dist
-    uint32_t near, steps, evals;
-};
-enum : int { kRayContinue = 0, kRayHit = 1, kRayMiss = 2 };
-
-RKF_DEV void prim_init(PrimRay &r, f3 o, f3 dir, float vs) {
-    r.o = o;
-    r.sd = safe_dir(dir);
-    r.inv = mk(vs / r.sd.x, vs / r.sd.y, vs / r.sd.z);
-    r.t = 0.0f;
-    r.prev_t = 0.0f;
-    r.prev_dist = kMaxFloat;
-    r.near = 0;
-    r.steps = 0;
-    r.evals = 0;
-}
-
-RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &out) {
-    if (r.steps >= kHardMax || r.t > su.max_distance || r.near >= su.max_steps) return kRayMiss;
-    const float vs = S.vs;
-    r.steps++;
-    f3 pos = add(r.o, muls(r.sd, r.t));
-    float tx;
-    if (sp_free(S, pos, r.sd, r.inv, r.t, tx)) {
-        r.prev_t = tx;
-        r.prev_dist = vs * 8.0f;
-        r.t = tx + vs * 0.00390625f;
-        return kRayContinue;
-    }
-    SpCtx c{&S, r.evals};
-    float d = sp_sample(c, pos, false, true);
-    if (d < 0.0f && r.prev_dist > hit_eps(eps_mode, su.hit_threshold, r.prev_t)) {
-        float lo = r.prev_t, hi = r.t, mt = r.t, md = d;
-        for (uint32_t b = 0; b < kBisect; b++) {
-            float mid = (lo + hi) * 0.5f;
-            float mdd = sp_sample(c, add(r.o, muls(r.sd, mid)), false, true);
-            if (mdd < 0.0f) {
-                hi = mid;
-            } else {
-                lo = mid;
-                mt = mid;
-                md = mdd;
-            }
-        }
-        r.evals = c.evals;
-        out.t = (lo + hi) * 0.5f;
-        out.hit = true;
-        out.mat_p = add(r.o, muls(r.sd, mt));
-        out.mat_d = md;
-        return kRayHit;
-    }
-    r.evals = c.evals;
-    if (d < hit_eps(eps_mode, su.hit_threshold, r.t)) {
-        out.t = r.t;
-        out.hit = true;
-        out.mat_p = pos;
-        out.mat_d = d;
-        return kRayHit;
-    }
-    if (d < kNear) r.near++;
-    r.prev_t = r.t;
-    r.prev_dist = d;
-    r.t += fmaxf(d, kMinStep);
-    return kRayContinue;
-}
-
-// Resumable soft shadow (shade_common_shading.wgsl:9-33) in the same state
-// record: sd = light direction, prev_dist = running visibility.
-using ShadowRay = PrimRay;
-
-RKF_DEV void shadow_init(ShadowRay &r, f3 o, f3 L, float vs) {
-    r.o = o;
-    r.sd = L;
-    r.inv = mk(vs / L.x, vs / L.y, vs / L.z);
-    r.t = 0.005f;
-    r.prev_t = 0.0f;
-    r.prev_dist = 1.0f;
-    r.near = 0;
-    r.steps = 0;
-    r.evals = 0;
-}
-
-// Returns true when finished; r.prev_dist then holds the visibility.
-RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) {
-    if (r.steps >= 256u || r.near >= 32u || r.t > maxd) {
-        r.prev_dist = clampf(r.prev_dist, 0.0f, 1.0f);
-        return true;
-    }
-    r.steps++;
-    f3 p = add(r.o, muls(r.sd, r.t));
-    float tx;
-    if (sp_free(S, p, r.sd, r.inv, r.t, tx)) {
-        r.t = tx + S.vs * 0.00390625f;
-        return false;
-    }
-    SpCtx c{&S, r.evals};
-    float d = sp_sample(c, p, false, true);
-    r.evals = c.evals;
-    if (d < 0.005f) {
-        r.prev_dist = 0.0f;
-        return true;
-    }
-    if (d < 0.02f) {
-        r.prev_dist = fminf(r.prev_dist, k * d / r.t);
-        r.near++;
-    }
-    r.t += fmaxf(d, 0.005f);
-    return false;
-}
-
-RKF_DEV float shadow_maxd(const SparseMarchParams &P) { return P.ext.shadow_max_dist > 0.0f ? P.ext.shadow_max_dist : 25.0f; }
-RKF_DEV float shadow_k(const SparseMarchParams &P) { return P.ext.shadow_k > 0.0f ? P.ext.shadow_k : 16.0f; }
-
-// Writes the G-buffer (and debug record without shadow counters) for a
-// finished primary ray; returns true and the shadow-ray origin when a shadow
-// ray is due.
-RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, const PrimRay &r, int status,
-                         const SpRes &res, f3 &shadow_o) {
-    const uint32_t W = P.g.width, H = P.g.height;
-    uint64_t p = (uint64_t)py * W + px;
-    if (status != kRayHit) {
-        store_miss(P.g, p);
-        store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, r.evals);
-        if (P.g.shadow) P.g.shadow[p] = 0.0f;
-        return false;
-    }
-    const SparseDev &S = P.S;
-    SpCtx c{&S, 0u};
-    f3 o, dir;
-    camera_ray(P.cam, W, H, px, py, o, dir);
-    f3 hp = add(o, muls(dir, res.t));
-    uint32_t mat = 0;
-    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p);
-    float e = S.vs * 2.0f;
-    float gx = sp_sample(c, add(hp, mk(e, 0.0f, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(e, 0.0f, 0.0f)), true, false);
-    float gy = sp_sample(c, add(hp, mk(0.0f, e, 0.0f)), true, false) - sp_sample(c, sub(hp, mk(0.0f, e, 0.0f)), true, false);
-    float gz = sp_sample(c, add(hp, mk(0.0f, 0.0f, e)), true, false) - sp_sample(c, sub(hp, mk(0.0f, 0.0f, e)), true, false);
-    f3 gv = mk(gx, gy, gz);
-    float len = length(gv);
-    f3 n = len < 1e-10f ? mk(0.0f, 1.0f, 0.0f) : mk(gv.x / len, gv.y / len, gv.z / len);
-    uint32_t slot, vox, cell;
-    uint64_t vi = sp_nearest(S, hp, slot, vox, cell);
-    uint32_t sec = 0, blend = 0;
-    if (S.aux && vi != ~0ull) {
-        uint32_t a = S.aux[vi];
-        sec = a & 0x3Fu;
-        blend = (a >> 8) & 0xFFu;
-    }
-    float sdf_at = sp_sample(c, hp, false, false);
-    uint32_t packed = (mat & 0x3Fu) | (sec << 6) | (blend << 12) | (P.object_id << 24);
-    float mx, my;
-    motion_vec(P.cam, W, H, px, py, hp, mx, my);
-    store_hit(P.g, p, hp, res.t, n, packed, mx, my, sdf_at);
-    store_debug(P.g, p, n, slot, vox, cell, r.steps, r.evals);
-    if (!P.g.shadow) return false;
-    if (!P.ext.shadow_enable) {
-        P.g.shadow[p] = 0.0f;
-        return false;
-    }
-    // The shading pass reads the normal back from the rgba16f target.
-    f3 nq = mk(h2f(f2h(n.x)), h2f(f2h(n.y)), h2f(f2h(n.z)));
-    f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
-    const float bias = 0.08f;
-    shadow_o = add(add(hp, muls(nq, bias)), muls(L, bias * 0.5f));
-    return true;
-}
-
-RKF_DEV void shadow_store(const SparseMarchParams &P, uint64_t p, const ShadowRay &s) {
-    P.g.shadow[p] = s.prev_dist;
-    if (P.g.debug) {
-        P.g.debug[p].shadow_steps = s.steps;
-        P.g.debug[p].shadow_evals = s.evals;
-    }
-}
 
 // Plain variant: one lane per pixel, 8x8 pixels per wavefront, each ray
 // marched to completion, then its shadow ray.
@@ -374,24 +10,7 @@ __global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= P.g.width || py >= P.g.height) return;
-    f3 o, dir;
-    camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
-    PrimRay r;
-    prim_init(r, o, dir, P.S.vs);
-    SpRes res;
-    int st;
-    do {
-        st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
-    } while (st == kRayContinue);
-    f3 so;
-    if (prim_finish(P, px, py, r, st, res, so)) {
-        ShadowRay sr;
-        shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
-        float maxd = shadow_maxd(P), k = shadow_k(P);
-        while (!shadow_step(P.S, sr, maxd, k)) {
-        }
-        shadow_store(P, (uint64_t)py * P.g.width + px, sr);
-    }
+    view_pixel(P, px, py);
 }
 
 // Tuned variant: persistent wavefronts with per-lane ray refill and no
@@ -472,34 +91,13 @@ __global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, ui
     }
 }
 
-// Point queries (sdf_collision.rs:38-56): distance, nearest material,
-// inside flag, central-difference gradient at eps 0.01 with
-// normalize_or_zero (v * (1/len) when that reciprocal is finite and > 0).
+// Point queries: one lane per point.
 __global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const float *xyz, uint32_t n,
                                                            rkfm_point_result *out) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    SpCtx c{&S, 0u};
     f3 p = mk(xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]);
-    float d = sp_sample(c, p, false, false);
-    const float e = 0.01f;
-    float dx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), false, false) - sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), false, false);
-    float dy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), false, false) - sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), false, false);
-    float dz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), false, false) - sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), false, false);
-    float len = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-    float rcp = 1.0f / len;
-    rkfm_point_result r;
-    r.distance = d;
-    r.material = sp_material(S, p);
-    r.inside = d < 0.0f ? 1u : 0u;
-    if (isfinite(rcp) && rcp > 0.0f) {
-        r.gradient[0] = dx * rcp;
-        r.gradient[1] = dy * rcp;
-        r.gradient[2] = dz * rcp;
-    } else {
-        r.gradient[0] = r.gradient[1] = r.gradient[2] = 0.0f;
-    }
-    out[i] = r;
+    point_query(S, p, out[i]);
 }
 
 }  // namespace rkf

@@ -136,6 +136,20 @@ POINT_RESULT = np.dtype([
     ("gradient", "<f4", (3,)),
 ])
 
+GBUFFER_DESC = np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<u8"), ("motion", "<u8"),
+                         ("debug", "<u8"), ("shadow", "<u8"), ("width", "<u4"), ("height", "<u4")])
+
+BATCH_VIEW, BATCH_SHADOW, BATCH_POINTS = 0, 1, 2
+SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
+
+BATCH_ITEM = np.dtype([
+    ("kind", "<u4"), ("n", "<u4"), ("scene", "<u8"),
+    ("camera", CAMERA_UNIFORMS), ("su", SCENE_UNIFORMS), ("ext", SCENE_EXT), ("out", GBUFFER_DESC),
+    ("origins", "<u8"), ("visibility", "<u8"), ("xyz", "<u8"), ("results", "<u8"),
+])
+
+BATCH_TUNING = np.dtype([("schedule", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"), ("waves_per_cu", "<u4")])
+
 assert VOXEL_SAMPLE.itemsize == 8
 assert GPU_OBJECT.itemsize == 256
 assert CAMERA_UNIFORMS.itemsize == 144
@@ -146,6 +160,8 @@ assert SCENE_EXT.itemsize == 48
 assert DEBUG_PX.itemsize == 48
 assert PRIM.itemsize == 80
 assert POINT_RESULT.itemsize == 24
+assert GBUFFER_DESC.itemsize == 56
+assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
 
 
 def scene_uniforms(num_objects=0, max_steps=128, max_distance=100.0, hit_threshold=0.001):

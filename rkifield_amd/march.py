@@ -79,6 +79,12 @@ _SIGS = {
     "rkfm_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, _U64]),
     "rkfm_memset": (C.c_int, [_VP, _VP, C.c_int, _U64]),
     "rkfm_device_copy": (C.c_int, [_VP, _VP, _VP, _U64]),
+    "rkfm_batch_run": (C.c_int, [_VP, _VP, _U32, _VP, _VP]),
+    "rkfm_batch_autotune": (C.c_int, [_VP, _VP, _U32, _U32, _VP, C.POINTER(_F)]),
+    "rkfm_batch_graph_create": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
+    "rkfm_batch_graph_launch": (C.c_int, [_VP, _VP]),
+    "rkfm_batch_graph_destroy": (C.c_int, [_VP]),
+    "rkfm_batch_work_items": (_U64, [_VP, _U32]),
     "rkfm_timer_begin": (C.c_int, [_VP]),
     "rkfm_timer_end": (C.c_int, [_VP, C.POINTER(_F)]),
 }
@@ -213,9 +219,7 @@ class GBuffer:
         self.motion = ctx.array((n, 4), np.float32)
         self.debug = ctx.array((n,), L.DEBUG_PX) if debug else None
         self.shadow = ctx.array((n,), np.float32) if shadow else None
-        desc = np.zeros(1, np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<u8"),
-                                     ("motion", "<u8"), ("debug", "<u8"), ("shadow", "<u8"),
-                                     ("width", "<u4"), ("height", "<u4")]))
+        desc = np.zeros(1, L.GBUFFER_DESC)
         desc["position"] = self.position.ptr
         desc["normal"] = self.normal.ptr
         desc["material"] = self.material.ptr
@@ -467,3 +471,101 @@ class SparseScene:
 
     def query(self, xyz_dev, out_dev, n, stream=None):
         _check(lib().rkfm_sparse_query_points(self.ctx.h, self.h, _p(xyz_dev), n, _p(out_dev), stream))
+
+
+class Batch:
+    """A set of uneven batches (views, shadow rays, point queries) over
+    resident sparse scenes, run in one launch (rkfm_batch_run) or captured
+    into a HIP graph. Keeps its device buffers alive."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.items = []
+        self.keep = []
+
+    def add_view(self, scene, cam, su, ext, gbuffer):
+        it = np.zeros((), L.BATCH_ITEM)
+        it["kind"] = L.BATCH_VIEW
+        it["scene"] = scene.h
+        it["camera"] = cam
+        it["su"] = su
+        it["ext"] = ext
+        it["out"] = gbuffer.desc[0]
+        self.items.append(it)
+        self.keep.append(gbuffer)
+        return len(self.items) - 1
+
+    def add_shadow(self, scene, origins_dev, vis_dev, n, ext):
+        it = np.zeros((), L.BATCH_ITEM)
+        it["kind"] = L.BATCH_SHADOW
+        it["n"] = n
+        it["scene"] = scene.h
+        it["ext"] = ext
+        it["origins"] = origins_dev.ptr
+        it["visibility"] = vis_dev.ptr
+        self.items.append(it)
+        self.keep += [origins_dev, vis_dev]
+        return len(self.items) - 1
+
+    def add_points(self, scene, xyz_dev, out_dev, n):
+        it = np.zeros((), L.BATCH_ITEM)
+        it["kind"] = L.BATCH_POINTS
+        it["n"] = n
+        it["scene"] = scene.h
+        it["xyz"] = xyz_dev.ptr
+        it["results"] = out_dev.ptr
+        self.items.append(it)
+        self.keep += [xyz_dev, out_dev]
+        return len(self.items) - 1
+
+    def array(self):
+        return np.ascontiguousarray(np.array(self.items, L.BATCH_ITEM))
+
+    def work_items(self):
+        a = self.array()
+        return int(lib().rkfm_batch_work_items(a.ctypes.data, a.size))
+
+    def run(self, tuning=None, stream=None):
+        a = self.array()
+        t = None
+        if tuning is not None:
+            t = np.zeros(1, L.BATCH_TUNING)
+            t[0] = tuning
+        _check(lib().rkfm_batch_run(self.ctx.h, a.ctypes.data, a.size, _p(t), stream))
+
+    def autotune(self, reps=3):
+        a = self.array()
+        t = np.zeros(1, L.BATCH_TUNING)
+        ms = _F()
+        _check(lib().rkfm_batch_autotune(self.ctx.h, a.ctypes.data, a.size, reps, t.ctypes.data, C.byref(ms)))
+        return t[0], ms.value
+
+    def graph(self, tuning=None):
+        a = self.array()
+        t = None
+        if tuning is not None:
+            t = np.zeros(1, L.BATCH_TUNING)
+            t[0] = tuning
+        h = _VP()
+        _check(lib().rkfm_batch_graph_create(self.ctx.h, a.ctypes.data, a.size, _p(t), C.byref(h)))
+        return BatchGraph(self, h.value)
+
+
+class BatchGraph:
+    def __init__(self, batch, handle):
+        self.batch = batch
+        self.h = handle
+
+    def launch(self, stream=None):
+        _check(lib().rkfm_batch_graph_launch(self.h, stream))
+
+    def destroy(self):
+        if self.h:
+            lib().rkfm_batch_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -180,3 +180,25 @@ def prim_bounds(prims, pad):
     lo = (prims["centre"] - r[:, None]).min(axis=0) - pad
     hi = (prims["centre"] + r[:, None]).max(axis=0) + pad
     return lo.astype(np.float32), hi.astype(np.float32)
+
+
+def mixed_batches(seed=2, n_views=16, n_shadow=16, n_points=32, half_cube=128.0):
+    """Config-2 batch mix over a 256 m scene: views with W ~ U{256..1920},
+    H ~ U{256..1080} (camera 64 m from the centre on a random direction,
+    looking at a point U[-16,16]^3), shadow batches and point batches of
+    U{1k..64k} rays/points with origins/points U in the scene box. Draw order:
+    all views (W, H, direction, target), then shadow sizes, then point sizes."""
+    rng = Pcg32(seed)
+    views = []
+    for _ in range(n_views):
+        w = 256 + int(rng.uniform(0, 1665))
+        h = 256 + int(rng.uniform(0, 825))
+        d = rng.unit_vector()
+        tgt = [rng.uniform(-16, 16) for _ in range(3)]
+        views.append((w, h, d * 64.0, tgt))
+    shadows = [1024 + int(rng.uniform(0, 64 * 1024 - 1024 + 1)) for _ in range(n_shadow)]
+    points = [1024 + int(rng.uniform(0, 64 * 1024 - 1024 + 1)) for _ in range(n_points)]
+    nrng = np.random.default_rng(seed)
+    shadow_origins = [nrng.uniform(-half_cube, half_cube, (n, 3)).astype(np.float32) for n in shadows]
+    point_xyz = [nrng.uniform(-half_cube, half_cube, (n, 3)).astype(np.float32) for n in points]
+    return views, shadow_origins, point_xyz
