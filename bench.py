@@ -15,7 +15,8 @@ entry), 6 * 36 + 20 B per hit (normal, material/blend voxel, map), 44 B
 G-buffer write per pixel, 4 B per shadow-ray result.
 
 --workload config0 runs the drop-in kernel on the reference layouts instead
-(64 voxelised spheres, 640x360).
+(64 voxelised spheres, 640x360); config2 the batched graph-captured launch;
+config3 the served request stream.
 """
 import argparse
 import json
@@ -169,12 +170,109 @@ def run_config2(args, ctx, M, L, scenes):
     print(json.dumps(line), flush=True)
 
 
+def run_config3(args, ctx, M, L, scenes):
+    """Served mixed stream over 8 resident config1-type scenes (seeds 1-8):
+    Poisson arrivals at 2000 requests/s (70% point batches U{1k..64k}, 25%
+    512x512 views, 5% 1920x1080 views, each view with a shadow ray per hit),
+    admitted by a 4 GiB memory budget. The same trace is replayed by a native
+    client against the batched server and against the per-request policy
+    (one plain launch per request), first at its arrival times (latency), then
+    all at once (saturation throughput). Point results are copied to pinned
+    host memory; view G-buffers stay in HBM for their GPU consumer and are
+    released on completion."""
+    n_scenes = 8
+    resident = []
+    for seed in range(1, n_scenes + 1):
+        prims = scenes.primitive_field(seed=seed, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8)
+        lo, hi = scenes.prim_bounds(prims, 1.0)
+        resident.append(M.SparseScene.build(ctx, prims, 0.02, lo, hi, smin_k=0.1,
+                                            mat_seed=scenes.material_seed(seed)))
+    ctx.sync()
+    gib = sum(float(s.info["device_bytes"]) for s in resident) / 2**30
+    log(f"config3: {n_scenes} resident scenes, {sum(int(s.info['n_bricks']) for s in resident)} bricks, {gib:.1f} GiB")
+    trace = scenes.served_trace(seed=3, n=args.requests, rate=2000.0, n_scenes=n_scenes)
+    pool_n, ring_n, cap = 64, 256, 64 * 1024
+    rng = np.random.default_rng(3)
+    pool = M.PinnedArray((pool_n, cap, 3), np.float32)
+    pool.a[:] = rng.uniform(-128.0, 128.0, (pool_n, cap, 3))
+    ring = M.PinnedArray((ring_n, cap), L.POINT_RESULT)
+    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
+    light = np.array([0.5, 1.0, 0.3])
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    reqs = np.zeros(len(trace), L.REQUEST)
+    arrivals = np.zeros(len(trace))
+    work = 0
+    for i, (t, kind, sc, p) in enumerate(trace):
+        arrivals[i] = t
+        if kind == "points":
+            r = reqs[i]
+            r["kind"], r["scene"], r["n"] = L.BATCH_POINTS, sc, p
+            r["in"] = pool.ptr + (i % pool_n) * cap * 12
+            r["out"] = ring.ptr + (i % ring_n) * cap * L.POINT_RESULT.itemsize
+            work += p
+        else:
+            w, h, pos, tgt = p
+            cam = M.camera_look_at(pos, tgt, 60.0, w, h)
+            reqs[i] = M.Server.view_request(sc, cam, su, ext, w, h, keep_on_device=True)
+            work += w * h
+    offered = work / arrivals[-1] / 1e6
+    log(f"trace: {len(trace)} requests over {arrivals[-1]:.2f} s, {work / len(trace) / 1e3:.1f} k rays+queries "
+        f"per request, offered {offered:.1f} Mrays/s")
+    results = {}
+    for name, policy in (("served", L.SERVE_BATCHED), ("plain", L.SERVE_PER_REQUEST)):
+        srv = M.Server(ctx, resident, memory_budget=4 << 30, max_requests=ring_n, rounds_in_flight=2,
+                       policy=policy, max_pending=len(trace) + 16)
+        srv.replay(reqs[:64], np.zeros(64))  # warm-up
+        lat, wall = srv.replay(reqs, arrivals)
+        st = srv.stats()
+        ok = np.isfinite(lat)
+        lat_ms = lat[ok] * 1e3
+        burst_lat, burst_wall = srv.replay(reqs, np.zeros(len(trace)))
+        st2 = srv.stats()
+        srv.close()
+        r = dict(p50_ms=float(np.percentile(lat_ms, 50)), p99_ms=float(np.percentile(lat_ms, 99)),
+                 mean_ms=float(lat_ms.mean()), max_ms=float(lat_ms.max()), failed=int((~ok).sum()),
+                 mrays_s_at_2k=work / wall / 1e6, rounds=int(st["rounds"]),
+                 saturation_mrays_s=work / burst_wall / 1e6, saturation_s=burst_wall,
+                 saturation_rounds=int(st2["rounds"] - st["rounds"]), gpu_s=float(st2["gpu_seconds"]),
+                 arena_peak_mib=float(st2["arena_peak"]) / 2**20)
+        for kind in ("points", "view512", "view1080"):
+            sel = np.array([(k == "points") if kind == "points" else (k == "view" and v[0] == (512 if kind == "view512" else 1920))
+                            for _, k, _, v in trace]) & ok
+            if sel.any():
+                r[f"p99_ms_{kind}"] = float(np.percentile(lat[sel] * 1e3, 99))
+        results[name] = r
+        log(f"{name}: p50 {r['p50_ms']:.3f} ms p99 {r['p99_ms']:.3f} ms (points {r.get('p99_ms_points', 0):.3f}, "
+            f"512^2 {r.get('p99_ms_view512', 0):.3f}, 1080p {r.get('p99_ms_view1080', 0):.3f}); "
+            f"{r['mrays_s_at_2k']:.1f} Mrays/s at 2k req/s in {r['rounds']} rounds; saturation "
+            f"{r['saturation_mrays_s']:.1f} Mrays/s ({r['saturation_s']:.3f} s, {r['saturation_rounds']} rounds); "
+            f"arena peak {r['arena_peak_mib']:.0f} MiB")
+    ratio = results["served"]["saturation_mrays_s"] / results["plain"]["saturation_mrays_s"]
+    line = {
+        "metric": "served Mrays/s (rays + queries) at saturation; p50/p99 latency at 2000 req/s, served vs plain",
+        "value": round(results["served"]["saturation_mrays_s"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(results["served"]["saturation_s"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+        "config": {"workload": "config3: 8 resident config1-type scenes (seeds 1-8); Poisson 2000 req/s; 70% point "
+                               "batches U{1k..64k}, 25% 512x512 views, 5% 1920x1080 views (+ shadow ray per hit); "
+                               "4 GiB admission budget",
+                   "requests": len(trace), "work_items": int(work), "offered_mrays_s": round(offered, 2),
+                   "served_vs_plain_saturation": round(ratio, 3),
+                   "served": {k: round(v, 4) if isinstance(v, float) else v for k, v in results["served"].items()},
+                   "plain": {k: round(v, 4) if isinstance(v, float) else v for k, v in results["plain"].items()}},
+    }
+    print(json.dumps(line), flush=True)
+    for s in resident:
+        s.destroy()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2"])
+    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2", "config3"])
+    ap.add_argument("--requests", type=int, default=4000, help="config3: requests in the arrival trace")
     ap.add_argument("--variant", type=int, default=0, help="sparse kernel: 0 per-pixel, 1 persistent refill")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -191,8 +289,8 @@ def main():
     from rkifield_amd import scenes
 
     ctx = M.Context(0)
-    if args.workload == "config2":
-        run_config2(args, ctx, M, L, scenes)
+    if args.workload in ("config2", "config3"):
+        (run_config2 if args.workload == "config2" else run_config3)(args, ctx, M, L, scenes)
         ctx.close()
         return
     if args.workload == "config1":

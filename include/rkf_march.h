@@ -433,8 +433,123 @@ int rkfm_batch_graph_destroy(rkfm_batch_graph *g);
 uint64_t rkfm_batch_work_items(const rkfm_batch_item *items, uint32_t n);
 
 /* ------------------------------------------------------------------------ */
+/* Request server: scenes stay resident, ray/query requests arrive from any  */
+/* host thread, are admitted against a device-memory budget and run in      */
+/* launch rounds that take every admitted request (continuous batching:     */
+/* up to rounds_in_flight rounds overlap on the device, and requests that   */
+/* arrive meanwhile join the next round as slots and memory free up). No    */
+/* reference counterpart: the reference renders one view per frame on its   */
+/* engine thread (engine_loop.rs:70-90) and answers spatial queries on the  */
+/* CPU (rkf-editor/src/engine/query.rs:108-190).                           */
+/* ------------------------------------------------------------------------ */
+typedef struct rkfm_server rkfm_server;
+
+#define RKFM_SERVE_BATCHED 0u     /* one batched launch per round           */
+#define RKFM_SERVE_PER_REQUEST 1u /* one plain launch per request, one at a */
+                                  /* time (the per-request baseline)        */
+
+typedef struct rkfm_server_config {
+    uint64_t memory_budget;    /* device bytes for request inputs/outputs,   */
+                               /* allocated once (0 = the context's budget,  */
+                               /* else 1 GiB)                               */
+    uint32_t max_requests;     /* admitted-request slots (0 = 1024)          */
+    uint32_t rounds_in_flight; /* 1..4 (0 = 2); PER_REQUEST forces 1         */
+    uint64_t max_round_work;   /* rays + queries per round (0 = no cap; a    */
+                               /* round always takes at least one request)  */
+    uint32_t max_round_requests; /* requests per round (0 = 512)             */
+    uint32_t policy;           /* RKFM_SERVE_*                               */
+    rkfm_batch_tuning tuning;  /* batched launch shape (zero = grid 8x8)     */
+    uint32_t max_pending;      /* submitted, not yet admitted (0 = 65536)    */
+    uint32_t _reserved;
+} rkfm_server_config;
+
+/* Request flag: a view's G-buffer stays in the server's memory until
+ * rkfm_server_release (read it with rkfm_server_view_planes); otherwise it is
+ * copied to view_out's host planes. */
+#define RKFM_REQ_KEEP_ON_DEVICE 1u
+
+typedef struct rkfm_request {
+    uint32_t kind;             /* RKFM_BATCH_VIEW / _SHADOW / _POINTS        */
+    uint32_t scene;            /* index into the server's scene table        */
+    uint32_t n;                /* shadow rays / points                       */
+    uint32_t flags;            /* RKFM_REQ_*                                 */
+    uint32_t width, height;    /* views                                      */
+    rkf_camera_uniforms camera;
+    rkf_scene_uniforms su;
+    rkfm_scene_ext ext;        /* views: shadow plane written iff            */
+                               /* ext.shadow_enable; shadow batches: light   */
+    const float *in;           /* host: shadow origins / points (3 floats)   */
+    void *out;                 /* host: float visibility[n] /                */
+                               /* rkfm_point_result[n]                       */
+    rkfm_gbuffer view_out;     /* views: host planes (NULL plane = skipped;  */
+                               /* width/height ignored)                     */
+    uint64_t tag;              /* caller's value, returned in the status     */
+} rkfm_request;
+/* Host buffers must stay valid until the request is done; memory from
+ * rkfm_host_alloc (pinned) keeps the copies asynchronous. */
+
+#define RKFM_REQ_QUEUED 0u   /* waiting for memory / a slot                  */
+#define RKFM_REQ_ADMITTED 1u /* holds its memory, waiting for a round        */
+#define RKFM_REQ_RUNNING 2u  /* in a launched round                          */
+#define RKFM_REQ_DONE 3u
+#define RKFM_REQ_FAILED 4u
+
+typedef struct rkfm_request_status {
+    uint32_t state;            /* RKFM_REQ_*                                 */
+    int32_t error;             /* status code if FAILED                      */
+    uint64_t tag;
+    uint64_t round;            /* launch round that ran it                   */
+    uint32_t round_requests;   /* requests in that round                     */
+    uint32_t _pad;
+    double t_submit, t_admit, t_launch, t_done; /* seconds, rkfm_server_now  */
+} rkfm_request_status;
+
+typedef struct rkfm_server_stats {
+    uint64_t submitted, completed, failed, rounds;
+    uint64_t work_items;       /* rays + queries of completed requests       */
+    uint64_t arena_bytes, arena_in_use, arena_peak;
+    uint32_t max_round_requests_seen, queued, admitted, running;
+    double gpu_seconds;        /* sum of round durations (device events)     */
+} rkfm_server_stats;
+
+/* The scenes must stay resident (not destroyed) while the server runs. The
+ * server owns a worker thread and its own streams; the context must outlive
+ * it. */
+int rkfm_server_create(rkfm_ctx *ctx, const rkfm_sparse_scene *const *scenes,
+                       uint32_t n_scenes, const rkfm_server_config *cfg,
+                       rkfm_server **out);
+/* Thread-safe. RKFM_ERR_OOM: the request can never fit the budget;
+ * RKFM_ERR_BUSY: max_pending requests are already queued. */
+int rkfm_server_submit(rkfm_server *s, const rkfm_request *req, uint64_t *id);
+int rkfm_server_status(rkfm_server *s, uint64_t id, rkfm_request_status *st);
+/* Waits until the request is DONE or FAILED; RKFM_ERR_NOT_READY after
+ * timeout_s seconds (negative = no limit). */
+int rkfm_server_wait(rkfm_server *s, uint64_t id, double timeout_s,
+                     rkfm_request_status *st);
+/* Device planes of a DONE view kept on the device. */
+int rkfm_server_view_planes(rkfm_server *s, uint64_t id, rkfm_gbuffer *planes);
+/* Forgets a DONE or FAILED request and frees what it still holds. Every
+ * request is released once its results are consumed. */
+int rkfm_server_release(rkfm_server *s, uint64_t id);
+int rkfm_server_stats_get(rkfm_server *s, rkfm_server_stats *st);
+double rkfm_server_now(rkfm_server *s);
+/* Finishes the launched rounds, fails what is still queued, joins. */
+int rkfm_server_destroy(rkfm_server *s);
+
+/* Replays an arrival trace from a native client thread: request i is
+ * submitted at arrival_s[i] seconds after the call (arrival_s ascending),
+ * waited for and released; latency_s[i] = completion - arrival (NaN if it
+ * failed). *wall_s receives the time from the call to the last completion. */
+int rkfm_server_replay(rkfm_server *s, const rkfm_request *reqs,
+                       const double *arrival_s, uint32_t n, double *latency_s,
+                       double *wall_s);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and timing helpers (the C ABI takes device pointers)        */
 /* ------------------------------------------------------------------------ */
+/* Pinned host memory (asynchronous copies to and from the device). */
+int rkfm_host_alloc(uint64_t bytes, void **host_ptr);
+int rkfm_host_free(void *host_ptr);
 int rkfm_alloc(rkfm_ctx *ctx, uint64_t bytes, void **dev_ptr);
 int rkfm_free(rkfm_ctx *ctx, void *dev_ptr);
 int rkfm_memcpy_h2d(rkfm_ctx *ctx, void *dst_dev, const void *src_host,
@@ -463,6 +578,11 @@ static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes")
 static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
 static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
 static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
+static_assert(sizeof(rkfm_batch_item) == 312, "batch item record");
+static_assert(sizeof(rkfm_request) == 312, "request record");
+static_assert(sizeof(rkfm_server_config) == 56, "server config record");
+static_assert(sizeof(rkfm_request_status) == 64, "request status record");
+static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 #else
 _Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
 _Static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
@@ -470,6 +590,11 @@ _Static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes"
 _Static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
 _Static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
 _Static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
+_Static_assert(sizeof(rkfm_batch_item) == 312, "batch item record");
+_Static_assert(sizeof(rkfm_request) == 312, "request record");
+_Static_assert(sizeof(rkfm_server_config) == 56, "server config record");
+_Static_assert(sizeof(rkfm_request_status) == 64, "request status record");
+_Static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 #endif
 
 #endif /* RKF_MARCH_H */

@@ -13,8 +13,10 @@ from .march import (  # noqa: F401
     DeviceArray,
     GBuffer,
     GpuScene,
+    PinnedArray,
     RayMarchPass,
     RkfMarchError,
+    Server,
     SparseScene,
     TileObjectCullPass,
     build_bvh,

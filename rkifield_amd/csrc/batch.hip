@@ -24,15 +24,6 @@
 
 namespace rkf {
 
-struct BatchDev {
-    SparseMarchParams P;  // scene, camera, uniforms, extension, outputs
-    uint32_t kind, n, unit_begin, n_units;
-    uint32_t tile_w, tile_h, tiles_x, pad;
-    const float *in;
-    float *vis;
-    rkfm_point_result *res;
-};
-
 RKF_DEV uint32_t find_batch(const uint32_t *begins, uint32_t n_items, uint32_t unit) {
     uint32_t lo = 0, hi = n_items;
     while (hi - lo > 1u) {
@@ -103,22 +94,26 @@ struct rkfm_batch_graph {
 
 namespace {
 
-bool valid_shape(const rkfm_batch_tuning &t) {
+bool owned(const rkfm_ctx *c, const rkfm_sparse_scene *s) {
+    return std::find(c->scenes.begin(), c->scenes.end(), s) != c->scenes.end();
+}
+
+}  // namespace
+
+bool rkf_batch_valid_shape(const rkfm_batch_tuning &t) {
     if (t.schedule > RKFM_SCHEDULE_PERSISTENT) return false;
     if (t.tile_w == 0 || t.tile_h == 0 || t.tile_w * t.tile_h != 64u) return false;
     if (t.schedule == RKFM_SCHEDULE_PERSISTENT && (t.waves_per_cu == 0 || t.waves_per_cu > 64)) return false;
     return true;
 }
 
-bool owned(const rkfm_ctx *c, const rkfm_sparse_scene *s) {
-    return std::find(c->scenes.begin(), c->scenes.end(), s) != c->scenes.end();
-}
-
-// Host descriptors for a batch set; returns the unit total.
-int prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning &t,
+// Host descriptors for a batch set; returns the unit total. c == nullptr
+// skips the check that every scene is resident in the context (the caller
+// has checked it).
+int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning &t,
             std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total) {
     if (!items || n == 0) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch: no items");
-    if (!valid_shape(t)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch: invalid launch shape");
+    if (!rkf_batch_valid_shape(t)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch: invalid launch shape");
     dev.assign(n, rkf::BatchDev{});
     begins.assign(n, 0);
     uint64_t acc = 0;
@@ -126,7 +121,7 @@ int prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_ba
         const rkfm_batch_item &it = items[i];
         rkf::BatchDev &d = dev[i];
         memset(&d, 0, sizeof(d));
-        if (!it.scene || !owned(c, it.scene))
+        if (!it.scene || (c && !owned(c, it.scene)))
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: scene not resident in this context", i);
         if (it.ext.hit_eps_mode > RKFM_EPS_RELATIVE)
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: hit_eps_mode", i);
@@ -170,7 +165,7 @@ int prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_ba
     return RKFM_OK;
 }
 
-int device_waves(uint32_t per_cu, uint32_t &waves) {
+int rkf_device_waves(uint32_t per_cu, uint32_t &waves) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return rkf_hip_check(e, "hipGetDevice");
@@ -182,8 +177,8 @@ int device_waves(uint32_t per_cu, uint32_t &waves) {
 }
 
 // Enqueues reset (persistent only) + kernel on `st`; `waves` is the
-// persistent grid (device_waves, computed outside any capture).
-int enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32_t *begins, uint32_t n,
+// persistent grid (rkf_device_waves, computed outside any capture).
+int rkf_batch_enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32_t *begins, uint32_t n,
             uint32_t total, uint32_t *counter, uint32_t waves, hipStream_t st) {
     if (total == 0) return RKFM_OK;
     if (t.schedule == RKFM_SCHEDULE_GRID) {
@@ -200,7 +195,7 @@ int enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32
     return rkf_hip_check(hipGetLastError(), "batch launch");
 }
 
-rkfm_batch_tuning default_tuning(const rkfm_ctx *c) {
+rkfm_batch_tuning rkf_batch_default_tuning(const rkfm_ctx *c) {
     if (c->batch_tuned) return c->batch_tuning;
     rkfm_batch_tuning t;
     t.schedule = RKFM_SCHEDULE_GRID;
@@ -209,8 +204,6 @@ rkfm_batch_tuning default_tuning(const rkfm_ctx *c) {
     t.waves_per_cu = 16;
     return t;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -224,18 +217,18 @@ uint64_t rkfm_batch_work_items(const rkfm_batch_item *items, uint32_t n) {
 int rkfm_batch_run(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning *tuning,
                    void *stream) {
     if (!c) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch_run: null ctx");
-    rkfm_batch_tuning t = tuning ? *tuning : default_tuning(c);
+    rkfm_batch_tuning t = tuning ? *tuning : rkf_batch_default_tuning(c);
     std::vector<rkf::BatchDev> dev;
     std::vector<uint32_t> begins;
     uint32_t total = 0;
-    int rc = prepare(c, items, n, t, dev, begins, total);
+    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     // The descriptors are rewritten per call: order against earlier launches
     // that read them (they ran on this stream or were synchronised).
     uint32_t waves = 0;
     if (t.schedule == RKFM_SCHEDULE_PERSISTENT) {
-        rc = device_waves(t.waves_per_cu, waves);
+        rc = rkf_device_waves(t.waves_per_cu, waves);
         if (rc) return rc;
     }
     const uint64_t ibytes = dev.size() * sizeof(rkf::BatchDev);
@@ -254,7 +247,7 @@ int rkfm_batch_run(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const 
                                           c->batch_host_begins.size() * 4, hipMemcpyHostToDevice, st),
                            "batch begins");
     if (rc) return rc;
-    return enqueue(t, (const rkf::BatchDev *)c->batch_items.ptr, (const uint32_t *)c->batch_begins.ptr, n, total,
+    return rkf_batch_enqueue(t, (const rkf::BatchDev *)c->batch_items.ptr, (const uint32_t *)c->batch_begins.ptr, n, total,
                    (uint32_t *)c->batch_counter.ptr, waves, st);
 }
 
@@ -296,15 +289,15 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
                             rkfm_batch_graph **out) {
     if (!c || !out) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch_graph_create: null argument");
     *out = nullptr;
-    rkfm_batch_tuning t = tuning ? *tuning : default_tuning(c);
+    rkfm_batch_tuning t = tuning ? *tuning : rkf_batch_default_tuning(c);
     std::vector<rkf::BatchDev> dev;
     std::vector<uint32_t> begins;
     uint32_t total = 0;
-    int rc = prepare(c, items, n, t, dev, begins, total);
+    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total);
     if (rc) return rc;
     uint32_t waves = 0;
     if (t.schedule == RKFM_SCHEDULE_PERSISTENT) {
-        rc = device_waves(t.waves_per_cu, waves);
+        rc = rkf_device_waves(t.waves_per_cu, waves);
         if (rc) return rc;
     }
     rkfm_batch_graph *g = new rkfm_batch_graph();
@@ -322,7 +315,7 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
     if (!rc) rc = rkf_hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
     if (!rc) rc = rkf_hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
     if (!rc) {
-        int rc2 = enqueue(t, (const rkf::BatchDev *)g->items.ptr, (const uint32_t *)g->begins.ptr, n, total,
+        int rc2 = rkf_batch_enqueue(t, (const rkf::BatchDev *)g->items.ptr, (const uint32_t *)g->begins.ptr, n, total,
                           (uint32_t *)g->counter.ptr, waves, cap);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(cap, &graph);

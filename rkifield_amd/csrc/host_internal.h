@@ -60,6 +60,19 @@ struct rkfm_ctx {
 
 void rkf_sparse_release_all(rkfm_ctx *c);
 
+// Batched launch internals (batch.hip), shared with the request server.
+bool rkf_batch_valid_shape(const rkfm_batch_tuning &t);
+rkfm_batch_tuning rkf_batch_default_tuning(const rkfm_ctx *c);
+// Device descriptors and unit begins of a batch set; *total = work units.
+int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning &t,
+                      std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total);
+// Persistent-grid size: per_cu wavefronts on every CU of the current device.
+int rkf_device_waves(uint32_t per_cu, uint32_t &waves);
+// Enqueues the batch kernel (and the counter reset of the persistent
+// schedule) over device copies of the descriptors and begins.
+int rkf_batch_enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32_t *begins, uint32_t n,
+                      uint32_t total, uint32_t *counter, uint32_t waves, hipStream_t st);
+
 // 64-bit finaliser (splitmix64) used for the per-voxel material hash of the
 // synthetic scenes; the same function runs on the device in sparse.hip.
 static inline uint64_t rkf_mix64(uint64_t z) {

@@ -73,6 +73,17 @@ struct SparseMarchParams {
     uint32_t object_id;
 };
 
+// One batch of a batched launch (batch.hip): the batch's scene and
+// parameters plus its range of 64-lane work units.
+struct BatchDev {
+    SparseMarchParams P;  // scene, camera, uniforms, extension, outputs
+    uint32_t kind, n, unit_begin, n_units;
+    uint32_t tile_w, tile_h, tiles_x, pad;
+    const float *in;
+    float *vis;
+    rkfm_point_result *res;
+};
+
 }  // namespace rkf
 
 extern "C" {

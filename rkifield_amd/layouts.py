@@ -150,6 +150,34 @@ BATCH_ITEM = np.dtype([
 
 BATCH_TUNING = np.dtype([("schedule", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"), ("waves_per_cu", "<u4")])
 
+SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
+REQ_KEEP_ON_DEVICE = 1
+REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
+
+SERVER_CONFIG = np.dtype([
+    ("memory_budget", "<u8"), ("max_requests", "<u4"), ("rounds_in_flight", "<u4"),
+    ("max_round_work", "<u8"), ("max_round_requests", "<u4"), ("policy", "<u4"),
+    ("tuning", BATCH_TUNING), ("max_pending", "<u4"), ("_reserved", "<u4"),
+])
+
+REQUEST = np.dtype([
+    ("kind", "<u4"), ("scene", "<u4"), ("n", "<u4"), ("flags", "<u4"), ("width", "<u4"), ("height", "<u4"),
+    ("camera", CAMERA_UNIFORMS), ("su", SCENE_UNIFORMS), ("ext", SCENE_EXT),
+    ("in", "<u8"), ("out", "<u8"), ("view_out", GBUFFER_DESC), ("tag", "<u8"),
+])
+
+REQUEST_STATUS = np.dtype([
+    ("state", "<u4"), ("error", "<i4"), ("tag", "<u8"), ("round", "<u8"), ("round_requests", "<u4"),
+    ("_pad", "<u4"), ("t_submit", "<f8"), ("t_admit", "<f8"), ("t_launch", "<f8"), ("t_done", "<f8"),
+])
+
+SERVER_STATS = np.dtype([
+    ("submitted", "<u8"), ("completed", "<u8"), ("failed", "<u8"), ("rounds", "<u8"), ("work_items", "<u8"),
+    ("arena_bytes", "<u8"), ("arena_in_use", "<u8"), ("arena_peak", "<u8"),
+    ("max_round_requests_seen", "<u4"), ("queued", "<u4"), ("admitted", "<u4"), ("running", "<u4"),
+    ("gpu_seconds", "<f8"),
+])
+
 assert VOXEL_SAMPLE.itemsize == 8
 assert GPU_OBJECT.itemsize == 256
 assert CAMERA_UNIFORMS.itemsize == 144
@@ -162,6 +190,10 @@ assert PRIM.itemsize == 80
 assert POINT_RESULT.itemsize == 24
 assert GBUFFER_DESC.itemsize == 56
 assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
+assert SERVER_CONFIG.itemsize == 56
+assert REQUEST.itemsize == 312
+assert REQUEST_STATUS.itemsize == 64
+assert SERVER_STATS.itemsize == 88
 
 
 def scene_uniforms(num_objects=0, max_steps=128, max_distance=100.0, hit_threshold=0.001):

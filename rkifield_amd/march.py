@@ -85,6 +85,18 @@ _SIGS = {
     "rkfm_batch_graph_launch": (C.c_int, [_VP, _VP]),
     "rkfm_batch_graph_destroy": (C.c_int, [_VP]),
     "rkfm_batch_work_items": (_U64, [_VP, _U32]),
+    "rkfm_host_alloc": (C.c_int, [_U64, C.POINTER(_VP)]),
+    "rkfm_host_free": (C.c_int, [_VP]),
+    "rkfm_server_create": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
+    "rkfm_server_submit": (C.c_int, [_VP, _VP, C.POINTER(_U64)]),
+    "rkfm_server_status": (C.c_int, [_VP, _U64, _VP]),
+    "rkfm_server_wait": (C.c_int, [_VP, _U64, C.c_double, _VP]),
+    "rkfm_server_view_planes": (C.c_int, [_VP, _U64, _VP]),
+    "rkfm_server_release": (C.c_int, [_VP, _U64]),
+    "rkfm_server_stats_get": (C.c_int, [_VP, _VP]),
+    "rkfm_server_now": (C.c_double, [_VP]),
+    "rkfm_server_destroy": (C.c_int, [_VP]),
+    "rkfm_server_replay": (C.c_int, [_VP, _VP, _VP, _U32, _VP, C.POINTER(C.c_double)]),
     "rkfm_timer_begin": (C.c_int, [_VP]),
     "rkfm_timer_end": (C.c_int, [_VP, C.POINTER(_F)]),
 }
@@ -567,5 +579,169 @@ class BatchGraph:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Pinned host memory (rkfm_host_alloc) viewed as a numpy array
+    (``.a``); copies between it and the device run asynchronously."""
+
+    def __init__(self, shape, dtype):
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        count = int(np.prod(self.shape))
+        nbytes = max(count * self.dtype.itemsize, 1)
+        h = _VP()
+        _check(lib().rkfm_host_alloc(nbytes, C.byref(h)))
+        self.ptr = h.value
+        buf = (C.c_ubyte * nbytes).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=count).reshape(self.shape)
+
+    def free(self):
+        if self.ptr:
+            self.a = None
+            lib().rkfm_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """rkfm_server: resident scenes, memory-budget admission, continuous
+    batching (see include/rkf_march.h). Request builders return REQUEST
+    records; submit() keeps the host arrays a request points at alive until
+    release()."""
+
+    def __init__(self, ctx, scenes, memory_budget=1 << 30, max_requests=0, rounds_in_flight=0, max_round_work=0,
+                 max_round_requests=0, policy=L.SERVE_BATCHED, tuning=None, max_pending=0):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        handles = np.array([s.h for s in self.scenes], np.uint64)
+        cfg = np.zeros(1, L.SERVER_CONFIG)
+        cfg["memory_budget"] = memory_budget
+        cfg["max_requests"] = max_requests
+        cfg["rounds_in_flight"] = rounds_in_flight
+        cfg["max_round_work"] = max_round_work
+        cfg["max_round_requests"] = max_round_requests
+        cfg["policy"] = policy
+        cfg["max_pending"] = max_pending
+        if tuning is not None:
+            cfg["tuning"] = tuning
+        h = _VP()
+        _check(lib().rkfm_server_create(ctx.h, handles.ctypes.data, handles.size, cfg.ctypes.data, C.byref(h)))
+        self.h = h.value
+        self._keep = {}
+
+    # request builders -------------------------------------------------------
+    @staticmethod
+    def view_request(scene, cam, su, ext, width, height, keep_on_device=True, host=None, tag=0):
+        """host: dict of PinnedArray planes (position/normal/material/motion/shadow)."""
+        r = np.zeros((), L.REQUEST)
+        r["kind"] = L.BATCH_VIEW
+        r["scene"] = scene
+        r["flags"] = L.REQ_KEEP_ON_DEVICE if keep_on_device else 0
+        r["width"], r["height"] = width, height
+        r["camera"], r["su"], r["ext"] = cam, su, ext
+        r["tag"] = tag
+        for k, arr in (host or {}).items():
+            r["view_out"][k] = arr.ptr
+        return r
+
+    @staticmethod
+    def points_request(scene, xyz, out, n=None, tag=0):
+        r = np.zeros((), L.REQUEST)
+        r["kind"] = L.BATCH_POINTS
+        r["scene"] = scene
+        r["n"] = xyz.shape[0] if n is None else n
+        r["in"], r["out"] = xyz.ptr, out.ptr
+        r["tag"] = tag
+        return r
+
+    @staticmethod
+    def shadow_request(scene, origins, vis, ext, n=None, tag=0):
+        r = np.zeros((), L.REQUEST)
+        r["kind"] = L.BATCH_SHADOW
+        r["scene"] = scene
+        r["n"] = origins.shape[0] if n is None else n
+        r["ext"] = ext
+        r["in"], r["out"] = origins.ptr, vis.ptr
+        r["tag"] = tag
+        return r
+
+    # calls ------------------------------------------------------------------
+    def submit(self, req, keep=()):
+        r = np.ascontiguousarray(req, L.REQUEST).reshape(1)
+        i = _U64()
+        _check(lib().rkfm_server_submit(self.h, r.ctypes.data, C.byref(i)))
+        self._keep[i.value] = keep
+        return i.value
+
+    def status(self, rid):
+        st = np.zeros(1, L.REQUEST_STATUS)
+        _check(lib().rkfm_server_status(self.h, rid, st.ctypes.data))
+        return st[0]
+
+    def wait(self, rid, timeout=-1.0):
+        st = np.zeros(1, L.REQUEST_STATUS)
+        _check(lib().rkfm_server_wait(self.h, rid, timeout, st.ctypes.data))
+        return st[0]
+
+    def view_planes(self, rid):
+        g = np.zeros(1, L.GBUFFER_DESC)
+        _check(lib().rkfm_server_view_planes(self.h, rid, g.ctypes.data))
+        return g[0]
+
+    def read_view(self, rid):
+        """Host copies of a finished view kept on the device."""
+        g = self.view_planes(rid)
+        n = int(g["width"]) * int(g["height"])
+        out = {}
+        for k, shape, dt in (("position", (n, 4), np.float32), ("normal", (n, 4), np.uint16),
+                             ("material", (n,), np.uint32), ("motion", (n, 4), np.float32),
+                             ("shadow", (n,), np.float32)):
+            if int(g[k]) == 0:
+                continue
+            a = np.empty(shape, dt)
+            _check(lib().rkfm_memcpy_d2h(self.ctx.h, a.ctypes.data, int(g[k]), a.nbytes))
+            out[k] = a
+        return out
+
+    def release(self, rid):
+        _check(lib().rkfm_server_release(self.h, rid))
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        st = np.zeros(1, L.SERVER_STATS)
+        _check(lib().rkfm_server_stats_get(self.h, st.ctypes.data))
+        return st[0]
+
+    def now(self):
+        return lib().rkfm_server_now(self.h)
+
+    def replay(self, reqs, arrivals):
+        """Native replay of an arrival trace; returns (latency seconds per
+        request, NaN if failed; wall seconds to the last completion)."""
+        r = np.ascontiguousarray(reqs, L.REQUEST)
+        a = np.ascontiguousarray(arrivals, np.float64)
+        lat = np.full(r.size, np.nan)
+        wall = C.c_double()
+        _check(lib().rkfm_server_replay(self.h, r.ctypes.data, a.ctypes.data, r.size, lat.ctypes.data,
+                                        C.byref(wall)))
+        return lat, wall.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rkfm_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -202,3 +202,28 @@ def mixed_batches(seed=2, n_views=16, n_shadow=16, n_points=32, half_cube=128.0)
     shadow_origins = [nrng.uniform(-half_cube, half_cube, (n, 3)).astype(np.float32) for n in shadows]
     point_xyz = [nrng.uniform(-half_cube, half_cube, (n, 3)).astype(np.float32) for n in points]
     return views, shadow_origins, point_xyz
+
+
+def served_trace(seed=3, n=4000, rate=2000.0, n_scenes=8):
+    """Served-stream arrival trace: Poisson arrivals at `rate` requests/s
+    (inter-arrival -ln(1-U)/rate); kind by U: < 0.70 a point-query batch of
+    U{1k..64k} points, < 0.95 a 512x512 view, else a 1920x1080 view; scene
+    U{0..n_scenes-1}; a view's camera sits 64 m from the centre on a random
+    direction looking at a point U[-16,16]^3. Draw order per request: gap,
+    kind, scene, then the batch size (points) or direction (2) and target (3).
+    Returns a list of (arrival_s, kind, scene, size_or_view)."""
+    rng = Pcg32(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        u = rng.uniform()
+        sc = min(int(rng.uniform(0.0, n_scenes)), n_scenes - 1)
+        if u < 0.70:
+            out.append((t, "points", sc, 1024 + int(rng.uniform(0, 64 * 1024 - 1024 + 1))))
+        else:
+            w, h = (512, 512) if u < 0.95 else (1920, 1080)
+            d = rng.unit_vector()
+            tgt = [rng.uniform(-16, 16) for _ in range(3)]
+            out.append((t, "view", sc, (w, h, d * 64.0, tgt)))
+    return out
