@@ -343,7 +343,9 @@ int rkfm_sparse_download(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
  * step rules, multi-level DDA over free space, normals, G-buffer write and
  * optional shadow rays. variant 0 = one lane per pixel, 8x8 pixels per
  * wavefront, hardware-scheduled; variant 1 = persistent wavefronts that
- * refill finished lanes (same results, bit for bit). */
+ * refill finished lanes; variant 2 = primary pass + compacted hit list +
+ * shading/shadow pass (the context keeps a 36 B-per-pixel hit list). All
+ * variants give the same results, bit for bit. */
 int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                       const rkf_camera_uniforms *camera,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
@@ -365,12 +367,19 @@ int rkfm_sparse_query_points(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
 /* Free-space structures used by the DDA: free_idx has one u32 per L2 cell
  * (0xFFFFFFFF = every brick of the cell is free, else a row of free_bits);
  * free_bits holds 8 u64 per row, bit (x + 8y + 64z) set = that brick is not
- * free; prox has one u8 per L2 cell, the Chebyshev distance in cells (capped
- * at 16) to the nearest cell that is not entirely free (0 = not free).
- * *n_rows receives the row count (free_bits needs 8 * n_rows words). */
+ * free; oct has one u32 per L2 cell holding 8 octant extents of 4 bits
+ * (octant o: bit a of o set = axis a negative), each the side in cells
+ * (capped at 15) of the largest cube of entirely free cells with this cell at
+ * its corner, extending in the octant's directions (0 = cell not free);
+ * brick_ext holds per row 8 octant planes of 512 brick extents, 4 bits each
+ * (byte (row * 8 + o) * 256 + lb / 2, low nibble for even lb): the side in
+ * bricks (capped at 15) of a cube of free bricks with that brick at its
+ * corner (0 = brick not free). *n_rows receives the row count (free_bits
+ * needs 8 * n_rows words, brick_ext 2048 * n_rows bytes). */
 int rkfm_sparse_download_free(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                               uint32_t *free_idx, uint64_t *free_bits,
-                              uint8_t *prox, uint64_t *n_rows);
+                              uint32_t *oct, uint8_t *brick_ext,
+                              uint64_t *n_rows);
 
 /* ------------------------------------------------------------------------ */
 /* Batched launch: many uneven ray/query batches, over any resident sparse   */

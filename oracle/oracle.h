@@ -156,8 +156,10 @@ typedef struct orc_sparse_scene {
     const uint16_t *dist;
     const uint8_t *mat;
     const uint8_t *free_l2;    /* per L2 cell: 1 = every brick of it is free */
-    const uint8_t *prox;       /* per L2 cell: Chebyshev distance (cells, <= 16)
-                                  to the nearest non-free cell; 0 = not free */
+    const uint32_t *oct;       /* per L2 cell: 8 octant extents (4 bits each) */
+    const uint32_t *bext;      /* per L2 cell x 512 bricks: 8 brick octant
+                                  extents (4 bits each); used in cells that
+                                  are not entirely free */
 } orc_sparse_scene;
 
 /* Unpacked per-ray result. For the FP32 twin all values are exactly the
@@ -192,9 +194,19 @@ void orc_ref_trace(const orc_ref_scene *s, int prec, uint64_t p0, uint64_t p1,
  * EMPTY; that test is evaluated on the fly). */
 uint64_t orc_sparse_n_bricks_global(const orc_sparse_scene *s);
 void orc_sparse_free_l2(const orc_sparse_scene *s, uint8_t *free_l2);
-/* prox from free_l2 (cap 16), by 16 relaxation passes of the 26-neighbour
- * minimum (d = min(16, 1 + min over the 3x3x3 box)); out-of-grid = free. */
-void orc_sparse_prox(const orc_sparse_scene *s, const uint8_t *free_l2, uint8_t *prox);
+/* Octant extents from free_l2: for octant o (bit a of o set = axis a runs
+ * negative) E_o(C) = 0 if C is not free, else the side (capped at 15) of the
+ * largest cube of free cells with C at its corner extending in o's
+ * directions, out-of-grid cells counting as free. Computed by one sweep per
+ * octant of E = min(15, 1 + min over the 7 neighbours further along o). */
+void orc_sparse_oct(const orc_sparse_scene *s, const uint8_t *free_l2, uint32_t *oct);
+/* Brick octant extents for every brick of every cell that is not entirely
+ * free (out[cell * 512 + lb], 0 elsewhere): E_o(b) = 0 if brick b is not
+ * free, else min(15, 1 + min over the 7 neighbours further along o), where a
+ * neighbour outside the brick grid counts 15 and a neighbour n in an entirely
+ * free cell C counts min(15, distance from n to the far side of the free
+ * cube of cells C + [0, E_o(C)) on each axis). One sweep per octant. */
+void orc_sparse_bext(const orc_sparse_scene *s, const uint8_t *free_l2, const uint32_t *oct, uint32_t *out);
 int orc_sparse_brick_free(const orc_sparse_scene *s, uint32_t bx, uint32_t by, uint32_t bz);
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,

@@ -59,7 +59,7 @@ class SparseScene(C.Structure):
         ("origin", C.c_float * 3), ("voxel_size", C.c_float),
         ("cells", C.c_uint32 * 3),
         ("l2", C.c_void_p), ("l1", C.c_void_p), ("dist", C.c_void_p), ("mat", C.c_void_p),
-        ("free_l2", C.c_void_p), ("prox", C.c_void_p),
+        ("free_l2", C.c_void_p), ("oct", C.c_void_p), ("bext", C.c_void_p),
     ]
 
 
@@ -272,7 +272,8 @@ class SparseInputs:
         cells = [int(c) for c in cells]
         nc = cells[0] * cells[1] * cells[2]
         self.free_l2 = np.zeros(nc, np.uint8)
-        self.prox = np.zeros(nc, np.uint8)
+        self.oct = np.zeros(nc, np.uint32)
+        self.bext = np.zeros(nc * 512, np.uint32)
         s = SparseScene()
         s.origin[:] = [float(x) for x in origin]
         s.voxel_size = float(voxel_size)
@@ -282,10 +283,12 @@ class SparseInputs:
         s.dist = _p(self.dist).value
         s.mat = _p(self.mat).value
         s.free_l2 = _p(self.free_l2).value
-        s.prox = _p(self.prox).value
+        s.oct = _p(self.oct).value
+        s.bext = _p(self.bext).value
         self.s = s
         lib().orc_sparse_free_l2(C.byref(s), _p(self.free_l2))
-        lib().orc_sparse_prox(C.byref(s), _p(self.free_l2), _p(self.prox))
+        lib().orc_sparse_oct(C.byref(s), _p(self.free_l2), _p(self.oct))
+        lib().orc_sparse_bext(C.byref(s), _p(self.free_l2), _p(self.oct), _p(self.bext))
 
     def brick_free(self, bx, by, bz):
         return bool(lib().orc_sparse_brick_free(C.byref(self.s), C.c_uint32(bx), C.c_uint32(by), C.c_uint32(bz)))
@@ -346,3 +349,15 @@ def pack_gbuffer(res):
     r = np.ascontiguousarray(res)
     lib().orc_pack_gbuffer(_p(r), C.c_uint64(n), _p(pos), _p(nrm), _p(mat), _p(mot))
     return pos, nrm, mat, mot
+
+
+def sparse_oct(cells, free_l2):
+    """orc_sparse_oct on a bare cell grid (only `cells` and the free flags
+    matter): per cell 8 octant extents, 4 bits each."""
+    cells = [int(c) for c in cells]
+    s = SparseScene()
+    s.cells[:] = cells
+    f = np.ascontiguousarray(free_l2, np.uint8)
+    out = np.zeros(f.size, np.uint32)
+    lib().orc_sparse_oct(C.byref(s), _p(f), _p(out))
+    return out

@@ -122,35 +122,77 @@ void orc_sparse_free_l2(const orc_sparse_scene *s, uint8_t *free_l2) {
             }
 }
 
-void orc_sparse_prox(const orc_sparse_scene *s, const uint8_t *free_l2, uint8_t *prox) {
-    const uint32_t cx = s->cells[0], cy = s->cells[1], cz = s->cells[2];
-    const uint64_t n = (uint64_t)cx * cy * cz;
-    const uint8_t cap = 16;
-    uint8_t *cur = (uint8_t *)malloc(n), *nxt = (uint8_t *)malloc(n);
-    for (uint64_t i = 0; i < n; i++) cur[i] = free_l2[i] ? cap : 0;
-    for (int pass = 0; pass < cap; pass++) {
-        for (uint32_t z = 0; z < cz; z++)
-            for (uint32_t y = 0; y < cy; y++)
-                for (uint32_t x = 0; x < cx; x++) {
-                    uint64_t i = (uint64_t)x + (uint64_t)y * cx + (uint64_t)z * cx * cy;
-                    if (!free_l2[i]) { nxt[i] = 0; continue; }
-                    uint32_t m = 255;
-                    for (int dz = -1; dz <= 1; dz++)
-                        for (int dy = -1; dy <= 1; dy++)
-                            for (int dx = -1; dx <= 1; dx++) {
-                                int64_t xx = (int64_t)x + dx, yy = (int64_t)y + dy, zz = (int64_t)z + dz;
-                                if (xx < 0 || yy < 0 || zz < 0 || xx >= cx || yy >= cy || zz >= cz) continue;
-                                uint32_t v = cur[(uint64_t)xx + (uint64_t)yy * cx + (uint64_t)zz * cx * cy];
-                                if (v < m) m = v;
-                            }
-                    uint32_t d = m + 1u;
-                    nxt[i] = (uint8_t)(d < cap ? d : cap);
+void orc_sparse_oct(const orc_sparse_scene *s, const uint8_t *free_l2, uint32_t *oct) {
+    const int64_t cx = s->cells[0], cy = s->cells[1], cz = s->cells[2];
+    const uint32_t cap = 15;
+    for (int64_t i = 0; i < cx * cy * cz; i++) oct[i] = 0;
+    for (int o = 0; o < 8; o++) {
+        const int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
+        /* visit cells so that the neighbours further along o come first */
+        for (int64_t kz = 0; kz < cz; kz++)
+            for (int64_t ky = 0; ky < cy; ky++)
+                for (int64_t kx = 0; kx < cx; kx++) {
+                    const int64_t x = sx > 0 ? cx - 1 - kx : kx;
+                    const int64_t y = sy > 0 ? cy - 1 - ky : ky;
+                    const int64_t z = sz > 0 ? cz - 1 - kz : kz;
+                    const int64_t i = x + y * cx + z * cx * cy;
+                    if (!free_l2[i]) continue; /* E = 0 */
+                    uint32_t m = cap;
+                    for (int k = 1; k < 8; k++) {
+                        const int64_t xx = x + (k & 1) * sx, yy = y + ((k >> 1) & 1) * sy, zz = z + (k >> 2) * sz;
+                        uint32_t v = cap; /* out of the grid: free */
+                        if (xx >= 0 && yy >= 0 && zz >= 0 && xx < cx && yy < cy && zz < cz)
+                            v = (oct[xx + yy * cx + zz * cx * cy] >> (4 * o)) & 15u;
+                        if (v < m) m = v;
+                    }
+                    const uint32_t e = m + 1 < cap ? m + 1 : cap;
+                    oct[i] |= e << (4 * o);
                 }
-        uint8_t *t = cur; cur = nxt; nxt = t;
     }
-    memcpy(prox, cur, n);
-    free(cur);
-    free(nxt);
+}
+
+void orc_sparse_bext(const orc_sparse_scene *s, const uint8_t *free_l2, const uint32_t *oct, uint32_t *out) {
+    const int64_t cx = s->cells[0], cy = s->cells[1], cz = s->cells[2];
+    const int64_t nx = cx * 8, ny = cy * 8, nz = cz * 8;
+    const uint32_t cap = 15;
+    for (int64_t i = 0; i < cx * cy * cz * 512; i++) out[i] = 0;
+    for (int o = 0; o < 8; o++) {
+        const int sg[3] = {(o & 1) ? -1 : 1, (o & 2) ? -1 : 1, (o & 4) ? -1 : 1};
+        for (int64_t kz = 0; kz < nz; kz++)
+            for (int64_t ky = 0; ky < ny; ky++)
+                for (int64_t kx = 0; kx < nx; kx++) {
+                    const int64_t b[3] = {sg[0] > 0 ? nx - 1 - kx : kx, sg[1] > 0 ? ny - 1 - ky : ky,
+                                          sg[2] > 0 ? nz - 1 - kz : kz};
+                    const int64_t cell = (b[0] >> 3) + (b[1] >> 3) * cx + (b[2] >> 3) * cx * cy;
+                    if (free_l2[cell]) continue;
+                    if (!orc_sparse_brick_free(s, (uint32_t)b[0], (uint32_t)b[1], (uint32_t)b[2])) continue; /* 0 */
+                    uint32_t m = cap;
+                    for (int k = 1; k < 8; k++) {
+                        const int64_t n[3] = {b[0] + (k & 1) * sg[0], b[1] + ((k >> 1) & 1) * sg[1],
+                                              b[2] + (k >> 2) * sg[2]};
+                        uint32_t v = cap;
+                        if (n[0] >= 0 && n[1] >= 0 && n[2] >= 0 && n[0] < nx && n[1] < ny && n[2] < nz) {
+                            const int64_t c2 = (n[0] >> 3) + (n[1] >> 3) * cx + (n[2] >> 3) * cx * cy;
+                            if (!free_l2[c2]) {
+                                const int64_t lb = (n[0] & 7) + (n[1] & 7) * 8 + (n[2] & 7) * 64;
+                                v = (out[c2 * 512 + lb] >> (4 * o)) & 15u;
+                            } else {
+                                const int64_t e = (oct[c2] >> (4 * o)) & 15u;
+                                const int64_t cc[3] = {n[0] >> 3, n[1] >> 3, n[2] >> 3};
+                                int64_t mm = cap;
+                                for (int a = 0; a < 3; a++) {
+                                    int64_t d = sg[a] < 0 ? n[a] - (cc[a] - e + 1) * 8 + 1 : (cc[a] + e) * 8 - n[a];
+                                    if (d < mm) mm = d;
+                                }
+                                v = (uint32_t)mm;
+                            }
+                        }
+                        if (v < m) m = v;
+                    }
+                    const uint32_t e = m + 1 < cap ? m + 1 : cap;
+                    out[cell * 512 + (b[0] & 7) + (b[1] & 7) * 8 + (b[2] & 7) * 64] |= e << (4 * o);
+                }
+    }
 }
 
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,

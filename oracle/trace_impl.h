@@ -523,10 +523,12 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
  * b0 = floor(floor(g)/8). A sample's 8 corners lie in bricks {b0, b0+1}^3, so
  * if all of those are EMPTY (b0 "free") the field is the EMPTY constant
  * 8 vs over the whole region floor(g/8) == b0; if every brick of an L2 cell is
- * free, over floor(g/64) == cell; and prox[cell] = d >= 1 (the Chebyshev
- * distance, in cells, to the nearest cell that is not entirely free, capped
- * at 16; cells outside the grid count as free) extends that region to the
- * cube of cells within distance d - 1. Sphere tracing through such
+ * free, over floor(g/64) == cell, and the cell's octant extent E >= 1 for the
+ * ray's octant (oct[cell], see orc_sparse_oct) extends that region to the
+ * cube of cells q + [0, E) taken in the ray's direction on each axis; in a
+ * cell that is not entirely free, a free brick's octant extent E_b (bext,
+ * see orc_sparse_bext) extends it to the cube of bricks b0 + [0, E_b). Sphere
+ * tracing through such
  * a region only takes positive, non-near steps; the march instead advances to
  * the region's exit t_x along the ray, sets prev_t = t_x, prev_dist = 8 vs
  * (the field value there) and resumes at t_x + vs/256. Each skip counts as one
@@ -634,16 +636,23 @@ static int F(sp_free)(const F(sctx) *c, V3 pos, V3 sd, V3 inv, R t, R *t_exit) {
     uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * s->cells[0] + (uint64_t)q[2] * s->cells[0] * s->cells[1];
     R lo_b[3], hi_b[3];
-    uint32_t d = s->prox[cell];
-    if (d > 0) {
-        /* every cell within Chebyshev distance d-1 of this one is free */
+    uint32_t octant = (sd.x < (R)0 ? 1u : 0u) | (sd.y < (R)0 ? 2u : 0u) | (sd.z < (R)0 ? 4u : 0u);
+    uint32_t e = (s->oct[cell] >> (4u * octant)) & 15u;
+    if (e > 0) {
+        /* the cube of cells q + [0, e) in the travel direction of each axis is free */
         for (int a = 0; a < 3; a++) {
-            lo_b[a] = ((R)q[a] - (R)(d - 1u)) * (R)64.0f;
-            hi_b[a] = ((R)q[a] + (R)d) * (R)64.0f;
+            lo_b[a] = ((R)q[a] - (R)(e - 1u)) * (R)64.0f;
+            hi_b[a] = ((R)q[a] + (R)e) * (R)64.0f;
         }
     } else {
-        if (!orc_sparse_brick_free(s, b[0], b[1], b[2])) return 0;
-        for (int a = 0; a < 3; a++) { lo_b[a] = (R)(b[a] * 8u); hi_b[a] = (R)(b[a] * 8u + 8u); }
+        uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
+        uint32_t eb = (s->bext[cell * 512u + lb] >> (4u * octant)) & 15u;
+        if (eb == 0) return 0;
+        /* the cube of bricks b + [0, eb) in the travel direction of each axis is free */
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((R)b[a] - (R)(eb - 1u)) * (R)8.0f;
+            hi_b[a] = ((R)b[a] + (R)eb) * (R)8.0f;
+        }
     }
     R dv[3] = {sd.x, sd.y, sd.z};
     R iv[3] = {inv.x, inv.y, inv.z};

@@ -26,7 +26,7 @@ struct DevBuf {
 struct rkfm_sparse_scene {
     rkfm_ctx *owner = nullptr;
     rkfm_sparse_info info{};
-    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, prox;
+    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, oct, cellw, bext;
     uint64_t n_free_rows = 0;
     rkf::SparseDev dev() const;
 };
@@ -49,6 +49,8 @@ struct rkfm_ctx {
     std::vector<rkfm_sparse_scene *> scenes;
     // Persistent-wave count of the tuned sparse kernel (occupancy x CUs).
     uint32_t tuned_waves = 0;
+    // Split sparse march: compact hit list and its counter.
+    DevBuf march_hits, march_count;
     // Batched launch: device descriptors (rewritten per call, stream-ordered),
     // their host staging copies, the persistent work counter, the tuning.
     DevBuf batch_items, batch_begins, batch_counter;

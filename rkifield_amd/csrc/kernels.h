@@ -46,6 +46,8 @@ struct CoarseParams {
 
 // Sparse two-level scene as the kernels see it (sparse.hip).
 constexpr uint32_t kFreeCell = 0xFFFFFFFFu;
+constexpr uint32_t kOctCap = 15u;          // octant extents saturate at 15 cells
+constexpr uint32_t kNoSlot = 0xFFFFFFFDu;  // "slot not prefetched"
 struct SparseDev {
     float origin[3];
     float vs;
@@ -59,9 +61,13 @@ struct SparseDev {
     const uint16_t *aux;       // per slot: 512 (secondary | blend << 8), or null
     const uint32_t *free_idx;  // per L2 cell: kFreeCell or row of free_bits
     const uint64_t *free_bits; // 8 words per row: bit set = brick NOT free
-    const uint8_t *prox;       // per L2 cell: Chebyshev distance (cells, <= 16)
-                               // to the nearest non-free cell; 0 = not free
+    const uint32_t *oct;       // per L2 cell: 8 octant extents, 4 bits each
+    const uint8_t *bext;       // per row: 8 octant planes of 512 brick extents,
+                               // 4 bits each ((row * 8 + o) * 256 + lb / 2)
+    const uint64_t *cellw;     // per L2 cell, what one DDA step needs in one
+                               // load: free row << 32 | octant extents
     uint64_t n_cells;
+    uint32_t n_blocks;         // rows below n_blocks are the L1 blocks
 };
 
 struct SparseMarchParams {
@@ -94,6 +100,9 @@ hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream);
 hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,
                                    hipStream_t stream);
 hipError_t rkf_occupancy_waves(uint32_t which, uint32_t *waves);
+uint64_t rkf_hit_record_bytes();
+hipError_t rkf_launch_sparse_march_split(const rkf::SparseMarchParams *P, void *hits, uint32_t *count,
+                                         uint32_t shade_waves, hipStream_t stream);
 hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,
                                    hipStream_t stream);
 }

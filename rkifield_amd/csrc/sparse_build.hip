@@ -229,35 +229,167 @@ __global__ __launch_bounds__(512) void free_mask_kernel(SparseDev S, uint64_t nr
     if (t == 0) row_any[row] = any;
 }
 
-// Proximity field: d = 0 for cells that are not entirely free, else
-// min(16, 1 + min over the 3x3x3 box); out-of-grid cells are ignored (free).
-constexpr uint32_t kProxCap = 16u;
-__global__ __launch_bounds__(256) void prox_init_kernel(const uint32_t *free_idx, uint64_t n, uint8_t *prox) {
+// Octant extents. For octant o (bit a set = axis a runs negative) and a cell
+// C: E_o(C) = 0 if C is not entirely free, else min(15, 1 + min over the 7
+// neighbours C + (i sx, j sy, k sz), (i, j, k) in {0,1}^3 minus 0), with
+// out-of-grid neighbours counted as 15 (free): the side of the largest cube of
+// free cells that has C at its corner and extends in o's directions. Packed
+// 4 bits per octant. Jacobi passes from "free = 15" reach the fixed point in
+// 15 passes (a value below 15 is the octant distance to a non-free cell).
+__global__ __launch_bounds__(256) void oct_init_kernel(const uint32_t *free_idx, uint64_t n, uint32_t *oct) {
     uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (i < n) prox[i] = free_idx[i] == kFreeCell ? (uint8_t)kProxCap : (uint8_t)0;
+    if (i < n) oct[i] = free_idx[i] == kFreeCell ? 0xFFFFFFFFu : 0u;
 }
-__global__ __launch_bounds__(256) void prox_relax_kernel(uint32_t cx, uint32_t cy, uint32_t cz, const uint8_t *cur,
-                                                         uint8_t *nxt) {
+__global__ __launch_bounds__(256) void oct_relax_kernel(uint32_t cx, uint32_t cy, uint32_t cz, const uint32_t *cur,
+                                                        uint32_t *nxt) {
     uint64_t n = (uint64_t)cx * cy * cz;
     uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    uint8_t self = cur[i];
-    if (self == 0) {
-        nxt[i] = 0;
+    if (cur[i] == 0u) {
+        nxt[i] = 0u;
         return;
     }
     uint32_t x = (uint32_t)(i % cx), y = (uint32_t)((i / cx) % cy), z = (uint32_t)(i / ((uint64_t)cx * cy));
-    uint32_t m = 255;
+    uint32_t nb[27];
     for (int dz = -1; dz <= 1; dz++)
         for (int dy = -1; dy <= 1; dy++)
             for (int dx = -1; dx <= 1; dx++) {
                 int64_t xx = (int64_t)x + dx, yy = (int64_t)y + dy, zz = (int64_t)z + dz;
-                if (xx < 0 || yy < 0 || zz < 0 || xx >= cx || yy >= cy || zz >= cz) continue;
-                uint32_t v = cur[(uint64_t)xx + (uint64_t)yy * cx + (uint64_t)zz * cx * cy];
-                m = v < m ? v : m;
+                bool in = xx >= 0 && yy >= 0 && zz >= 0 && xx < cx && yy < cy && zz < cz;
+                nb[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] =
+                    in ? cur[(uint64_t)xx + (uint64_t)yy * cx + (uint64_t)zz * cx * cy] : 0xFFFFFFFFu;
             }
-    uint32_t d = m + 1u;
-    nxt[i] = (uint8_t)(d < kProxCap ? d : kProxCap);
+    uint32_t out = 0;
+    for (int o = 0; o < 8; o++) {
+        int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
+        uint32_t m = kOctCap;
+        for (int k = 1; k < 8; k++) {
+            int ox = (k & 1) * sx, oy = ((k >> 1) & 1) * sy, oz = (k >> 2) * sz;
+            uint32_t v = (nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] >> (4 * o)) & 15u;
+            m = v < m ? v : m;
+        }
+        uint32_t e = m + 1u < kOctCap ? m + 1u : kOctCap;
+        out |= e << (4 * o);
+    }
+    nxt[i] = out;
+}
+constexpr uint32_t kOctPasses = 16u;  // >= 15, even: the result lands in the first buffer
+
+// Brick octant extents inside rows (cells that are not entirely free): for
+// a brick b of a row and octant o, E_o(b) = 0 if b is not free, else
+// min(15, 1 + min over the 7 neighbours b + (i sx, j sy, k sz)) where a
+// neighbour outside the brick grid counts 15 and a neighbour n in an entirely
+// free cell C counts the bound its cell extent gives, min(15, distance from n
+// to the far side of the free cube of cells C + [0, E_o(C)) on each axis),
+// which is at least 1. Every brick of the cube b + [0, E_o(b)) in o's
+// directions is then free. Jacobi passes from "free = 15" over packed u32
+// (8 nibbles per brick) reach the fixed point in 15 passes; the march reads
+// per-octant nibble planes.
+struct BextArgs {
+    uint32_t cells[3];
+    uint32_t nb[3];
+    const uint32_t *free_idx;
+    const uint32_t *oct;
+    const uint64_t *free_bits;
+    const uint32_t *row_cell;
+    uint64_t n_rows;
+};
+
+RKF_DEV uint32_t free_cell_bound(const BextArgs &A, uint32_t cell, int64_t x, int64_t y, int64_t z) {
+    const uint32_t ov = A.oct[cell];
+    const int64_t c[3] = {(int64_t)(cell % A.cells[0]), (int64_t)((cell / A.cells[0]) % A.cells[1]),
+                          (int64_t)(cell / (A.cells[0] * A.cells[1]))};
+    const int64_t n[3] = {x, y, z};
+    uint32_t out = 0;
+    for (int o = 0; o < 8; o++) {
+        const int64_t e = (ov >> (4 * o)) & 15u;
+        int64_t m = kOctCap;
+        for (int a = 0; a < 3; a++) {
+            int64_t d = ((o >> a) & 1) ? n[a] - (c[a] - e + 1) * 8 + 1 : (c[a] + e) * 8 - n[a];
+            m = d < m ? d : m;
+        }
+        out |= (uint32_t)m << (4 * o);
+    }
+    return out;
+}
+
+__global__ __launch_bounds__(512) void bext_init_kernel(BextArgs A, uint32_t *cur) {
+    uint64_t item = item_index();
+    if (item >= A.n_rows) return;
+    const uint32_t t = threadIdx.x;
+    bool nf = (A.free_bits[item * 8u + (t >> 6)] >> (t & 63u)) & 1ull;
+    cur[item * 512u + t] = nf ? 0u : 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(512) void bext_relax_kernel(BextArgs A, const uint32_t *cur, uint32_t *nxt) {
+    uint64_t item = item_index();
+    if (item >= A.n_rows) return;
+    const uint32_t t = threadIdx.x;
+    const uint64_t self = item * 512u + t;
+    if (cur[self] == 0u) {
+        nxt[self] = 0u;
+        return;
+    }
+    const uint32_t cell = A.row_cell[item];
+    const int64_t bx = (int64_t)(cell % A.cells[0]) * 8 + (t & 7u);
+    const int64_t by = (int64_t)((cell / A.cells[0]) % A.cells[1]) * 8 + ((t >> 3) & 7u);
+    const int64_t bz = (int64_t)(cell / (A.cells[0] * A.cells[1])) * 8 + (t >> 6);
+    uint32_t nbv[27];
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int k = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                const int64_t x = bx + dx, y = by + dy, z = bz + dz;
+                if (k == 13) {
+                    nbv[k] = 0;
+                    continue;
+                }
+                if (x < 0 || y < 0 || z < 0 || x >= A.nb[0] || y >= A.nb[1] || z >= A.nb[2]) {
+                    nbv[k] = 0xFFFFFFFFu;
+                    continue;
+                }
+                const uint32_t c2 = (uint32_t)((x >> 3) + (y >> 3) * A.cells[0] + (z >> 3) * A.cells[0] * A.cells[1]);
+                const uint32_t lb = (uint32_t)((x & 7) + (y & 7) * 8 + (z & 7) * 64);
+                if (c2 == cell) {
+                    nbv[k] = cur[item * 512u + lb];
+                } else {
+                    const uint32_t r2 = A.free_idx[c2];
+                    nbv[k] = r2 != kFreeCell ? cur[(uint64_t)r2 * 512u + lb] : free_cell_bound(A, c2, x, y, z);
+                }
+            }
+    uint32_t out = 0;
+    for (int o = 0; o < 8; o++) {
+        const int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
+        uint32_t m = kOctCap;
+        for (int k = 1; k < 8; k++) {
+            const int ox = (k & 1) * sx, oy = ((k >> 1) & 1) * sy, oz = (k >> 2) * sz;
+            const uint32_t v = (nbv[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] >> (4 * o)) & 15u;
+            m = v < m ? v : m;
+        }
+        const uint32_t e = m + 1u < kOctCap ? m + 1u : kOctCap;
+        out |= e << (4 * o);
+    }
+    nxt[self] = out;
+}
+
+// Packed u32 per brick -> per-octant nibble planes (brick lb in the low
+// nibble when lb is even).
+__global__ __launch_bounds__(256) void bext_planes_kernel(uint64_t n_rows, const uint32_t *packed, uint8_t *planes) {
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x + (uint64_t)blockIdx.y * gridDim.x * 256u;
+    if (i >= n_rows * 2048u) return;
+    const uint64_t row = i / 2048u;
+    const uint32_t o = (uint32_t)((i / 256u) % 8u), j = (uint32_t)(i % 256u);
+    const uint32_t lo = (packed[row * 512u + 2u * j] >> (4u * o)) & 15u;
+    const uint32_t hi = (packed[row * 512u + 2u * j + 1u] >> (4u * o)) & 15u;
+    planes[i] = (uint8_t)(lo | (hi << 4));
+}
+
+// cellw = free row << 32 | octant extents (row kFreeCell: every brick free).
+__global__ __launch_bounds__(256) void cellword_kernel(const uint32_t *free_idx, const uint32_t *oct, uint64_t n,
+                                                       uint64_t *cellw) {
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    cellw[i] = ((uint64_t)free_idx[i] << 32) | (uint64_t)oct[i];
 }
 
 }  // namespace rkf
@@ -279,8 +411,11 @@ rkf::SparseDev rkfm_sparse_scene::dev() const {
     d.aux = (const uint16_t *)aux.ptr;
     d.free_idx = (const uint32_t *)free_idx.ptr;
     d.free_bits = (const uint64_t *)free_bits.ptr;
-    d.prox = (const uint8_t *)prox.ptr;
+    d.oct = (const uint32_t *)oct.ptr;
+    d.bext = (const uint8_t *)bext.ptr;
+    d.cellw = (const uint64_t *)cellw.ptr;
     d.n_cells = (uint64_t)info.cells[0] * info.cells[1] * info.cells[2];
+    d.n_blocks = info.n_blocks;
     return d;
 }
 
@@ -292,7 +427,9 @@ static void release_scene(rkfm_sparse_scene *s) {
     s->aux.release();
     s->free_idx.release();
     s->free_bits.release();
-    s->prox.release();
+    s->oct.release();
+    s->cellw.release();
+    s->bext.release();
 }
 
 void rkf_sparse_release_all(rkfm_ctx *c) {
@@ -337,20 +474,29 @@ uint64_t used_bytes(const rkfm_ctx *c) {
 int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32_t> &block_cells) {
     const uint32_t *cells = s->info.cells;
     uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
-    // Cells whose {C..C+1}^3 neighbourhood holds a block: C in block - {0,1}^3.
-    std::vector<uint32_t> rows;
-    rows.reserve(block_cells.size() * 8);
+    // Rows: cells whose {C..C+1}^3 neighbourhood holds a block, i.e. C in
+    // block - {0,1}^3. Row i < n_blocks is block i's own cell (an occupied
+    // cell is never entirely free), so a row number is also the cell's L1
+    // block when it is below n_blocks.
+    std::vector<uint32_t> rows(block_cells), extra;
+    std::vector<uint8_t> occupied(ncell, 0);
+    for (uint32_t bc : block_cells) occupied[bc] = 1;
+    extra.reserve(block_cells.size() * 7);
     for (uint32_t bc : block_cells) {
         uint32_t qx = bc % cells[0], qy = (bc / cells[0]) % cells[1], qz = bc / (cells[0] * cells[1]);
         for (uint32_t dz = 0; dz < 2; dz++)
             for (uint32_t dy = 0; dy < 2; dy++)
                 for (uint32_t dx = 0; dx < 2; dx++) {
                     if (qx < dx || qy < dy || qz < dz) continue;
-                    rows.push_back((qx - dx) + (qy - dy) * cells[0] + (qz - dz) * cells[0] * cells[1]);
+                    uint32_t cc = (qx - dx) + (qy - dy) * cells[0] + (qz - dz) * cells[0] * cells[1];
+                    if (!occupied[cc]) extra.push_back(cc);
                 }
     }
-    std::sort(rows.begin(), rows.end());
-    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    std::sort(extra.begin(), extra.end());
+    extra.erase(std::unique(extra.begin(), extra.end()), extra.end());
+    rows.insert(rows.end(), extra.begin(), extra.end());
+    if (rows.size() >= (uint64_t)rkf::kFreeCell)
+        return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "sparse scene: %zu free-mask rows", rows.size());
     TmpBufs tmp;
     DevBuf *d_rows = tmp.make(), *d_any = tmp.make();
     int rc = put(c->stream, *d_rows, rows.data(), rows.size() * 4);
@@ -379,29 +525,69 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
     rc = rkf_hip_check(hipMemcpyAsync(s->free_idx.ptr, fidx.data(), ncell * 4, hipMemcpyHostToDevice, c->stream),
                        "free idx upload");
     if (rc) return rc;
-    // proximity field
+    // octant extents
     DevBuf *tmpb = tmp.make();
-    rc = s->prox.reserve(ncell, false);
-    if (!rc) rc = tmpb->reserve(ncell, false);
+    rc = s->oct.reserve(ncell * 4, false);
+    if (!rc) rc = tmpb->reserve(ncell * 4, false);
     if (rc) return rc;
-    s->prox.size = ncell;
+    s->oct.size = ncell * 4;
     dim3 g((unsigned)((ncell + 255) / 256)), b(256);
-    hipLaunchKernelGGL(rkf::prox_init_kernel, g, b, 0, c->stream, (const uint32_t *)s->free_idx.ptr, ncell,
-                       (uint8_t *)s->prox.ptr);
-    uint8_t *cur = (uint8_t *)s->prox.ptr, *nxt = (uint8_t *)tmpb->ptr;
-    for (uint32_t pass = 0; pass < rkf::kProxCap; pass++) {
-        hipLaunchKernelGGL(rkf::prox_relax_kernel, g, b, 0, c->stream, cells[0], cells[1], cells[2], cur, nxt);
+    hipLaunchKernelGGL(rkf::oct_init_kernel, g, b, 0, c->stream, (const uint32_t *)s->free_idx.ptr, ncell,
+                       (uint32_t *)s->oct.ptr);
+    uint32_t *cur = (uint32_t *)s->oct.ptr, *nxt = (uint32_t *)tmpb->ptr;
+    for (uint32_t pass = 0; pass < rkf::kOctPasses; pass++) {
+        hipLaunchKernelGGL(rkf::oct_relax_kernel, g, b, 0, c->stream, cells[0], cells[1], cells[2], cur, nxt);
         std::swap(cur, nxt);
     }
-    static_assert(rkf::kProxCap % 2u == 0u, "an even pass count leaves the result in s->prox");
-    rc = rkf_hip_check(hipGetLastError(), "prox launch");
+    static_assert(rkf::kOctPasses % 2u == 0u && rkf::kOctPasses >= rkf::kOctCap, "pass count");
+    rc = rkf_hip_check(hipGetLastError(), "octant launch");
+    if (!rc) rc = s->cellw.reserve(ncell * 8, false);
+    if (rc) return rc;
+    s->cellw.size = ncell * 8;
+    hipLaunchKernelGGL(rkf::cellword_kernel, g, b, 0, c->stream, (const uint32_t *)s->free_idx.ptr,
+                       (const uint32_t *)s->oct.ptr, ncell, (uint64_t *)s->cellw.ptr);
+    rc = rkf_hip_check(hipGetLastError(), "cell word launch");
+    if (rc) return rc;
+    // brick extents inside rows
+    const uint64_t nrows = rows.size();
+    if (!rc) rc = s->bext.reserve(nrows * 2048 + 64, false);
+    DevBuf *pa = tmp.make(), *pb = tmp.make();
+    if (!rc) rc = pa->reserve(nrows * 2048 + 64, false);
+    if (!rc) rc = pb->reserve(nrows * 2048 + 64, false);
+    if (rc) return rc;
+    s->bext.size = nrows * 2048;
+    if (nrows) {
+        rkf::BextArgs A;
+        for (int a = 0; a < 3; a++) {
+            A.cells[a] = cells[a];
+            A.nb[a] = cells[a] * 8u;
+        }
+        A.free_idx = (const uint32_t *)s->free_idx.ptr;
+        A.oct = (const uint32_t *)s->oct.ptr;
+        A.free_bits = (const uint64_t *)s->free_bits.ptr;
+        A.row_cell = (const uint32_t *)d_rows->ptr;
+        A.n_rows = nrows;
+        hipLaunchKernelGGL(rkf::bext_init_kernel, rkf::item_grid(nrows), dim3(512), 0, c->stream, A,
+                           (uint32_t *)pa->ptr);
+        uint32_t *bc = (uint32_t *)pa->ptr, *bn = (uint32_t *)pb->ptr;
+        for (uint32_t pass = 0; pass < rkf::kOctPasses; pass++) {
+            hipLaunchKernelGGL(rkf::bext_relax_kernel, rkf::item_grid(nrows), dim3(512), 0, c->stream, A, bc, bn);
+            std::swap(bc, bn);
+        }
+        const uint64_t n_pl = nrows * 2048u;
+        const uint64_t blocks = (n_pl + 255u) / 256u;
+        dim3 pg((unsigned)(blocks < 65535u ? blocks : 65535u), (unsigned)((blocks + 65534u) / 65535u));
+        hipLaunchKernelGGL(rkf::bext_planes_kernel, pg, dim3(256), 0, c->stream, nrows, bc, (uint8_t *)s->bext.ptr);
+        rc = rkf_hip_check(hipGetLastError(), "brick extent launch");
+        if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "brick extent sync");
+    }
     if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "free idx sync");
     return rc;
 }
 
 void finish_info(rkfm_sparse_scene *s) {
     s->info.device_bytes = s->l2.cap + s->l1.cap + s->dist.cap + s->mat.cap + s->aux.cap + s->free_idx.cap +
-                           s->free_bits.cap + s->prox.cap;
+                           s->free_bits.cap + s->oct.cap + s->cellw.cap + s->bext.cap;
 }
 
 }  // namespace
@@ -696,7 +882,7 @@ int rkfm_sparse_download(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *l2, 
 }
 
 int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t *free_idx, uint64_t *free_bits,
-                              uint8_t *prox, uint64_t *n_rows) {
+                              uint32_t *oct, uint8_t *brick_ext, uint64_t *n_rows) {
     if (!c || !s || !n_rows) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_download_free: null argument");
     *n_rows = s->n_free_rows;
     int rc = rkf_hip_check(hipStreamSynchronize(c->stream), "download sync");
@@ -704,7 +890,9 @@ int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t 
         rc = rkf_hip_check(hipMemcpy(free_idx, s->free_idx.ptr, s->free_idx.size, hipMemcpyDeviceToHost), "free idx");
     if (!rc && free_bits && s->free_bits.size)
         rc = rkf_hip_check(hipMemcpy(free_bits, s->free_bits.ptr, s->free_bits.size, hipMemcpyDeviceToHost), "free bits");
-    if (!rc && prox) rc = rkf_hip_check(hipMemcpy(prox, s->prox.ptr, s->prox.size, hipMemcpyDeviceToHost), "prox");
+    if (!rc && oct) rc = rkf_hip_check(hipMemcpy(oct, s->oct.ptr, s->oct.size, hipMemcpyDeviceToHost), "octants");
+    if (!rc && brick_ext && s->bext.size)
+        rc = rkf_hip_check(hipMemcpy(brick_ext, s->bext.ptr, s->bext.size, hipMemcpyDeviceToHost), "brick extents");
     return rc;
 }
 
@@ -712,7 +900,7 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, const rkfm_gbuffer *out,
                       uint32_t variant, void *stream) {
     if (!c || !s || !cam || !su) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: null argument");
-    if (variant > 1) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: variant %u", variant);
+    if (variant > 2) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: variant %u", variant);
     if (ext && ext->hit_eps_mode > RKFM_EPS_RELATIVE)
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: hit_eps_mode %u", ext->hit_eps_mode);
     rkf::SparseMarchParams P;
@@ -728,6 +916,18 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
     if (variant == 1 && !c->tuned_waves) {
         rc = rkf_hip_check(rkf_occupancy_waves(0, &c->tuned_waves), "occupancy");
         if (rc) return rc;
+    }
+    if (variant == 2) {
+        uint64_t px = (uint64_t)out->width * out->height;
+        rc = c->march_hits.reserve(px * rkf_hit_record_bytes(), false);
+        if (!rc) rc = c->march_count.reserve(256, false);
+        if (rc) return rc;
+        uint32_t shade_waves = 0;
+        rc = rkf_device_waves(16, shade_waves);
+        if (rc) return rc;
+        return rkf_hip_check(rkf_launch_sparse_march_split(&P, c->march_hits.ptr, (uint32_t *)c->march_count.ptr,
+                                                           shade_waves, st),
+                             "sparse_march launch");
     }
     return rkf_hip_check(rkf_launch_sparse_march(&P, variant, c->tuned_waves, st),
                          "sparse_march launch");

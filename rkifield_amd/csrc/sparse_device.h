@@ -14,12 +14,14 @@
 // Multi-level free-space DDA (no reference counterpart): with
 // g = (p - origin)/vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
 // in bricks {b0, b0+1}^3. If all of them are EMPTY the field is the constant
-// 8 vs over the region floor(g/8) == b0 (or, when the whole L2 cell is free,
-// over floor(g/64) == cell, and over the cube of cells within Chebyshev
-// distance prox-1 of it, prox being the distance to the nearest cell that is
-// not entirely free, capped at 16); the ray jumps to the region's exit t_x, takes
-// prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip counts
-// as one loop iteration.
+// 8 vs over the region floor(g/8) == b0, and over the cube of free bricks
+// that has b0 at its corner and extends E_b bricks in the directions the ray
+// travels (E_b = b0's brick extent for the ray's octant, at most 15). When
+// the whole L2 cell is free, the region is the cube of free cells that has
+// this cell at its corner and extends E cells in those directions (E = the
+// cell's extent for the ray's octant, at most 15). The ray jumps to the region's exit t_x,
+// takes prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip
+// counts as one loop iteration.
 #pragma once
 
 #include "kernels.h"
@@ -51,7 +53,9 @@ RKF_DEV float sp_voxel(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz, f
     return h2f(S.dist[idx]);
 }
 
-RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count) {
+// pre: the slot of the brick holding corner (x0, y0, z0) when the caller
+// already loaded it (sp_free), else kNoSlot.
+RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count, uint32_t pre = kNoSlot) {
     const SparseDev &S = *c.S;
     float vs = S.vs;
     float be = vs * 8.0f;
@@ -71,7 +75,7 @@ RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count) {
     bool one_brick = x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + 1 < S.tot[0] && y0 + 1 < S.tot[1] &&
                      z0 + 1 < S.tot[2] && (x0 & 7) != 7 && (y0 & 7) != 7 && (z0 & 7) != 7;
     if (one_brick) {
-        uint32_t slot = sp_slot(S, x0, y0, z0);
+        uint32_t slot = pre != kNoSlot ? pre : sp_slot(S, x0, y0, z0);
         if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) {
             float v = slot == RKF_EMPTY_SLOT ? vs * 8.0f : (grad ? -(vs * 8.0f) : -(vs * 2.0f));
             c000 = c100 = c010 = c110 = c001 = c101 = c011 = c111 = v;
@@ -132,8 +136,13 @@ RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
     return i == ~0ull ? 0u : (uint32_t)(S.mat[i] & 0x3Fu);
 }
 
-// Free-space test + exit parameter (see file header).
-RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit) {
+// Free-space test + exit parameter (see file header). One load of the cell
+// word decides cell-level skips; otherwise the brick's free bit and its L1
+// slot (the row is the L1 block when below n_blocks) are loaded side by side,
+// so a step that must sample has its slot (returned in `slot`, else kNoSlot)
+// after two dependent loads.
+RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, uint32_t &slot) {
+    slot = kNoSlot;
     float vs = S.vs;
     float be = vs * 8.0f;
     f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
@@ -151,23 +160,31 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
     uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
     float lo_b[3], hi_b[3];
-    uint32_t d = S.prox[cell];
-    if (d > 0) {
-        // every cell within Chebyshev distance d-1 of this one is free
+    const uint64_t cw = S.cellw[cell];
+    const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
+    const uint32_t e = ((uint32_t)cw >> (4u * octant)) & 15u;
+    if (e > 0) {
+        // the cube of cells q + [0, e) in the travel direction of each axis is free
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            lo_b[a] = ((float)q[a] - (float)(d - 1u)) * 64.0f;
-            hi_b[a] = ((float)q[a] + (float)d) * 64.0f;
+            lo_b[a] = ((float)q[a] - (float)(e - 1u)) * 64.0f;
+            hi_b[a] = ((float)q[a] + (float)e) * 64.0f;
         }
     } else {
-        uint32_t row = S.free_idx[cell];
+        uint32_t row = (uint32_t)(cw >> 32);
         uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
-        uint64_t w = row == kFreeCell ? 0ull : S.free_bits[(uint64_t)row * 8u + (lb >> 6)];
-        if ((w >> (lb & 63u)) & 1ull) return false;
+        uint32_t eb = S.bext[((uint64_t)row * 8u + octant) * 256u + (lb >> 1)];
+        uint32_t sl = row < S.n_blocks ? S.l1[(uint64_t)row * 512u + lb] : RKF_EMPTY_SLOT;
+        eb = (eb >> ((lb & 1u) * 4u)) & 15u;
+        if (eb == 0) {
+            slot = sl;
+            return false;
+        }
+        // the cube of bricks b + [0, eb) in the travel direction of each axis is free
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            lo_b[a] = (float)(b[a] * 8u);
-            hi_b[a] = (float)(b[a] * 8u + 8u);
+            lo_b[a] = ((float)b[a] - (float)(eb - 1u)) * 8.0f;
+            hi_b[a] = ((float)b[a] + (float)eb) * 8.0f;
         }
     }
     float dv[3] = {sd.x, sd.y, sd.z};
@@ -219,14 +236,15 @@ RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &
     r.steps++;
     f3 pos = add(r.o, muls(r.sd, r.t));
     float tx;
-    if (sp_free(S, pos, r.sd, r.inv, r.t, tx)) {
+    uint32_t slot;
+    if (sp_free(S, pos, r.sd, r.inv, r.t, tx, slot)) {
         r.prev_t = tx;
         r.prev_dist = vs * 8.0f;
         r.t = tx + vs * 0.00390625f;
         return kRayContinue;
     }
     SpCtx c{&S, r.evals};
-    float d = sp_sample(c, pos, false, true);
+    float d = sp_sample(c, pos, false, true, slot);
     if (d < 0.0f && r.prev_dist > hit_eps(eps_mode, su.hit_threshold, r.prev_t)) {
         float lo = r.prev_t, hi = r.t, mt = r.t, md = d;
         for (uint32_t b = 0; b < kBisect; b++) {
@@ -287,12 +305,13 @@ RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) 
     r.steps++;
     f3 p = add(r.o, muls(r.sd, r.t));
     float tx;
-    if (sp_free(S, p, r.sd, r.inv, r.t, tx)) {
+    uint32_t slot;
+    if (sp_free(S, p, r.sd, r.inv, r.t, tx, slot)) {
         r.t = tx + S.vs * 0.00390625f;
         return false;
     }
     SpCtx c{&S, r.evals};
-    float d = sp_sample(c, p, false, true);
+    float d = sp_sample(c, p, false, true, slot);
     r.evals = c.evals;
     if (d < 0.005f) {
         r.prev_dist = 0.0f;

@@ -91,6 +91,89 @@ __global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, ui
     }
 }
 
+// Split variant. Primary pass: one lane per pixel, 8x8 pixels per wavefront;
+// misses are written at once, hits are appended (one atomic per wavefront,
+// ballot-ranked) to a compact list. Shading pass: persistent wavefronts over
+// the list (grid-stride) compute normal, material, G-buffer and the shadow
+// ray, so hit pixels run at full lane occupancy and the primary march carries
+// none of the shading registers. Same per-ray arithmetic as view_pixel.
+struct HitRec {
+    uint32_t px, py;
+    float t, mat_d;
+    float mp[3];
+    uint32_t steps, evals;
+};
+
+__global__ void hits_reset_kernel(uint32_t *count) { *count = 0u; }
+
+__global__ __launch_bounds__(64) void sparse_march_primary(SparseMarchParams P, HitRec *hits, uint32_t *count) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t px = blockIdx.x * 8u + (lane & 7u);
+    uint32_t py = blockIdx.y * 8u + (lane >> 3);
+    bool active = px < P.g.width && py < P.g.height;
+    int st = kRayMiss;
+    PrimRay r;
+    SpRes res;
+    if (active) {
+        f3 o, dir;
+        camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
+        prim_init(r, o, dir, P.S.vs);
+        do {
+            st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+        } while (st == kRayContinue);
+        if (st != kRayHit) {
+            f3 so;
+            (void)prim_finish(P, px, py, r, st, res, so);
+        }
+    }
+    bool hit = active && st == kRayHit;
+    uint64_t m = __ballot(hit);
+    if (m == 0ull) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0);
+    if (hit) {
+        uint32_t i = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        HitRec h;
+        h.px = px;
+        h.py = py;
+        h.t = res.t;
+        h.mat_d = res.mat_d;
+        h.mp[0] = res.mat_p.x;
+        h.mp[1] = res.mat_p.y;
+        h.mp[2] = res.mat_p.z;
+        h.steps = r.steps;
+        h.evals = r.evals;
+        hits[i] = h;
+    }
+}
+
+__global__ __launch_bounds__(64) void sparse_march_shade(SparseMarchParams P, const HitRec *hits,
+                                                         const uint32_t *count) {
+    const uint32_t n = *count;
+    const float maxd = shadow_maxd(P), k = shadow_k(P);
+    const f3 L = mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]);
+    for (uint32_t i = blockIdx.x * 64u + threadIdx.x; i < n; i += gridDim.x * 64u) {
+        HitRec h = hits[i];
+        PrimRay r;
+        r.steps = h.steps;
+        r.evals = h.evals;
+        SpRes res;
+        res.t = h.t;
+        res.mat_d = h.mat_d;
+        res.mat_p = mk(h.mp[0], h.mp[1], h.mp[2]);
+        res.hit = true;
+        f3 so;
+        if (prim_finish(P, h.px, h.py, r, kRayHit, res, so)) {
+            ShadowRay sr;
+            shadow_init(sr, so, L, P.S.vs);
+            while (!shadow_step(P.S, sr, maxd, k)) {
+            }
+            shadow_store(P, (uint64_t)h.py * P.g.width + h.px, sr);
+        }
+    }
+}
+
 // Point queries: one lane per point.
 __global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const float *xyz, uint32_t n,
                                                            rkfm_point_result *out) {
@@ -113,6 +196,21 @@ extern "C" hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, u
     uint32_t n = waves < tiles ? waves : tiles;
     if (n == 0) n = 1;
     hipLaunchKernelGGL(rkf::sparse_march_tuned, dim3(n), dim3(64), 0, stream, *P, n);
+    return hipGetLastError();
+}
+
+extern "C" uint64_t rkf_hit_record_bytes() { return sizeof(rkf::HitRec); }
+
+extern "C" hipError_t rkf_launch_sparse_march_split(const rkf::SparseMarchParams *P, void *hits, uint32_t *count,
+                                                    uint32_t shade_waves, hipStream_t stream) {
+    hipLaunchKernelGGL(rkf::hits_reset_kernel, dim3(1), dim3(1), 0, stream, count);
+    dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
+    hipLaunchKernelGGL(rkf::sparse_march_primary, grid, dim3(64, 1, 1), 0, stream, *P, (rkf::HitRec *)hits, count);
+    uint64_t most = ((uint64_t)P->g.width * P->g.height + 63u) / 64u;
+    uint32_t waves = (uint32_t)(most < shade_waves ? most : shade_waves);
+    if (waves == 0) waves = 1;
+    hipLaunchKernelGGL(rkf::sparse_march_shade, dim3(waves), dim3(64, 1, 1), 0, stream, *P,
+                       (const rkf::HitRec *)hits, (const uint32_t *)count);
     return hipGetLastError();
 }
 

@@ -70,7 +70,7 @@ _SIGS = {
     "rkfm_sparse_info_get": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_destroy": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
-    "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
@@ -465,14 +465,17 @@ class SparseScene:
         return l2, l1, dist, mat
 
     def download_free(self):
+        """(free_idx, free_bits, oct, brick_ext) as in rkfm_sparse_download_free."""
         nc = int(np.prod(self.info["cells"]))
         rows = np.zeros(1, np.uint64)
-        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, None, None, None, _p(rows)))
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, None, None, None, None, _p(rows)))
         fidx = np.zeros(nc, np.uint32)
         bits = np.zeros(max(int(rows[0]) * 8, 1), np.uint64)
-        prox = np.zeros(nc, np.uint8)
-        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, _p(fidx), _p(bits), _p(prox), _p(rows)))
-        return fidx, bits, prox
+        oct_ = np.zeros(nc, np.uint32)
+        bext = np.zeros(max(int(rows[0]) * 2048, 1), np.uint8)
+        _check(lib().rkfm_sparse_download_free(self.ctx.h, self.h, _p(fidx), _p(bits), _p(oct_), _p(bext),
+                                               _p(rows)))
+        return fidx, bits, oct_, bext
 
     def march(self, cam, su, ext, gbuffer, variant=0, stream=None):
         c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)

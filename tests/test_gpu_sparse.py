@@ -3,7 +3,7 @@
 * GPU voxeliser: every stored voxel distance and every allocation decision
   equals the brute-force oracle field (all primitives, all groups), bit for bit.
 * Free-space masks: equal to the oracle's dense masks.
-* March (plain and tuned variants), shadow rays and point queries: bit-exact
+* March (plain, tuned and split variants), shadow rays and point queries: bit-exact
   vs the FP32 twin; depth/normal within tolerance of the FP64 oracle.
 """
 import ctypes as C
@@ -108,8 +108,23 @@ def test_voxeliser_matches_brute_force_field(small_scene):
 def test_free_masks_match_oracle(small_scene):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)
-    fidx, bits, prox = sc.download_free()
-    assert np.array_equal(prox, sp.prox)
+    fidx, bits, oct_, bext = sc.download_free()
+    assert np.array_equal(oct_, sp.oct)
+    assert (oct_ != 0).any() and (oct_ == 0).any()
+    # brick extents of every row cell, all 8 octant planes
+    ref = sp.bext.reshape(-1, 512)
+    rows = np.flatnonzero(fidx != 0xFFFFFFFF)
+    planes = bext.reshape(-1, 8, 256)
+    for cell in rows:
+        pl = planes[fidx[cell]]
+        got = np.zeros(512, np.uint32)
+        for o in range(8):
+            nib = np.empty(512, np.uint32)
+            nib[0::2] = pl[o] & 15
+            nib[1::2] = pl[o] >> 4
+            got |= nib << np.uint32(4 * o)
+        assert np.array_equal(got, ref[cell]), cell
+    assert (ref[rows] != 0).any()
     cells = [int(c) for c in sc.info["cells"]]
     nb = [c * 8 for c in cells]
     for cz in range(cells[2]):
@@ -126,7 +141,7 @@ def test_free_masks_match_oracle(small_scene):
                     assert notfree == (0 if sp.brick_free(bx, by, bz) else 1)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_sparse_march_parity(ctx, small_scene, variant):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)

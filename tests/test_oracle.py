@@ -327,3 +327,82 @@ def test_sparse_field_equals_reference_field_at_origin():
     # DDA changes only free-space steps: depths agree within the hit threshold
     assert np.abs(ref["t"][hit] - spr["t"][hit]).max() < 2e-3
     assert np.array_equal(ref["material_id"][hit], spr["material_id"][hit])
+
+
+def test_octant_extents_match_brute_force():
+    """orc_sparse_oct against its definition: for a free cell C and octant o,
+    E = min over non-free cells M in o's quadrant of max_a |M_a - C_a|, capped
+    at 15 (no such cell: 15); E = 0 for non-free cells. Grids with a random
+    sprinkle of non-free cells, one cell hugging each face, and the empty grid."""
+    rng = np.random.default_rng(4)
+    for cells, p_busy in (((9, 7, 5), 0.05), ((20, 3, 18), 0.01), ((4, 4, 4), 0.0), ((17, 17, 17), 0.002)):
+        nx, ny, nz = cells
+        free = (rng.random(nx * ny * nz) >= p_busy).astype(np.uint8)
+        free[0] = 0 if p_busy else free[0]
+        got = O.sparse_oct(cells, free)
+        idx = np.arange(nx * ny * nz)
+        X, Y, Z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+        busy = np.flatnonzero(free == 0)
+        for c in rng.choice(idx, size=min(idx.size, 300), replace=False):
+            for o in range(8):
+                e = (int(got[c]) >> (4 * o)) & 15
+                if not free[c]:
+                    assert e == 0
+                    continue
+                s = [(-1 if (o >> a) & 1 else 1) for a in range(3)]
+                dx, dy, dz = (X[busy] - X[c]) * s[0], (Y[busy] - Y[c]) * s[1], (Z[busy] - Z[c]) * s[2]
+                inq = (dx >= 0) & (dy >= 0) & (dz >= 0)
+                want = 15
+                if inq.any():
+                    want = min(15, int(np.max(np.stack([dx[inq], dy[inq], dz[inq]]), axis=0).min()))
+                assert e == want, (cells, c, o, e, want)
+
+
+def test_brick_extents_are_sound():
+    """orc_sparse_bext: in cells that are not entirely free, a brick's octant
+    extent E spans a cube of bricks b + [0, E) (in the octant's directions)
+    that holds only free bricks (checked by brute force against the brick
+    free test); non-free bricks have E = 0; some extents exceed one brick and
+    some reach into neighbouring cells."""
+    rng = np.random.default_rng(6)
+    cells = (3, 2, 3)
+    nc = cells[0] * cells[1] * cells[2]
+    l2 = np.full(nc, 0xFFFFFFFF, np.uint32)
+    occ = [0, 4, 13]
+    l1 = np.full(len(occ) * 512, 0xFFFFFFFF, np.uint32)
+    nb = 0
+    for k, c in enumerate(occ):
+        l2[c] = k
+        for lb in rng.choice(512, size=6, replace=False):
+            l1[k * 512 + lb] = nb
+            nb += 1
+    sp = O.SparseInputs((0, 0, 0), 0.02, cells, l2, l1, np.zeros(nb * 512, np.uint16), np.zeros(nb * 512, np.uint8))
+    NX, NY, NZ = cells[0] * 8, cells[1] * 8, cells[2] * 8
+    free = np.zeros((NZ, NY, NX), bool)
+    for z in range(NZ):
+        for y in range(NY):
+            for x in range(NX):
+                free[z, y, x] = sp.brick_free(x, y, z)
+    ext = sp.bext.reshape(nc, 512)
+    checked = big = cross = 0
+    for cell in np.flatnonzero(sp.free_l2 == 0):
+        cx, cy, cz = cell % cells[0], (cell // cells[0]) % cells[1], cell // (cells[0] * cells[1])
+        for lb in range(512):
+            b = (cx * 8 + (lb & 7), cy * 8 + ((lb >> 3) & 7), cz * 8 + (lb >> 6))
+            for o in range(8):
+                e = (int(ext[cell, lb]) >> (4 * o)) & 15
+                if not free[b[2], b[1], b[0]]:
+                    assert e == 0
+                    continue
+                assert e >= 1
+                lo, hi = [], []
+                for a in range(3):
+                    if (o >> a) & 1:
+                        lo.append(max(b[a] - e + 1, 0)), hi.append(b[a] + 1)
+                    else:
+                        lo.append(b[a]), hi.append(min(b[a] + e, (NX, NY, NZ)[a]))
+                assert free[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]].all(), (cell, lb, o, e)
+                checked += 1
+                big += e > 1
+                cross += any((lo[a] >> 3) != (b[a] >> 3) or ((hi[a] - 1) >> 3) != (b[a] >> 3) for a in range(3))
+    assert checked > 10000 and big > 1000 and cross > 100
