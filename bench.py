@@ -16,7 +16,8 @@ G-buffer write per pixel, 4 B per shadow-ray result.
 
 --workload config0 runs the drop-in kernel on the reference layouts instead
 (64 voxelised spheres, 640x360); config2 the batched graph-captured launch;
-config3 the served request stream.
+config3 the served request stream; config4 the 4K terrain frame with 32 CSG
+edits per frame.
 """
 import argparse
 import json
@@ -277,12 +278,99 @@ def run_config3(args, ctx, M, L, scenes):
         s.destroy()
 
 
+def run_config4(args, ctx, M, L, scenes):
+    """Terrain stress case: 1 km x 1 km fBm heightfield (6 octaves) at
+    vs 0.05 unioned with 20k smooth-min rocks; per frame 32 random sphere
+    add/subtract edits (radius U[0.5,3] m, smooth union / smooth subtract,
+    blend 0.3 r) rewrite the affected bricks in one launch, then a 3840x2160
+    view plus one shadow ray per hit is marched. A step is one frame: edits +
+    march. Edits land on the terrain in front of the camera."""
+    prims, lo, hi = scenes.terrain_scene(seed=4)
+    t0 = time.time()
+    sc = M.SparseScene.build(ctx, prims, 0.05, lo, hi, smin_k=0.1, mat_seed=scenes.material_seed(4))
+    ctx.sync()
+    log(f"config4 scene: {int(sc.info['n_bricks'])} bricks, {int(sc.info['n_blocks'])} L1 blocks, cells "
+        f"{[int(c) for c in sc.info['cells']]}, {sc.info['device_bytes'] / 2**30:.2f} GiB, built in {time.time() - t0:.1f} s")
+    W, H = 3840, 2160
+    t = prims[0]
+    eye = (0.0, float(scenes.terrain_height(t, 0.0, -60.0)) + 14.0, -60.0)
+    tgt = (0.0, float(scenes.terrain_height(t, 0.0, 25.0)), 25.0)
+    cam = M.camera_look_at(eye, tgt, 60.0, W, H)
+    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
+    light = np.array([0.5, 1.0, 0.3])
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    rng = scenes.Pcg32(44)
+
+    def frame_edits():
+        ops = []
+        for k in range(32):
+            x, z = rng.uniform(-40.0, 40.0), rng.uniform(-20.0, 70.0)
+            r = rng.uniform(0.5, 3.0)
+            y = float(scenes.terrain_height(t, x, z)) + rng.uniform(-0.5, 0.5)
+            kind = L.EDIT_SMOOTH_UNION if k % 2 == 0 else L.EDIT_SMOOTH_SUBTRACT
+            ops.append(L.edit_op((x, y, z), kind, L.SHAPE_SPHERE, (r, 0.0, 0.0), 1.0, 0.3 * r, L.FALLOFF_SMOOTH,
+                                 1 + k % 15))
+        return np.array(ops, L.EDIT)
+
+    gbd = M.GBuffer(ctx, W, H, debug=True, shadow=True)
+    sc.march(cam, su, ext, gbd)
+    out = gbd.read()
+    hit = out["position"][:, 3] != L.MAX_FLOAT
+    nbytes = algorithmic_bytes(out["debug"], hit)
+    n_shadow = int(hit.sum())
+    dbg = out["debug"]
+    log(f"hits {n_shadow} / {W * H}; steps/ray mean {dbg['steps'].mean():.2f} p99 {np.percentile(dbg['steps'], 99):.0f}; "
+        f"shadow steps mean {dbg['shadow_steps'][hit].mean() if hit.any() else 0:.1f}; algorithmic bytes {nbytes / 1e9:.3f} GB")
+    del gbd, out
+    gb = M.GBuffer(ctx, W, H, shadow=True)
+    edits = [frame_edits() for _ in range(args.warmup + args.steps)]
+    mod = sc.edit(edits[0])
+    log(f"first frame's edits modified {len(mod)} bricks")
+    for k in range(1, args.warmup):
+        sc.edit(edits[k], report=False)
+        sc.march(cam, su, ext, gb)
+    ctx.sync()
+    # frame = edits + march
+    ctx.timer_begin()
+    for k in range(args.steps):
+        sc.edit(edits[args.warmup + k], report=False)
+        sc.march(cam, su, ext, gb)
+    ms = ctx.timer_end() / args.steps
+    ctx.sync()
+    ctx.timer_begin()
+    for k in range(args.steps):
+        sc.march(cam, su, ext, gb)
+    ms_march = ctx.timer_end() / args.steps
+    ctx.sync()
+    ctx.timer_begin()
+    for k in range(args.steps):
+        sc.edit(edits[k], report=False)
+    ms_edit = ctx.timer_end() / args.steps
+    ctx.sync()
+    rays = W * H + n_shadow
+    log(f"config4: frame {ms:.3f} ms (march {ms_march:.3f} ms, 32 edits {ms_edit:.3f} ms)")
+    line = {
+        "metric": "Mrays/s of the 4K terrain frame with 32 CSG edits per frame",
+        "value": round(rays / (ms / 1e3) / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+        "config": {"workload": "config4: 1 km^2 fBm terrain (6 octaves, vs 0.05) + 20k smooth-min rocks; 3840x2160 "
+                               "primary + 1 shadow ray per hit; 32 sphere add/subtract edits r~U[0.5,3] per frame",
+                   "bricks": int(sc.info["n_bricks"]), "scene_gib": round(float(sc.info["device_bytes"]) / 2**30, 2),
+                   "primary_rays": W * H, "shadow_rays": n_shadow, "march_ms": round(ms_march, 4),
+                   "edits_ms": round(ms_edit, 4),
+                   "march_algorithmic_gbs": round(nbytes / (ms_march / 1e3) / 1e9, 2)},
+    }
+    print(json.dumps(line), flush=True)
+    sc.destroy()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2", "config3"])
+    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2", "config3", "config4"])
     ap.add_argument("--requests", type=int, default=4000, help="config3: requests in the arrival trace")
     ap.add_argument("--variant", type=int, default=0,
                     help="sparse kernel: 0 per-pixel, 1 persistent refill, 2 primary + compacted shading pass")
@@ -301,8 +389,8 @@ def main():
     from rkifield_amd import scenes
 
     ctx = M.Context(0)
-    if args.workload in ("config2", "config3"):
-        (run_config2 if args.workload == "config2" else run_config3)(args, ctx, M, L, scenes)
+    if args.workload in ("config2", "config3", "config4"):
+        {"config2": run_config2, "config3": run_config3, "config4": run_config4}[args.workload](args, ctx, M, L, scenes)
         ctx.close()
         return
     if args.workload == "config1":

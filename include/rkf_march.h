@@ -250,9 +250,15 @@ int rkfm_dispatch(rkfm_ctx *ctx, const rkfm_gbuffer *out, void *hip_stream);
 
 /* Analytic primitive record used by the scene builders. type: 0 sphere
  * (p0 = radius), 1 box (p0..p2 = half extents), 2 capsule (p0 = radius,
- * p1 = half height along local y). rot is a row-major 3x3 world->local
- * rotation applied after subtracting centre. group: primitives with the same
- * group are blended with smin(k) (sdf.rs:48-51), groups are unioned by min. */
+ * p1 = half height along local y), 3 fBm heightfield terrain (world
+ * aligned, rot unused): d = p2 * ((y - centre.y) - h(x - centre.x,
+ * z - centre.z)) with h = p0 * sum_{i<6} 0.5^i * vnoise(seed + i, f_i x,
+ * f_i z), f_i = p1 * 2^i, vnoise the smoothstep-interpolated value noise of
+ * a splitmix64 lattice hash in [-1, 1) (the same height function terrain.rs
+ * evaluates as y - h, with fBm detail instead of sine layers). rot is a
+ * row-major 3x3 world->local rotation applied after subtracting centre.
+ * group: primitives with the same group are blended with smin(k)
+ * (sdf.rs:48-51), groups are unioned by min. */
 typedef struct rkfm_prim {
     float centre[3];
     uint32_t type;
@@ -260,7 +266,7 @@ typedef struct rkfm_prim {
     float rot[9];
     uint32_t group;
     uint32_t material; /* used when the scene has no per-voxel material hash */
-    uint32_t _pad;
+    uint32_t seed;     /* noise seed (type 3)                                */
 } rkfm_prim;
 
 /* voxelize_sdf (voxelize_object.rs:53-157) of one sphere |p| - r with the
@@ -380,6 +386,53 @@ int rkfm_sparse_download_free(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                               uint32_t *free_idx, uint64_t *free_bits,
                               uint32_t *oct, uint8_t *brick_ext,
                               uint64_t *n_rows);
+
+/* CSG / sculpt edits of a resident sparse scene: rkf-edit's EditOp
+ * (crates/rkf-edit/src/edit_op.rs:15-28, enums types.rs:19-124) applied as
+ * apply_edit_cpu (cpu_apply.rs:28-188) to the allocated bricks that
+ * find_affected_bricks (edit_op.rs:97-140) selects, the scene's global brick
+ * grid standing in for one object's brick map. */
+#define RKFM_EDIT_CSG_UNION 0u
+#define RKFM_EDIT_CSG_SUBTRACT 1u
+#define RKFM_EDIT_CSG_INTERSECT 2u
+#define RKFM_EDIT_SMOOTH_UNION 3u
+#define RKFM_EDIT_SMOOTH_SUBTRACT 4u
+#define RKFM_EDIT_SMOOTH 5u
+#define RKFM_EDIT_FLATTEN 6u
+#define RKFM_EDIT_PAINT 7u
+#define RKFM_EDIT_COLOR_PAINT 8u
+#define RKFM_SHAPE_SPHERE 0u
+#define RKFM_SHAPE_BOX 1u
+#define RKFM_SHAPE_CAPSULE 2u
+#define RKFM_SHAPE_CYLINDER 3u
+#define RKFM_SHAPE_TORUS 4u
+#define RKFM_SHAPE_PLANE 5u
+#define RKFM_FALLOFF_LINEAR 0u
+#define RKFM_FALLOFF_SMOOTH 1u
+#define RKFM_FALLOFF_SHARP 2u
+
+typedef struct rkfm_edit {
+    float position[3];      /* world (scene) space                          */
+    uint32_t edit_type;     /* RKFM_EDIT_*                                  */
+    float rotation[4];      /* unit quaternion x, y, z, w                   */
+    float dimensions[3];    /* shape parameters (types.rs:71-90)            */
+    uint32_t shape_type;    /* RKFM_SHAPE_*                                 */
+    float strength;
+    float blend_k;
+    uint32_t falloff;       /* RKFM_FALLOFF_*                               */
+    uint32_t material_id;   /* stored as its low 6 bits (voxel.rs:35)       */
+} rkfm_edit;
+
+/* Applies the edits in order (each voxel sees the previous edits' result)
+ * in one launch. When n_modified is not NULL the call waits for the device
+ * and reports the slots of bricks some edit changed by more than 0.1 vs in
+ * some voxel (or painted), sorted: *n_modified = their count, the first
+ * max_modified are copied to modified (may be NULL). Edits neither allocate
+ * nor free bricks. */
+int rkfm_sparse_edit(rkfm_ctx *ctx, rkfm_sparse_scene *scene,
+                     const rkfm_edit *ops, uint32_t n_ops, uint32_t *modified,
+                     uint64_t max_modified, uint64_t *n_modified,
+                     void *hip_stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched launch: many uneven ray/query batches, over any resident sparse   */
@@ -592,6 +645,7 @@ static_assert(sizeof(rkfm_request) == 312, "request record");
 static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
+static_assert(sizeof(rkfm_edit) == 64, "edit record");
 #else
 _Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
 _Static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
@@ -604,6 +658,7 @@ _Static_assert(sizeof(rkfm_request) == 312, "request record");
 _Static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 _Static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 _Static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
+_Static_assert(sizeof(rkfm_edit) == 64, "edit record");
 #endif
 
 #endif /* RKF_MARCH_H */

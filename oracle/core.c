@@ -183,9 +183,46 @@ float orc_smooth_min_cross(float a, float b, float k) {
     return fminf(a, b) - h * h * k * 0.25f;
 }
 
+/* fBm terrain (prim type 3, defined in include/rkf_march.h): value noise
+ * on a splitmix64-hashed integer lattice, smoothstep interpolation, 6
+ * octaves; d = p2 * ((y - cy) - p0 * h) as terrain.rs:229-246 (y - height). */
+static uint64_t terrain_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float terrain_lattice(uint32_t seed, int32_t i, int32_t j) {
+    uint64_t key = (uint64_t)(uint32_t)i | ((uint64_t)(uint32_t)j << 32);
+    uint64_t h = terrain_mix64(key ^ ((uint64_t)seed * 0x9E3779B97F4A7C15ull));
+    return (float)(uint32_t)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
+}
+static float terrain_vnoise(uint32_t seed, float u, float v) {
+    float fu = floorf(u), fv = floorf(v);
+    int32_t iu = (int32_t)fu, iv = (int32_t)fv;
+    float tu = u - fu, tv = v - fv;
+    float su = tu * tu * (3.0f - 2.0f * tu), sv = tv * tv * (3.0f - 2.0f * tv);
+    float c00 = terrain_lattice(seed, iu, iv), c10 = terrain_lattice(seed, iu + 1, iv);
+    float c01 = terrain_lattice(seed, iu, iv + 1), c11 = terrain_lattice(seed, iu + 1, iv + 1);
+    float a = c00 + (c10 - c00) * su;
+    float b = c01 + (c11 - c01) * su;
+    return a + (b - a) * sv;
+}
+float orc_terrain_height(const rkfm_prim *pr, float x, float z) {
+    float u = x - pr->centre[0], w = z - pr->centre[2];
+    float h = 0.0f, amp = 1.0f, freq = pr->params[1];
+    for (uint32_t i = 0; i < 6u; i++) {
+        h = h + amp * terrain_vnoise(pr->seed + i, u * freq, w * freq);
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    return pr->params[0] * h;
+}
+
 /* Scene-builder primitive: translate, rotate (row-major world->local), then
  * evaluate_primitive (voxelize_object.rs:439-474) for sphere/box/capsule. */
 float orc_prim_eval(const rkfm_prim *pr, const float p[3]) {
+    if (pr->type == 3u) return pr->params[2] * ((p[1] - pr->centre[1]) - orc_terrain_height(pr, p[0], p[2]));
     float d0 = p[0] - pr->centre[0], d1 = p[1] - pr->centre[1], d2 = p[2] - pr->centre[2];
     const float *R = pr->rot;
     float l[3];

@@ -80,6 +80,8 @@ float orc_capsule_sdf(const float a[3], const float b[3], float r, const float p
 float orc_smin(float a, float b, float k);
 /* gen_test_rkf.rs:22-25 variant with k.max(1e-6) */
 float orc_smooth_min_cross(float a, float b, float k);
+/* fBm terrain height of a type-3 primitive (without the distance scale). */
+float orc_terrain_height(const rkfm_prim *pr, float x, float z);
 float orc_prim_eval(const rkfm_prim *pr, const float p[3]);
 float orc_scene_field(const rkfm_prim *prims, uint32_t n, float k, const float p[3]);
 
@@ -208,6 +210,17 @@ void orc_sparse_oct(const orc_sparse_scene *s, const uint8_t *free_l2, uint32_t 
  * cube of cells C + [0, E_o(C)) on each axis). One sweep per octant. */
 void orc_sparse_bext(const orc_sparse_scene *s, const uint8_t *free_l2, const uint32_t *oct, uint32_t *out);
 int orc_sparse_brick_free(const orc_sparse_scene *s, uint32_t bx, uint32_t by, uint32_t bz);
+/* CSG / sculpt edits: for each op in order, find_affected_bricks
+ * (edit_op.rs:97-140; the scene's brick grid with origin as the grid
+ * corner) then apply_edit_cpu (cpu_apply.rs:28-188) brick by brick in z-y-x
+ * order, voxel by voxel. dist / mat are the scene's planes, updated in place
+ * (the scene's own pointers are only used for the index). modified[slot] is
+ * set to 1 for every brick an op reports modified. */
+void orc_sparse_edit(const orc_sparse_scene *s, uint16_t *dist, uint8_t *mat, const rkfm_edit *ops,
+                     uint32_t n_ops, uint8_t *modified);
+/* evaluate_shape / evaluate_falloff (cpu_apply.rs:231-281). */
+float orc_edit_shape(uint32_t shape, const float dims[3], const float p[3]);
+float orc_edit_falloff(uint32_t curve, float distance, float radius);
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
                       uint32_t width, uint32_t height, int prec, uint64_t p0,

@@ -361,3 +361,15 @@ def sparse_oct(cells, free_l2):
     out = np.zeros(f.size, np.uint32)
     lib().orc_sparse_oct(C.byref(s), _p(f), _p(out))
     return out
+
+
+def sparse_edit(sp, ops):
+    """orc_sparse_edit on copies of the scene's planes: returns (dist, mat,
+    sorted modified slots)."""
+    ops = np.ascontiguousarray(ops, L.EDIT)
+    dist = sp.dist.copy()
+    mat = sp.mat.copy()
+    n_bricks = dist.size // 512
+    mod = np.zeros(max(n_bricks, 1), np.uint8)
+    lib().orc_sparse_edit(C.byref(sp.s), _p(dist), _p(mat), _p(ops), C.c_uint32(ops.size), _p(mod))
+    return dist, mat, np.flatnonzero(mod[:n_bricks]).astype(np.uint32)

@@ -195,6 +195,169 @@ void orc_sparse_bext(const orc_sparse_scene *s, const uint8_t *free_l2, const ui
     }
 }
 
+/* ---- edits: cpu_apply.rs:28-281, edit_op.rs:86-140 ---- */
+static float e_len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+/* glam's Quat * Vec3 in scalar form: v (w^2 - b.b) + b (2 v.b) + (b x v)(2 w). */
+static void e_rotate(const float q[4], const float v[3], float o[3]) {
+    float bx = q[0], by = q[1], bz = q[2], w = q[3];
+    float b2 = bx * bx + by * by + bz * bz;
+    float sc = w * w - b2;
+    float d2 = (v[0] * bx + v[1] * by + v[2] * bz) * 2.0f;
+    float c[3] = {by * v[2] - bz * v[1], bz * v[0] - bx * v[2], bx * v[1] - by * v[0]};
+    float w2 = w * 2.0f;
+    o[0] = (v[0] * sc + bx * d2) + c[0] * w2;
+    o[1] = (v[1] * sc + by * d2) + c[1] * w2;
+    o[2] = (v[2] * sc + bz * d2) + c[2] * w2;
+}
+
+float orc_edit_shape(uint32_t shape, const float dims[3], const float p[3]) {
+    switch (shape) {
+    case RKFM_SHAPE_SPHERE: return e_len3(p[0], p[1], p[2]) - dims[0];
+    case RKFM_SHAPE_BOX: {
+        float q[3] = {fabsf(p[0]) - dims[0], fabsf(p[1]) - dims[1], fabsf(p[2]) - dims[2]};
+        float outside = e_len3(fmaxf(q[0], 0.0f), fmaxf(q[1], 0.0f), fmaxf(q[2], 0.0f));
+        float inside = fminf(fmaxf(q[0], fmaxf(q[1], q[2])), 0.0f);
+        return outside + inside;
+    }
+    case RKFM_SHAPE_CAPSULE: {
+        float a[3] = {0.0f, -dims[1], 0.0f}, b[3] = {0.0f, dims[1], 0.0f};
+        float pa[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+        float ba[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float h = (pa[0] * ba[0] + pa[1] * ba[1] + pa[2] * ba[2]) / (ba[0] * ba[0] + ba[1] * ba[1] + ba[2] * ba[2]);
+        h = h < 0.0f ? 0.0f : (h > 1.0f ? 1.0f : h);
+        return e_len3(pa[0] - ba[0] * h, pa[1] - ba[1] * h, pa[2] - ba[2] * h) - dims[0];
+    }
+    case RKFM_SHAPE_CYLINDER: {
+        float d_radial = e_len3(p[0], 0.0f, p[2]) - dims[0];
+        float d_height = fabsf(p[1]) - dims[1];
+        float ox = fmaxf(d_radial, 0.0f), oy = fmaxf(d_height, 0.0f);
+        return sqrtf(ox * ox + oy * oy) + fminf(fmaxf(d_radial, d_height), 0.0f);
+    }
+    case RKFM_SHAPE_TORUS: {
+        float qx = e_len3(p[0], 0.0f, p[2]) - dims[0];
+        return sqrtf(qx * qx + p[1] * p[1]) - dims[1];
+    }
+    default: return p[1] - dims[0];
+    }
+}
+
+float orc_edit_falloff(uint32_t curve, float distance, float radius) {
+    if (radius <= 0.0f || distance >= radius) return 0.0f;
+    float t = distance / radius;
+    switch (curve) {
+    case RKFM_FALLOFF_LINEAR: return 1.0f - t;
+    case RKFM_FALLOFF_SHARP: { float f = 1.0f - t; return f * f * f; }
+    default: return 1.0f - t * t * (3.0f - 2.0f * t);
+    }
+}
+
+static float e_smin(float a, float b, float k) {
+    float h = fmaxf(k - fabsf(a - b), 0.0f) / k;
+    return fminf(a, b) - h * h * k * 0.25f;
+}
+
+void orc_sparse_edit(const orc_sparse_scene *s, uint16_t *dist, uint8_t *mat, const rkfm_edit *ops,
+                     uint32_t n_ops, uint8_t *modified) {
+    const float vs = s->voxel_size, bs = vs * 8.0f;
+    const int32_t nb[3] = {(int32_t)s->cells[0] * 8, (int32_t)s->cells[1] * 8, (int32_t)s->cells[2] * 8};
+    for (uint32_t k = 0; k < n_ops; k++) {
+        const rkfm_edit *op = &ops[k];
+        const float max_dim = fmaxf(fmaxf(op->dimensions[0], op->dimensions[1]), op->dimensions[2]);
+        const float extent = max_dim + op->blend_k;
+        int32_t bmin[3], bmax[3];
+        for (int a = 0; a < 3; a++) {
+            float lo = floorf(((op->position[a] - extent) - s->origin[a]) / bs);
+            float hi = ceilf(((op->position[a] + extent) - s->origin[a]) / bs - 0.001f);
+            int32_t l = (int32_t)lo, h = (int32_t)hi;
+            bmin[a] = l < 0 ? 0 : l;
+            h = h < 0 ? 0 : h;
+            bmax[a] = h < nb[a] - 1 ? h : nb[a] - 1;
+        }
+        const float iq[4] = {-op->rotation[0], -op->rotation[1], -op->rotation[2], op->rotation[3]};
+        for (int32_t bz = bmin[2]; bz <= bmax[2]; bz++)
+            for (int32_t by = bmin[1]; by <= bmax[1]; by++)
+                for (int32_t bx = bmin[0]; bx <= bmax[0]; bx++) {
+                    uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * s->cells[0] +
+                                    (uint64_t)(bz >> 3) * s->cells[0] * s->cells[1];
+                    uint32_t blk = s->l2[cell];
+                    if (blk == RKF_EMPTY_SLOT) continue;
+                    uint32_t slot = s->l1[(uint64_t)blk * 512u + (bx & 7) + (by & 7) * 8 + (bz & 7) * 64];
+                    if (slot == RKF_EMPTY_SLOT || slot == RKF_INTERIOR_SLOT) continue;
+                    const float bmn[3] = {s->origin[0] + (float)bx * bs, s->origin[1] + (float)by * bs,
+                                          s->origin[2] + (float)bz * bs};
+                    int any = 0;
+                    for (uint32_t vz = 0; vz < 8; vz++)
+                        for (uint32_t vy = 0; vy < 8; vy++)
+                            for (uint32_t vx = 0; vx < 8; vx++) {
+                                const float vp[3] = {bmn[0] + (float)vx * vs + vs * 0.5f,
+                                                     bmn[1] + (float)vy * vs + vs * 0.5f,
+                                                     bmn[2] + (float)vz * vs + vs * 0.5f};
+                                const float rel[3] = {vp[0] - op->position[0], vp[1] - op->position[1],
+                                                      vp[2] - op->position[2]};
+                                float loc[3];
+                                e_rotate(iq, rel, loc);
+                                float shape_d = orc_edit_shape(op->shape_type, op->dimensions, loc);
+                                float w = orc_edit_falloff(op->falloff, e_len3(rel[0], rel[1], rel[2]),
+                                                           max_dim + op->blend_k);
+                                if (w <= 0.0f) continue;
+                                uint64_t vi = (uint64_t)slot * 512u + vx + vy * 8u + vz * 64u;
+                                float existing = orc_f16_to_f32(dist[vi]);
+                                uint32_t m = mat[vi];
+                                float strength = op->strength * w;
+                                float thr = vs * 0.1f;
+                                float nd = existing;
+                                uint32_t nm = m;
+                                uint32_t opm = op->material_id & 0x3Fu;
+                                switch (op->edit_type) {
+                                case RKFM_EDIT_CSG_UNION: {
+                                    float eff = shape_d + (1.0f - strength) * max_dim;
+                                    nd = fminf(existing, eff);
+                                    nm = eff < existing ? opm : ((nd < 0.0f && m == 0u) ? opm : m);
+                                    break;
+                                }
+                                case RKFM_EDIT_CSG_SUBTRACT: {
+                                    float eff = shape_d + (1.0f - strength) * max_dim;
+                                    nd = fmaxf(existing, -eff);
+                                    break;
+                                }
+                                case RKFM_EDIT_CSG_INTERSECT: {
+                                    float eff = shape_d - (1.0f - strength) * max_dim;
+                                    nd = fmaxf(existing, eff);
+                                    break;
+                                }
+                                case RKFM_EDIT_SMOOTH_UNION: {
+                                    float kk = fmaxf(op->blend_k, 0.001f);
+                                    float eff = shape_d + (1.0f - strength) * max_dim;
+                                    nd = e_smin(existing, eff, kk);
+                                    nm = eff < existing ? opm : ((nd < 0.0f && m == 0u) ? opm : m);
+                                    break;
+                                }
+                                case RKFM_EDIT_SMOOTH_SUBTRACT: {
+                                    float kk = fmaxf(op->blend_k, 0.001f);
+                                    float eff = shape_d + (1.0f - strength) * max_dim;
+                                    nd = -e_smin(-existing, eff, kk);
+                                    break;
+                                }
+                                case RKFM_EDIT_SMOOTH: nd = existing * (1.0f - strength * 0.3f); break;
+                                case RKFM_EDIT_FLATTEN: nd = existing + (loc[1] - existing) * (strength * 0.5f); break;
+                                case RKFM_EDIT_PAINT:
+                                    if (fabsf(existing) < op->dimensions[0] * 2.0f) {
+                                        mat[vi] = (uint8_t)opm;
+                                        any = 1;
+                                    }
+                                    continue;
+                                default: continue;
+                                }
+                                dist[vi] = orc_f32_to_f16(nd);
+                                mat[vi] = (uint8_t)nm;
+                                if (fabsf(nd - existing) > thr) any = 1;
+                            }
+                    if (any) modified[slot] = 1;
+                }
+    }
+}
+
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
                       const rkfm_scene_ext *ext, uint32_t w, uint32_t h, int prec, uint64_t p0, uint64_t p1,
                       orc_ray_result *out) {

@@ -110,6 +110,14 @@ int rkfm_destroy(rkfm_ctx *c) {
     c->batch_items.release();
     c->batch_begins.release();
     c->batch_counter.release();
+    c->march_hits.release();
+    c->march_count.release();
+    c->edit_ops.release();
+    c->edit_list.release();
+    c->edit_mod.release();
+    c->edit_count.release();
+    if (c->edit_host) (void)hipHostFree(c->edit_host);
+    if (c->edit_ev) (void)hipEventDestroy(c->edit_ev);
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);

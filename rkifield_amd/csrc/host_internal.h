@@ -27,6 +27,7 @@ struct rkfm_sparse_scene {
     rkfm_ctx *owner = nullptr;
     rkfm_sparse_info info{};
     DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, oct, cellw, bext;
+    DevBuf edit_touched;  // one bit per slot, all zero between edit calls
     uint64_t n_free_rows = 0;
     rkf::SparseDev dev() const;
 };
@@ -51,6 +52,13 @@ struct rkfm_ctx {
     uint32_t tuned_waves = 0;
     // Split sparse march: compact hit list and its counter.
     DevBuf march_hits, march_count;
+    // Edits: op records (and their pinned staging, reusable once edit_ev
+    // has fired), the listed bricks, their modified flags, the list count.
+    DevBuf edit_ops, edit_list, edit_mod, edit_count;
+    void *edit_host = nullptr;
+    uint64_t edit_host_cap = 0;
+    hipEvent_t edit_ev = nullptr;
+    bool edit_staged = false;
     // Batched launch: device descriptors (rewritten per call, stream-ordered),
     // their host staging copies, the persistent work counter, the tuning.
     DevBuf batch_items, batch_begins, batch_counter;

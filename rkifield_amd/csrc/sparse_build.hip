@@ -79,7 +79,38 @@ RKF_DEV float capsule_sdf(float r, float hh, float px, float py, float pz) {
     return __builtin_sqrtf(dx * dx + dy * dy + dz * dz) - r;
 }
 
+// fBm terrain (prim type 3): value noise on a hashed integer lattice,
+// smoothstep-interpolated, 6 octaves. The oracle evaluates the same
+// operations in the same order.
+RKF_DEV float terrain_lattice(uint32_t seed, int32_t i, int32_t j) {
+    uint64_t key = (uint64_t)(uint32_t)i | ((uint64_t)(uint32_t)j << 32);
+    uint64_t h = mix64(key ^ ((uint64_t)seed * 0x9E3779B97F4A7C15ull));
+    return (float)(uint32_t)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
+}
+RKF_DEV float terrain_vnoise(uint32_t seed, float u, float v) {
+    float fu = floorf(u), fv = floorf(v);
+    int32_t iu = (int32_t)fu, iv = (int32_t)fv;
+    float tu = u - fu, tv = v - fv;
+    float su = tu * tu * (3.0f - 2.0f * tu), sv = tv * tv * (3.0f - 2.0f * tv);
+    float c00 = terrain_lattice(seed, iu, iv), c10 = terrain_lattice(seed, iu + 1, iv);
+    float c01 = terrain_lattice(seed, iu, iv + 1), c11 = terrain_lattice(seed, iu + 1, iv + 1);
+    float a = c00 + (c10 - c00) * su;
+    float b = c01 + (c11 - c01) * su;
+    return a + (b - a) * sv;
+}
+RKF_DEV float terrain_sdf(const rkfm_prim &pr, float x, float y, float z) {
+    float u = x - pr.centre[0], w = z - pr.centre[2];
+    float h = 0.0f, amp = 1.0f, freq = pr.params[1];
+    for (uint32_t i = 0; i < 6u; i++) {
+        h = h + amp * terrain_vnoise(pr.seed + i, u * freq, w * freq);
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    return pr.params[2] * ((y - pr.centre[1]) - pr.params[0] * h);
+}
+
 RKF_DEV float prim_eval(const rkfm_prim &pr, float x, float y, float z) {
+    if (pr.type == 3u) return terrain_sdf(pr, x, y, z);
     float d0 = x - pr.centre[0], d1 = y - pr.centre[1], d2 = z - pr.centre[2];
     const float *R = pr.rot;
     float lx = R[0] * d0 + R[1] * d1 + R[2] * d2;
@@ -430,6 +461,7 @@ static void release_scene(rkfm_sparse_scene *s) {
     s->oct.release();
     s->cellw.release();
     s->bext.release();
+    s->edit_touched.release();
 }
 
 void rkf_sparse_release_all(rkfm_ctx *c) {
@@ -614,7 +646,9 @@ int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkf
     // Groups: ids compacted in ascending order, prims in original order.
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; i++) {
-        if (prims[i].type > 2) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: prim %u type", i);
+        if (prims[i].type > 3) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: prim %u type", i);
+        if (prims[i].type == 3 && !(prims[i].params[2] > 0.0f))
+            return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_build: terrain %u distance scale", i);
         order[i] = i;
     }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return prims[a].group < prims[b].group; });
@@ -632,6 +666,22 @@ int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkf
     std::vector<uint64_t> pairs;  // (cell << 32) | group
     for (uint32_t i = 0; i < n; i++) {
         const rkfm_prim &p = prims[i];
+        if (p.type == 3) {
+            // Terrain: |h| <= 1.96875 |p0|. Its group is binned into every
+            // cell that is not entirely above y = centre.y + H + M / p2 (above
+            // that the terrain's distance exceeds M); cells below keep it so
+            // that deep ground stays negative under partly buried groups.
+            double top = (double)p.centre[1] + 1.96875 * std::fabs((double)p.params[0]) + (double)M / p.params[2] + 1e-3;
+            int64_t yhi = (int64_t)std::floor((top - prm->bounds_min[1]) / cws);
+            yhi = std::min<int64_t>(yhi, (int64_t)cells[1] - 1);
+            for (int64_t z = 0; z < (int64_t)cells[2]; z++)
+                for (int64_t y = 0; y <= yhi; y++)
+                    for (int64_t x = 0; x < (int64_t)cells[0]; x++) {
+                        uint64_t cell = (uint64_t)x + (uint64_t)y * cells[0] + (uint64_t)z * cells[0] * cells[1];
+                        pairs.push_back((cell << 32) | gid_of[i]);
+                    }
+            continue;
+        }
         float R = p.type == 0 ? p.params[0]
                   : p.type == 1 ? std::sqrt(p.params[0] * p.params[0] + p.params[1] * p.params[1] + p.params[2] * p.params[2])
                                 : p.params[0] + p.params[1];

@@ -108,7 +108,7 @@ PRIM = np.dtype([
     ("rot", "<f4", (9,)),
     ("group", "<u4"),
     ("material", "<u4"),
-    ("_pad", "<u4"),
+    ("seed", "<u4"),
 ])
 
 SPARSE_PARAMS = np.dtype([
@@ -150,6 +150,16 @@ BATCH_ITEM = np.dtype([
 
 BATCH_TUNING = np.dtype([("schedule", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"), ("waves_per_cu", "<u4")])
 
+(EDIT_CSG_UNION, EDIT_CSG_SUBTRACT, EDIT_CSG_INTERSECT, EDIT_SMOOTH_UNION, EDIT_SMOOTH_SUBTRACT, EDIT_SMOOTH,
+ EDIT_FLATTEN, EDIT_PAINT, EDIT_COLOR_PAINT) = range(9)
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TORUS, SHAPE_PLANE = range(6)
+FALLOFF_LINEAR, FALLOFF_SMOOTH, FALLOFF_SHARP = range(3)
+
+EDIT = np.dtype([
+    ("position", "<f4", (3,)), ("edit_type", "<u4"), ("rotation", "<f4", (4,)), ("dimensions", "<f4", (3,)),
+    ("shape_type", "<u4"), ("strength", "<f4"), ("blend_k", "<f4"), ("falloff", "<u4"), ("material_id", "<u4"),
+])
+
 SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
 REQ_KEEP_ON_DEVICE = 1
 REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
@@ -190,6 +200,7 @@ assert PRIM.itemsize == 80
 assert POINT_RESULT.itemsize == 24
 assert GBUFFER_DESC.itemsize == 56
 assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
+assert EDIT.itemsize == 64
 assert SERVER_CONFIG.itemsize == 56
 assert REQUEST.itemsize == 312
 assert REQUEST_STATUS.itemsize == 64
@@ -215,4 +226,20 @@ def scene_ext(hit_eps_mode=EPS_ABSOLUTE, strategy=STRATEGY_AUTO, shadow_enable=0
     e["light_dir"] = light_dir
     e["shadow_max_dist"] = shadow_max_dist
     e["shadow_k"] = shadow_k
+    return e
+
+
+def edit_op(position, edit_type, shape_type, dimensions, strength=1.0, blend_k=0.0, falloff=FALLOFF_SMOOTH,
+            material_id=0, rotation=(0.0, 0.0, 0.0, 1.0)):
+    """An EditOp record (rkf-edit edit_op.rs:15-28)."""
+    e = np.zeros((), EDIT)
+    e["position"] = position
+    e["edit_type"] = edit_type
+    e["rotation"] = rotation
+    e["dimensions"] = dimensions
+    e["shape_type"] = shape_type
+    e["strength"] = strength
+    e["blend_k"] = blend_k
+    e["falloff"] = falloff
+    e["material_id"] = material_id
     return e

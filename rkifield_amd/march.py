@@ -73,6 +73,7 @@ _SIGS = {
     "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
+    "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
     "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
     "rkfm_free": (C.c_int, [_VP, _VP]),
     "rkfm_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, _U64]),
@@ -483,6 +484,20 @@ class SparseScene:
         e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
         _check(lib().rkfm_sparse_march(self.ctx.h, self.h, _p(c), _p(s), _p(e), gbuffer.desc.ctypes.data,
                                        variant, stream))
+
+    def edit(self, ops, report=True, stream=None):
+        """Applies CSG edits in order (rkfm_sparse_edit); with report=True
+        waits and returns the sorted slots of the modified bricks."""
+        o = np.ascontiguousarray(np.atleast_1d(ops), L.EDIT)
+        if not report:
+            _check(lib().rkfm_sparse_edit(self.ctx.h, self.h, o.ctypes.data, o.size, None, 0, None, stream))
+            return None
+        cap = max(int(self.info["n_bricks"]), 1)
+        out = np.zeros(cap, np.uint32)
+        n = np.zeros(1, np.uint64)
+        _check(lib().rkfm_sparse_edit(self.ctx.h, self.h, o.ctypes.data, o.size, out.ctypes.data, cap,
+                                      n.ctypes.data, stream))
+        return out[: int(n[0])]
 
     def query(self, xyz_dev, out_dev, n, stream=None):
         _check(lib().rkfm_sparse_query_points(self.ctx.h, self.h, _p(xyz_dev), n, _p(out_dev), stream))

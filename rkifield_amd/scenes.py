@@ -227,3 +227,87 @@ def served_trace(seed=3, n=4000, rate=2000.0, n_scenes=8):
             tgt = [rng.uniform(-16, 16) for _ in range(3)]
             out.append((t, "view", sc, (w, h, d * 64.0, tgt)))
     return out
+
+
+def _mix64_np(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def terrain_height(prim, x, z):
+    """Height of a type-3 terrain primitive at world (x, z), vectorised
+    (float32 steps in the order the builder uses; used to place rocks,
+    cameras and edits)."""
+    x = np.asarray(x, np.float32)
+    z = np.asarray(z, np.float32)
+    u = x - np.float32(prim["centre"][0])
+    w = z - np.float32(prim["centre"][2])
+    h = np.zeros(np.broadcast(u, w).shape, np.float32)
+    amp, freq = np.float32(1.0), np.float32(prim["params"][1])
+    with np.errstate(over="ignore"):
+        for i in range(6):
+            seed = np.uint64((int(prim["seed"]) + i) & 0xFFFFFFFF)
+            uu, vv = u * freq, w * freq
+            fu, fv = np.floor(uu), np.floor(vv)
+            iu, iv = fu.astype(np.int64), fv.astype(np.int64)
+            tu, tv = uu - fu, vv - fv
+            su = tu * tu * (np.float32(3.0) - np.float32(2.0) * tu)
+            sv = tv * tv * (np.float32(3.0) - np.float32(2.0) * tv)
+
+            def lat(a, b):
+                key = (a.astype(np.uint64) & np.uint64(0xFFFFFFFF)) | ((b.astype(np.uint64) & np.uint64(0xFFFFFFFF)) << np.uint64(32))
+                hh = _mix64_np(key ^ (seed * np.uint64(0x9E3779B97F4A7C15)))
+                return (hh >> np.uint64(40)).astype(np.float32) * np.float32(1.0 / 8388608.0) - np.float32(1.0)
+            c00, c10, c01, c11 = lat(iu, iv), lat(iu + 1, iv), lat(iu, iv + 1), lat(iu + 1, iv + 1)
+            a = c00 + (c10 - c00) * su
+            b = c01 + (c11 - c01) * su
+            h = h + amp * (a + (b - a) * sv)
+            amp = amp * np.float32(0.5)
+            freq = freq * np.float32(2.0)
+    return np.float32(prim["params"][0]) * h + np.float32(prim["centre"][1])
+
+
+def terrain_scene(seed=4, half=500.0, n_rocks=20000, amplitude=20.0, freq=1.0 / 160.0, scale=1.0,
+                  rock_size=(0.3, 2.5), group=8):
+    """Terrain workload: an fBm heightfield (type 3, 6 octaves, amplitude
+    `amplitude` m, base frequency `freq` 1/m, distance scale `scale`) over
+    [-half, half]^2 in x/z, and `n_rocks` spheres/boxes (size U[rock_size],
+    random rotation) sitting on it, blended with smin in groups of 8. Draws
+    (Pcg32(seed)) per rock: type, size, 3 aspect factors, quaternion (3),
+    x, z; a rock's centre sits half its size above the terrain. Returns
+    (prims, bounds_min, bounds_max)."""
+    rng = Pcg32(seed)
+    prims = np.zeros(1 + n_rocks, L.PRIM)
+    t = prims[0]
+    t["type"] = 3
+    t["params"][:3] = (amplitude, freq, scale)
+    t["rot"] = np.eye(3, dtype=np.float32).reshape(-1)
+    t["group"] = 0
+    t["seed"] = seed * 7919 + 1
+    for i in range(n_rocks):
+        ty = min(int(rng.uniform(0.0, 2.0)), 1)
+        s = rng.uniform(*rock_size)
+        asp = [rng.uniform(0.5, 1.0) for _ in range(3)]
+        q = rng.quaternion()
+        x = rng.uniform(-half, half)
+        z = rng.uniform(-half, half)
+        p = prims[1 + i]
+        p["type"] = ty
+        if ty == 0:
+            p["params"][0] = s * 0.5
+        else:
+            p["params"][:3] = [s * 0.5 * a for a in asp]
+        p["rot"] = quat_to_world_to_local(q).reshape(-1)
+        p["centre"] = (x, 0.0, z)
+        p["group"] = 1 + i // group
+        p["material"] = 1 + i % 15
+    xs, zs = prims["centre"][1:, 0], prims["centre"][1:, 2]
+    hs = terrain_height(t, xs, zs)
+    sizes = np.where(prims["type"][1:] == 0, prims["params"][1:, 0], prims["params"][1:, 1])
+    prims["centre"][1:, 1] = hs + sizes * np.float32(0.5)
+    top = amplitude * 1.96875 + rock_size[1] + 2.0
+    lo = np.array([-half, -top, -half], np.float32)
+    hi = np.array([half, top, half], np.float32)
+    return prims, lo, hi

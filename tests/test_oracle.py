@@ -5,6 +5,7 @@ Reference vectors used (file:line in the reference):
   voxel.rs:120-277, brick.rs:189-338, brick_map.rs:322-333, brick_pool.rs:198-540,
   sampling.rs:88-292, sdf.rs:58-167, scenes/test_cross.rkf + examples/gen_test_rkf.rs.
 """
+import ctypes as C
 import math
 import os
 
@@ -406,3 +407,114 @@ def test_brick_extents_are_sound():
                 big += e > 1
                 cross += any((lo[a] >> 3) != (b[a] >> 3) or ((hi[a] - 1) >> 3) != (b[a] >> 3) for a in range(3))
     assert checked > 10000 and big > 1000 and cross > 100
+
+
+# ---------------- CSG edits (cpu_apply.rs tests, edit_op.rs tests) ----------------
+
+def _one_brick(fill_d=1.0, fill_mat=0, vs=0.05):
+    """A one-cell scene whose brick (0,0,0) is slot 0, filled uniformly (the
+    reference tests' setup_pool_with_empty_brick, cpu_apply.rs:303-316)."""
+    l2 = np.zeros(1, np.uint32)
+    l1 = np.full(512, 0xFFFFFFFF, np.uint32)
+    l1[0] = 0
+    dist = np.full(512, O.f32_to_f16_bits([fill_d])[0], np.uint16)
+    mat = np.full(512, fill_mat, np.uint8)
+    return O.SparseInputs((0, 0, 0), vs, (1, 1, 1), l2, l1, dist, mat)
+
+
+def test_edit_sphere_union_adds_material():
+    """cpu_apply.rs:343-363 (sphere_union_adds_material)."""
+    sp = _one_brick(1.0, 0)
+    op = L.edit_op((0.2, 0.2, 0.2), L.EDIT_SMOOTH_UNION, L.SHAPE_SPHERE, (0.15, 0, 0), 1.0, 0.15 * 0.3,
+                   L.FALLOFF_SMOOTH, 5)
+    dist, mat, mod = O.sparse_edit(sp, [op])
+    assert list(mod) == [0]
+    v = 4 + 4 * 8 + 4 * 64
+    d = O.f16_bits_to_f32([dist[v]])[0]
+    assert d < 1.0
+    # closed form at voxel (4,4,4): centre 0.225^3, 0.0433 from the brush centre
+    r = np.sqrt(3) * 0.025
+    t = r / 0.195
+    w = 1 - t * t * (3 - 2 * t)
+    eff = (r - 0.15) + (1 - w) * 0.15
+    assert abs(d - eff) < 2e-3 and mat[v] == 5
+    # voxels outside the falloff radius keep their value and material
+    far = 7 + 7 * 8 + 0 * 64
+    assert dist[far] == sp.dist[far] or O.f16_bits_to_f32([dist[far]])[0] <= 1.0
+
+
+def test_edit_sphere_subtract_carves():
+    """cpu_apply.rs:366-399 (sphere_subtract_carves)."""
+    sp = _one_brick(-0.5, 2)
+    op = L.edit_op((0.2, 0.2, 0.2), L.EDIT_SMOOTH_SUBTRACT, L.SHAPE_SPHERE, (0.15, 0, 0), 1.0, 0.05,
+                   L.FALLOFF_SMOOTH, 0)
+    dist, mat, mod = O.sparse_edit(sp, [op])
+    assert len(mod) == 1
+    v = 4 + 4 * 8 + 4 * 64
+    assert O.f16_bits_to_f32([dist[v]])[0] > -0.5
+    assert np.all(mat == 2)
+
+
+def test_edit_paint_sets_material_near_surface():
+    """cpu_apply.rs:402-433 (paint_sets_material_near_surface)."""
+    sp = _one_brick(1.0, 0)
+    for x in range(8):
+        i = x + 4 * 8 + 4 * 64
+        sp.dist[i] = O.f32_to_f16_bits([0.01])[0]
+        sp.mat[i] = 1
+    op = L.edit_op((0.2, 0.2, 0.2), L.EDIT_PAINT, L.SHAPE_SPHERE, (0.5, 0, 0), 1.0, 0.0, L.FALLOFF_SMOOTH, 42)
+    dist, mat, mod = O.sparse_edit(sp, [op])
+    assert list(mod) == [0]
+    assert mat[4 + 4 * 8 + 4 * 64] == 42 & 0x3F
+    assert np.array_equal(dist, sp.dist)
+    # |d| = 1.0 is not below 2 * 0.5: untouched
+    assert mat[0] == 0
+
+
+def test_edit_outside_the_grid_touches_nothing():
+    """cpu_apply.rs:436-441 (no_affected_returns_empty) and edit_op.rs local_aabb."""
+    sp = _one_brick(1.0, 0)
+    op = L.edit_op((50.0, 50.0, 50.0), L.EDIT_CSG_UNION, L.SHAPE_SPHERE, (0.1, 0, 0), 1.0, 0.0, L.FALLOFF_SMOOTH, 1)
+    dist, mat, mod = O.sparse_edit(sp, [op])
+    assert len(mod) == 0 and np.array_equal(dist, sp.dist) and np.array_equal(mat, sp.mat)
+
+
+def test_edit_shape_and_falloff_closed_forms():
+    """cpu_apply.rs:446-462: sphere centre -r, box centre -half, smooth falloff 1 at the centre;
+    plus the other shapes at points with known distances."""
+    lib = O.lib()
+    lib.orc_edit_shape.restype = C.c_float
+    lib.orc_edit_shape.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.orc_edit_falloff.restype = C.c_float
+    lib.orc_edit_falloff.argtypes = [C.c_uint32, C.c_float, C.c_float]
+
+    def shape(s, dims, p):
+        d = np.asarray(dims, np.float32)
+        q = np.asarray(p, np.float32)
+        return lib.orc_edit_shape(s, d.ctypes.data, q.ctypes.data)
+    assert abs(shape(L.SHAPE_SPHERE, (1, 0, 0), (0, 0, 0)) + 1) < 1e-6
+    assert abs(shape(L.SHAPE_BOX, (0.5, 0.5, 0.5), (0, 0, 0)) + 0.5) < 1e-6
+    assert abs(shape(L.SHAPE_BOX, (1, 1, 1), (2, 2, 2)) - np.sqrt(3)) < 1e-6
+    assert abs(shape(L.SHAPE_CAPSULE, (0.5, 1, 0), (0, 3, 0)) - 1.5) < 1e-6
+    assert abs(shape(L.SHAPE_CYLINDER, (1, 2, 0), (3, 0, 0)) - 2.0) < 1e-6
+    assert abs(shape(L.SHAPE_TORUS, (2, 0.5, 0), (2, 0, 0)) + 0.5) < 1e-6
+    assert abs(shape(L.SHAPE_PLANE, (0.25, 0, 0), (5, 1, -3)) - 0.75) < 1e-6
+    assert abs(lib.orc_edit_falloff(L.FALLOFF_SMOOTH, 0.0, 1.0) - 1.0) < 1e-6
+    assert lib.orc_edit_falloff(L.FALLOFF_LINEAR, 1.0, 1.0) == 0.0
+    assert abs(lib.orc_edit_falloff(L.FALLOFF_SHARP, 0.5, 1.0) - 0.125) < 1e-7
+
+
+def test_terrain_height_generator_matches_oracle():
+    """The workload generator's vectorised height equals the builder's (oracle) height bit for bit."""
+    from rkifield_amd import scenes
+    pr, lo, hi = scenes.terrain_scene(n_rocks=0)
+    lib = O.lib()
+    lib.orc_terrain_height.restype = C.c_float
+    lib.orc_terrain_height.argtypes = [C.c_void_p, C.c_float, C.c_float]
+    t = np.ascontiguousarray(pr[:1])
+    rng = np.random.default_rng(1)
+    xs, zs = rng.uniform(-500, 500, 200).astype(np.float32), rng.uniform(-500, 500, 200).astype(np.float32)
+    got = scenes.terrain_height(t[0], xs, zs) - np.float32(t[0]["centre"][1])
+    want = np.array([lib.orc_terrain_height(t.ctypes.data, float(x), float(z)) for x, z in zip(xs, zs)], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.abs(want).max() < 20 * 1.96875 and np.abs(want).max() > 5
