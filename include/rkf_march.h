@@ -370,6 +370,41 @@ typedef struct rkfm_point_result {
 int rkfm_sparse_query_points(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                              const float *xyz_dev, uint32_t n,
                              rkfm_point_result *out_dev, void *hip_stream);
+/* Contact generation (rkf-physics generate_sdf_contacts,
+ * sdf_collision.rs:188-225) for many rigid bodies against the scene's field:
+ * each sample point of the body's shape (CollisionShape::sample_points,
+ * sdf_collision.rs:79-148) is placed at position + rotation * point; a point
+ * whose distance is below the body's threshold yields a contact with the
+ * central-difference normal (gradient_central_diff, eps 0.01) and
+ * penetration threshold - distance. A body's contacts are sorted deepest
+ * first (ties keep sample order). */
+#define RKFM_BODY_SPHERE 0u  /* dims[0] = radius                             */
+#define RKFM_BODY_BOX 1u     /* dims = half extents                           */
+#define RKFM_BODY_CAPSULE 2u /* dims[0] = half height, dims[1] = radius       */
+#define RKFM_MAX_CONTACTS 16u
+
+typedef struct rkfm_body {
+    float position[3];
+    uint32_t shape;          /* RKFM_BODY_*                                   */
+    float rotation[4];       /* unit quaternion x, y, z, w                    */
+    float dims[3];
+    float threshold;         /* contact_threshold                             */
+} rkfm_body;
+
+typedef struct rkfm_contact {
+    float position[3];
+    float normal[3];
+    float penetration;
+    uint32_t sample;         /* index of the shape's sample point             */
+} rkfm_contact;
+
+/* bodies, contacts (RKFM_MAX_CONTACTS per body) and counts (one per body)
+ * are device pointers. */
+int rkfm_sparse_contacts(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                         const rkfm_body *bodies, uint32_t n,
+                         rkfm_contact *contacts, uint32_t *counts,
+                         void *hip_stream);
+
 /* Free-space structures used by the DDA: free_idx has one u32 per L2 cell
  * (0xFFFFFFFF = every brick of the cell is free, else a row of free_bits);
  * free_bits holds 8 u64 per row, bit (x + 8y + 64z) set = that brick is not
@@ -646,6 +681,8 @@ static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 static_assert(sizeof(rkfm_edit) == 64, "edit record");
+static_assert(sizeof(rkfm_body) == 48, "body record");
+static_assert(sizeof(rkfm_contact) == 32, "contact record");
 #else
 _Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
 _Static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
@@ -659,6 +696,8 @@ _Static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 _Static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 _Static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 _Static_assert(sizeof(rkfm_edit) == 64, "edit record");
+_Static_assert(sizeof(rkfm_body) == 48, "body record");
+_Static_assert(sizeof(rkfm_contact) == 32, "contact record");
 #endif
 
 #endif /* RKF_MARCH_H */

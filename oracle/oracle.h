@@ -218,6 +218,11 @@ int orc_sparse_brick_free(const orc_sparse_scene *s, uint32_t bx, uint32_t by, u
  * set to 1 for every brick an op reports modified. */
 void orc_sparse_edit(const orc_sparse_scene *s, uint16_t *dist, uint8_t *mat, const rkfm_edit *ops,
                      uint32_t n_ops, uint8_t *modified);
+/* generate_sdf_contacts (sdf_collision.rs:188-225) per body over the FP32
+ * sparse field: out has RKFM_MAX_CONTACTS records per body, deepest first
+ * (stable), counts[i] = number of contacts of body i. */
+void orc_sparse_contacts(const orc_sparse_scene *s, const rkfm_body *bodies, uint32_t n, rkfm_contact *out,
+                         uint32_t *counts);
 /* evaluate_shape / evaluate_falloff (cpu_apply.rs:231-281). */
 float orc_edit_shape(uint32_t shape, const float dims[3], const float p[3]);
 float orc_edit_falloff(uint32_t curve, float distance, float radius);

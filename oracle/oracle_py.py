@@ -373,3 +373,12 @@ def sparse_edit(sp, ops):
     mod = np.zeros(max(n_bricks, 1), np.uint8)
     lib().orc_sparse_edit(C.byref(sp.s), _p(dist), _p(mat), _p(ops), C.c_uint32(ops.size), _p(mod))
     return dist, mat, np.flatnonzero(mod[:n_bricks]).astype(np.uint32)
+
+
+def sparse_contacts(sp, bodies):
+    """orc_sparse_contacts: (contacts[n, 16], counts[n])."""
+    b = np.ascontiguousarray(bodies, L.BODY)
+    out = np.zeros((max(b.size, 1), 16), L.CONTACT)
+    counts = np.zeros(max(b.size, 1), np.uint32)
+    lib().orc_sparse_contacts(C.byref(sp.s), _p(b), C.c_uint32(b.size), _p(out), _p(counts))
+    return out[: b.size], counts[: b.size]

@@ -195,6 +195,84 @@ void orc_sparse_bext(const orc_sparse_scene *s, const uint8_t *free_l2, const ui
     }
 }
 
+/* ---- contacts: sdf_collision.rs:79-148 (sample points), :188-225 ---- */
+static int body_points(const rkfm_body *b, float pts[16][3]) {
+    int n = 0;
+#define PUSH(X, Y, Z) do { pts[n][0] = (X); pts[n][1] = (Y); pts[n][2] = (Z); n++; } while (0)
+    if (b->shape == RKFM_BODY_SPHERE) {
+        float r = b->dims[0], d3 = r / sqrtf(3.0f);
+        PUSH(r, 0.0f, 0.0f); PUSH(-r, 0.0f, 0.0f); PUSH(0.0f, r, 0.0f);
+        PUSH(0.0f, -r, 0.0f); PUSH(0.0f, 0.0f, r); PUSH(0.0f, 0.0f, -r);
+        PUSH(d3, d3, d3); PUSH(d3, d3, -d3); PUSH(d3, -d3, d3); PUSH(d3, -d3, -d3);
+        PUSH(-d3, d3, d3); PUSH(-d3, d3, -d3); PUSH(-d3, -d3, d3); PUSH(-d3, -d3, -d3);
+    } else if (b->shape == RKFM_BODY_BOX) {
+        const float sgn[2] = {-1.0f, 1.0f};
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                for (int k = 0; k < 2; k++) PUSH(b->dims[0] * sgn[i], b->dims[1] * sgn[j], b->dims[2] * sgn[k]);
+        PUSH(b->dims[0], 0.0f, 0.0f); PUSH(-b->dims[0], 0.0f, 0.0f); PUSH(0.0f, b->dims[1], 0.0f);
+        PUSH(0.0f, -b->dims[1], 0.0f); PUSH(0.0f, 0.0f, b->dims[2]); PUSH(0.0f, 0.0f, -b->dims[2]);
+    } else if (b->shape == RKFM_BODY_CAPSULE) {
+        float hh = b->dims[0], r = b->dims[1];
+        PUSH(0.0f, hh + r, 0.0f); PUSH(r, hh, 0.0f); PUSH(-r, hh, 0.0f); PUSH(0.0f, hh, r); PUSH(0.0f, hh, -r);
+        PUSH(0.0f, -(hh + r), 0.0f); PUSH(r, -hh, 0.0f); PUSH(-r, -hh, 0.0f); PUSH(0.0f, -hh, r); PUSH(0.0f, -hh, -r);
+        PUSH(r, 0.0f, 0.0f); PUSH(-r, 0.0f, 0.0f);
+    }
+#undef PUSH
+    return n;
+}
+
+void orc_sparse_contacts(const orc_sparse_scene *s, const rkfm_body *bodies, uint32_t n, rkfm_contact *out,
+                         uint32_t *counts) {
+    for (uint32_t i = 0; i < n; i++) {
+        const rkfm_body *b = &bodies[i];
+        float pts[16][3];
+        int np = body_points(b, pts);
+        sctx_f32 c;
+        sp_init_f32(&c, s);
+        rkfm_contact list[16];
+        uint32_t m = 0;
+        for (int k = 0; k < np; k++) {
+            /* glam Quat * Vec3 in scalar form */
+            const float *q = b->rotation;
+            float bx = q[0], by = q[1], bz = q[2], w = q[3];
+            float b2 = bx * bx + by * by + bz * bz, sc = w * w - b2;
+            float *v = pts[k];
+            float d2 = (v[0] * bx + v[1] * by + v[2] * bz) * 2.0f, w2 = w * 2.0f;
+            float cr[3] = {by * v[2] - bz * v[1], bz * v[0] - bx * v[2], bx * v[1] - by * v[0]};
+            v3_f32 wp = mk_f32(b->position[0] + ((v[0] * sc + bx * d2) + cr[0] * w2),
+                               b->position[1] + ((v[1] * sc + by * d2) + cr[1] * w2),
+                               b->position[2] + ((v[2] * sc + bz * d2) + cr[2] * w2));
+            float d = sp_sample_f32(&c, wp, 0, 0);
+            if (!(d < b->threshold)) continue;
+            const float e = 0.01f;
+            float dx = sp_sample_f32(&c, add_f32(wp, mk_f32(e, 0, 0)), 0, 0) - sp_sample_f32(&c, sub_f32(wp, mk_f32(e, 0, 0)), 0, 0);
+            float dy = sp_sample_f32(&c, add_f32(wp, mk_f32(0, e, 0)), 0, 0) - sp_sample_f32(&c, sub_f32(wp, mk_f32(0, e, 0)), 0, 0);
+            float dz = sp_sample_f32(&c, add_f32(wp, mk_f32(0, 0, e)), 0, 0) - sp_sample_f32(&c, sub_f32(wp, mk_f32(0, 0, e)), 0, 0);
+            float rcp = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+            rkfm_contact ct;
+            ct.position[0] = wp.x; ct.position[1] = wp.y; ct.position[2] = wp.z;
+            if (isfinite(rcp) && rcp > 0.0f) {
+                ct.normal[0] = dx * rcp; ct.normal[1] = dy * rcp; ct.normal[2] = dz * rcp;
+            } else {
+                ct.normal[0] = ct.normal[1] = ct.normal[2] = 0.0f;
+            }
+            ct.penetration = b->threshold - d;
+            ct.sample = (uint32_t)k;
+            /* stable insertion: deepest first */
+            uint32_t at = m;
+            while (at > 0 && list[at - 1].penetration < ct.penetration) {
+                list[at] = list[at - 1];
+                at--;
+            }
+            list[at] = ct;
+            m++;
+        }
+        memcpy(out + (uint64_t)i * RKFM_MAX_CONTACTS, list, m * sizeof(rkfm_contact));
+        counts[i] = m;
+    }
+}
+
 /* ---- edits: cpu_apply.rs:28-281, edit_op.rs:86-140 ---- */
 static float e_len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 

@@ -101,6 +101,8 @@ hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t var
                                    hipStream_t stream);
 hipError_t rkf_occupancy_waves(uint32_t which, uint32_t *waves);
 uint64_t rkf_hit_record_bytes();
+hipError_t rkf_launch_sparse_contacts(const rkf::SparseDev *S, const rkfm_body *bodies, uint32_t n, rkfm_contact *out,
+                                      uint32_t *counts, hipStream_t stream);
 hipError_t rkf_launch_sparse_march_split(const rkf::SparseMarchParams *P, void *hits, uint32_t *count,
                                          uint32_t shade_waves, hipStream_t stream);
 hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,

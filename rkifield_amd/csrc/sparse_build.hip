@@ -983,6 +983,16 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
                          "sparse_march launch");
 }
 
+int rkfm_sparse_contacts(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkfm_body *bodies, uint32_t n,
+                         rkfm_contact *contacts, uint32_t *counts, void *stream) {
+    if (!c || !s || (n && (!bodies || !contacts || !counts)))
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "contacts: null argument");
+    if (n > (1u << 27)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "contacts: %u bodies", n);
+    rkf::SparseDev D = s->dev();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    return rkf_hip_check(rkf_launch_sparse_contacts(&D, bodies, n, contacts, counts, st), "contacts launch");
+}
+
 int rkfm_sparse_query_points(rkfm_ctx *c, const rkfm_sparse_scene *s, const float *xyz, uint32_t n,
                              rkfm_point_result *out, void *stream) {
     if (!c || !s || (n && (!xyz || !out))) return rkf_set_error(RKFM_ERR_INVALID_ARG, "query_points: null argument");

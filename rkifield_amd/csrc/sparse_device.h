@@ -527,4 +527,75 @@ RKF_DEV void point_query(const SparseDev &S, f3 p, rkfm_point_result &out) {
     out = r;
 }
 
+// CollisionShape::sample_points (sdf_collision.rs:79-148): sample k of a
+// body's shape in body-local space; false when the shape has no sample k.
+RKF_DEV bool body_sample(const rkfm_body &b, uint32_t k, f3 &p) {
+    if (b.shape == RKFM_BODY_SPHERE) {
+        if (k >= 14u) return false;
+        const float r = b.dims[0];
+        if (k < 6u) {
+            const float v = (k & 1u) ? -r : r;
+            p = k < 2u ? mk(v, 0.0f, 0.0f) : (k < 4u ? mk(0.0f, v, 0.0f) : mk(0.0f, 0.0f, v));
+            return true;
+        }
+        const float d3 = r / __builtin_sqrtf(3.0f);
+        const uint32_t j = k - 6u;  // signs in x, y, z order: +++ ++- +-+ +-- -++ -+- --+ ---
+        p = mk((j & 4u) ? -d3 : d3, (j & 2u) ? -d3 : d3, (j & 1u) ? -d3 : d3);
+        return true;
+    }
+    if (b.shape == RKFM_BODY_BOX) {
+        if (k >= 14u) return false;
+        const float hx = b.dims[0], hy = b.dims[1], hz = b.dims[2];
+        if (k < 8u) {  // corners: sx outer, sy, sz inner, -1 before +1
+            p = mk((k & 4u) ? hx : -hx, (k & 2u) ? hy : -hy, (k & 1u) ? hz : -hz);
+            return true;
+        }
+        const uint32_t j = k - 8u;  // face centres +x -x +y -y +z -z
+        const float v = j < 2u ? hx : (j < 4u ? hy : hz);
+        const float sv = (j & 1u) ? -v : v;
+        p = j < 2u ? mk(sv, 0.0f, 0.0f) : (j < 4u ? mk(0.0f, sv, 0.0f) : mk(0.0f, 0.0f, sv));
+        return true;
+    }
+    if (b.shape == RKFM_BODY_CAPSULE) {
+        if (k >= 12u) return false;
+        const float hh = b.dims[0], r = b.dims[1];
+        if (k < 10u) {  // each hemisphere: pole, +x, -x, +z, -z
+            const float y = k < 5u ? hh : -hh;
+            const uint32_t j = k % 5u;
+            if (j == 0u) p = mk(0.0f, k < 5u ? hh + r : -(hh + r), 0.0f);
+            else if (j == 1u) p = mk(r, y, 0.0f);
+            else if (j == 2u) p = mk(-r, y, 0.0f);
+            else if (j == 3u) p = mk(0.0f, y, r);
+            else p = mk(0.0f, y, -r);
+            return true;
+        }
+        p = mk(k == 10u ? r : -r, 0.0f, 0.0f);  // shaft midpoints
+        return true;
+    }
+    return false;
+}
+
+// glam Quat * Vec3 (scalar form): v (w^2 - b.b) + b (2 v.b) + (b x v)(2 w).
+RKF_DEV f3 quat_mul_vec(const float q[4], f3 v) {
+    const float bx = q[0], by = q[1], bz = q[2], w = q[3];
+    const float b2 = bx * bx + by * by + bz * bz;
+    const float s = w * w - b2;
+    const float d2 = (v.x * bx + v.y * by + v.z * bz) * 2.0f;
+    const float cx = by * v.z - bz * v.y, cy = bz * v.x - bx * v.z, cz = bx * v.y - by * v.x;
+    const float w2 = w * 2.0f;
+    return mk((v.x * s + bx * d2) + cx * w2, (v.y * s + by * d2) + cy * w2, (v.z * s + bz * d2) + cz * w2);
+}
+
+// SdfQueryable::gradient via gradient_central_diff (sdf_collision.rs:44-56).
+RKF_DEV f3 field_gradient(const SparseDev &S, f3 p) {
+    SpCtx c{&S, 0u};
+    const float e = 0.01f;
+    float dx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), false, false) - sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), false, false);
+    float dy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), false, false) - sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), false, false);
+    float dz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), false, false) - sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), false, false);
+    float rcp = 1.0f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    if (isfinite(rcp) && rcp > 0.0f) return mk(dx * rcp, dy * rcp, dz * rcp);
+    return mk(0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace rkf

@@ -183,7 +183,64 @@ __global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const fl
     point_query(S, p, out[i]);
 }
 
+// Contacts: 16 lanes per body (4 bodies per wavefront), one lane per sample
+// point; a lane's rank among the body's contacts (deeper first, then lower
+// sample index) comes from the 16 lanes' penetrations by shuffles.
+__global__ __launch_bounds__(256) void sparse_contacts_kernel(SparseDev S, const rkfm_body *bodies, uint32_t n,
+                                                              rkfm_contact *out, uint32_t *counts) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t body = gid >> 4, k = gid & 15u;
+    const uint32_t lane = threadIdx.x & 63u, base = lane & ~15u;
+    bool contact = false;
+    float pen = 0.0f;
+    f3 wp = mk(0.0f, 0.0f, 0.0f), nrm = mk(0.0f, 0.0f, 0.0f);
+    if (body < n) {
+        const rkfm_body b = bodies[body];
+        f3 lp;
+        if (body_sample(b, k, lp)) {
+            wp = add(mk(b.position[0], b.position[1], b.position[2]), quat_mul_vec(b.rotation, lp));
+            SpCtx c{&S, 0u};
+            const float d = sp_sample(c, wp, false, false);
+            if (d < b.threshold) {
+                contact = true;
+                pen = b.threshold - d;
+                nrm = field_gradient(S, wp);
+            }
+        }
+    }
+    const uint64_t m = __ballot(contact);
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < 16u; j++) {
+        const float pj = __shfl(pen, (int)(base + j));
+        const bool cj = (m >> (base + j)) & 1ull;
+        if (cj && (pj > pen || (pj == pen && j < k))) rank++;
+    }
+    if (body >= n) return;
+    if (contact) {
+        rkfm_contact r;
+        r.position[0] = wp.x;
+        r.position[1] = wp.y;
+        r.position[2] = wp.z;
+        r.normal[0] = nrm.x;
+        r.normal[1] = nrm.y;
+        r.normal[2] = nrm.z;
+        r.penetration = pen;
+        r.sample = k;
+        out[(uint64_t)body * RKFM_MAX_CONTACTS + rank] = r;
+    }
+    if (k == 0) counts[body] = (uint32_t)__popcll((m >> base) & 0xFFFFull);
+}
+
 }  // namespace rkf
+
+extern "C" hipError_t rkf_launch_sparse_contacts(const rkf::SparseDev *S, const rkfm_body *bodies, uint32_t n,
+                                                 rkfm_contact *out, uint32_t *counts, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t threads = (uint64_t)n * 16u;
+    hipLaunchKernelGGL(rkf::sparse_contacts_kernel, dim3((unsigned)((threads + 255u) / 256u)), dim3(256), 0, stream, *S,
+                       bodies, n, out, counts);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,
                                               hipStream_t stream) {

@@ -160,6 +160,12 @@ EDIT = np.dtype([
     ("shape_type", "<u4"), ("strength", "<f4"), ("blend_k", "<f4"), ("falloff", "<u4"), ("material_id", "<u4"),
 ])
 
+BODY_SPHERE, BODY_BOX, BODY_CAPSULE = 0, 1, 2
+MAX_CONTACTS = 16
+BODY = np.dtype([("position", "<f4", (3,)), ("shape", "<u4"), ("rotation", "<f4", (4,)), ("dims", "<f4", (3,)),
+                 ("threshold", "<f4")])
+CONTACT = np.dtype([("position", "<f4", (3,)), ("normal", "<f4", (3,)), ("penetration", "<f4"), ("sample", "<u4")])
+
 SERVE_BATCHED, SERVE_PER_REQUEST = 0, 1
 REQ_KEEP_ON_DEVICE = 1
 REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
@@ -201,6 +207,8 @@ assert POINT_RESULT.itemsize == 24
 assert GBUFFER_DESC.itemsize == 56
 assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
 assert EDIT.itemsize == 64
+assert BODY.itemsize == 48
+assert CONTACT.itemsize == 32
 assert SERVER_CONFIG.itemsize == 56
 assert REQUEST.itemsize == 312
 assert REQUEST_STATUS.itemsize == 64

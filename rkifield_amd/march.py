@@ -74,6 +74,7 @@ _SIGS = {
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
+    "rkfm_sparse_contacts": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
     "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
     "rkfm_free": (C.c_int, [_VP, _VP]),
     "rkfm_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, _U64]),
@@ -498,6 +499,12 @@ class SparseScene:
         _check(lib().rkfm_sparse_edit(self.ctx.h, self.h, o.ctypes.data, o.size, out.ctypes.data, cap,
                                       n.ctypes.data, stream))
         return out[: int(n[0])]
+
+    def contacts(self, bodies_dev, contacts_dev, counts_dev, n, stream=None):
+        """generate_sdf_contacts for n bodies (device arrays: BODY, CONTACT x 16
+        per body, u32 counts)."""
+        _check(lib().rkfm_sparse_contacts(self.ctx.h, self.h, _p(bodies_dev), n, _p(contacts_dev), _p(counts_dev),
+                                          stream))
 
     def query(self, xyz_dev, out_dev, n, stream=None):
         _check(lib().rkfm_sparse_query_points(self.ctx.h, self.h, _p(xyz_dev), n, _p(out_dev), stream))

@@ -518,3 +518,68 @@ def test_terrain_height_generator_matches_oracle():
     want = np.array([lib.orc_terrain_height(t.ctypes.data, float(x), float(z)) for x, z in zip(xs, zs)], np.float32)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert np.abs(want).max() < 20 * 1.96875 and np.abs(want).max() > 5
+
+
+# ---------------- contacts (sdf_collision.rs tests) ----------------
+
+def _ground_plane_scene(vs=0.05, half=(1.6, 1.2, 1.6)):
+    """y = 0 ground plane voxelised into a dense grid centred on the origin
+    (bricks within 1.8 bricks of the plane, exact per-voxel distances)."""
+    bws = vs * 8
+    dims = [int(np.ceil(2 * h / bws)) for h in half]
+    origin = np.array([-d * bws / 2 for d in dims], np.float32)
+    m = np.full(dims[0] * dims[1] * dims[2], L.EMPTY_SLOT, np.uint32)
+    bricks = []
+    for bz in range(dims[2]):
+        for by in range(dims[1]):
+            for bx in range(dims[0]):
+                cy = origin[1] + by * bws + bws / 2
+                if abs(cy) >= 1.8 * bws:
+                    continue
+                m[bx + by * dims[0] + bz * dims[0] * dims[1]] = len(bricks)
+                v = np.arange(512)
+                y = origin[1] + by * bws + ((v >> 3) & 7) * vs + vs / 2
+                b = np.zeros(512, L.VOXEL_SAMPLE)
+                b["word0"] = O.f32_to_f16_bits(y.astype(np.float32)).astype(np.uint32) | (1 << 16)
+                bricks.append(b)
+    return _sparse_from_dense_host(dims, m, np.concatenate(bricks), np.float32(vs), origin)
+
+
+def _body(pos, shape, dims, threshold=0.02, rot=(0, 0, 0, 1)):
+    b = np.zeros((), L.BODY)
+    b["position"], b["shape"], b["dims"], b["threshold"], b["rotation"] = pos, shape, dims, threshold, rot
+    return b
+
+
+def test_contacts_ground_plane_vectors():
+    """sdf_collision.rs:484-550: a sphere (r 0.5) above the ground has no
+    contacts; at y 0.3 it has contacts with upward normals; at y 0 the deepest
+    contact is the bottom pole with penetration ~0.52; contacts come deepest
+    first (:646-676)."""
+    sp = _ground_plane_scene()
+    bodies = np.array([_body((0, 2, 0), L.BODY_SPHERE, (0.5, 0, 0)), _body((0, 0.3, 0), L.BODY_SPHERE, (0.5, 0, 0)),
+                       _body((0, 0, 0), L.BODY_SPHERE, (0.5, 0, 0))], L.BODY)
+    ct, n = O.sparse_contacts(sp, bodies)
+    assert n[0] == 0 and n[1] > 0 and n[2] > 0
+    assert np.all(ct[1]["normal"][: n[1], 1] > 0.9)
+    assert abs(ct[2]["penetration"][0] - 0.52) < 0.05 and ct[2]["sample"][0] == 3  # bottom pole (0,-r,0)
+    pens = ct[2]["penetration"][: n[2]]
+    assert np.all(np.diff(pens) <= 0)
+
+
+def test_contact_sample_points_match_shapes():
+    """sdf_collision.rs:578-645: 14 points for a sphere at radius r, 14 for a
+    box (8 corners + 6 face centres), 12 for a capsule; with a huge
+    threshold every point is a contact whose position is the sample point."""
+    sp = _ground_plane_scene()
+    big = 1e9
+    bodies = np.array([_body((0, 0.2, 0), L.BODY_SPHERE, (0.3, 0, 0), big),
+                       _body((0, 0.2, 0), L.BODY_BOX, (0.1, 0.2, 0.3), big),
+                       _body((0, 0.2, 0), L.BODY_CAPSULE, (0.25, 0.1, 0), big)], L.BODY)
+    ct, n = O.sparse_contacts(sp, bodies)
+    assert list(n) == [14, 14, 12]
+    rel = [ct[i]["position"][: n[i]] - np.float32([0, 0.2, 0]) for i in range(3)]
+    assert np.allclose(np.linalg.norm(rel[0], axis=1), 0.3, atol=1e-6)
+    corners = rel[1][np.argsort(ct[1]["sample"][: n[1]])][:8]
+    assert np.allclose(np.abs(corners), [0.1, 0.2, 0.3], atol=1e-6)
+    assert np.isclose(rel[2][:, 1].max(), 0.35, atol=1e-6) and np.isclose(rel[2][:, 1].min(), -0.35, atol=1e-6)
