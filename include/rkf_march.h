@@ -45,6 +45,7 @@ extern "C" {
 #define RKFM_ERR_LAYOUT (-5)       /* inconsistent layout (e.g. map dims)    */
 #define RKFM_ERR_NOT_READY (-6)    /* required input never uploaded          */
 #define RKFM_ERR_BUSY (-7)         /* request queue full                     */
+#define RKFM_ERR_FORMAT (-8)       /* malformed or unsupported asset data    */
 
 /* ------------------------------------------------------------------------ */
 /* Reference constants                                                       */
@@ -469,6 +470,65 @@ int rkfm_sparse_edit(rkfm_ctx *ctx, rkfm_sparse_scene *scene,
                      uint64_t max_modified, uint64_t *n_modified,
                      void *hip_stream);
 
+/* .rkf v2 object assets (crates/rkf-core/src/asset_file.rs): read on the
+ * host (header, LOD table, LZ4 data blocks), written (save_object,
+ * :380-460, LODs stored coarsest first), and streamed into resident sparse
+ * scenes. Host-only functions: no device needed except for from_rkf. */
+typedef struct rkfm_rkf_header {  /* RkfFileHeader, asset_file.rs:84-108 */
+    char magic[4];                 /* "RKF2"                                 */
+    uint32_t version;              /* 2                                      */
+    uint32_t lod_count;
+    uint32_t material_count;
+    float aabb_min[3];
+    float aabb_max[3];
+    uint32_t analytical_type;      /* 0 none, 1 sphere .. 6 plane            */
+    float analytical_params[4];
+    uint16_t material_ids[16];
+    uint8_t _reserved[36];
+} rkfm_rkf_header;
+
+typedef struct rkfm_rkf_lod {     /* LodFileEntry, asset_file.rs:118-136     */
+    float voxel_size;
+    uint32_t brick_count;
+    uint32_t brick_dims[3];
+    uint32_t _pad;
+    uint64_t data_offset;
+    uint32_t compressed_size;
+    uint32_t uncompressed_size;
+} rkfm_rkf_lod;
+
+typedef struct rkfm_rkf_lod_src { /* SaveLodLevel, asset_file.rs:152-170    */
+    float voxel_size;
+    uint32_t dims[3];
+    const uint32_t *map;           /* slots or RKF_EMPTY_SLOT; renumbered in   */
+                                   /* order of first appearance               */
+    const rkf_voxel_sample *bricks;/* brick_count x 512, in that order         */
+    uint32_t brick_count;
+    uint32_t _pad;
+} rkfm_rkf_lod_src;
+
+/* RKFM_ERR_FORMAT: bad magic / version / sizes; copies up to max_lods
+ * entries of the LOD table. */
+int rkfm_rkf_read_header(const void *file, uint64_t len, rkfm_rkf_header *hdr,
+                         rkfm_rkf_lod *lods, uint32_t max_lods);
+/* Decodes LOD `lod` (index into the stored, coarsest-first table): map gets
+ * dims product u32 (bricks numbered 0..n-1), bricks n x 512 samples. Either
+ * may be NULL. RKFM_ERR_OUT_OF_RANGE for a missing LOD, RKFM_ERR_FORMAT for
+ * corrupt data. */
+int rkfm_rkf_decode_lod(const void *file, uint64_t len, uint32_t lod,
+                        uint32_t *map, rkf_voxel_sample *bricks);
+/* Writes an asset (magic, version and LOD count are filled in; aabb,
+ * analytical bound and materials are taken from hdr). out == NULL: *len
+ * receives the size. */
+int rkfm_rkf_encode(const rkfm_rkf_header *hdr, const rkfm_rkf_lod_src *lods,
+                    uint32_t n_lods, void *out, uint64_t cap, uint64_t *len);
+/* Decodes LOD `lod` and makes it a resident sparse scene whose grid corner
+ * sits at `origin` (NULL: -dims * brick / 2, the centred grid voxelize_sdf
+ * builds). */
+int rkfm_sparse_from_rkf(rkfm_ctx *ctx, const void *file, uint64_t len,
+                         uint32_t lod, const float origin[3],
+                         rkfm_sparse_scene **out);
+
 /* ------------------------------------------------------------------------ */
 /* Batched launch: many uneven ray/query batches, over any resident sparse   */
 /* scenes, in one kernel launch (capturable in a HIP graph). Each batch is   */
@@ -682,6 +742,8 @@ static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 static_assert(sizeof(rkfm_edit) == 64, "edit record");
 static_assert(sizeof(rkfm_body) == 48, "body record");
+static_assert(sizeof(rkfm_rkf_header) == 128, "RkfFileHeader is 128 bytes");
+static_assert(sizeof(rkfm_rkf_lod) == 40, "LodFileEntry is 40 bytes");
 static_assert(sizeof(rkfm_contact) == 32, "contact record");
 #else
 _Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
@@ -697,6 +759,8 @@ _Static_assert(sizeof(rkfm_request_status) == 64, "request status record");
 _Static_assert(sizeof(rkfm_server_stats) == 88, "server stats record");
 _Static_assert(sizeof(rkfm_edit) == 64, "edit record");
 _Static_assert(sizeof(rkfm_body) == 48, "body record");
+_Static_assert(sizeof(rkfm_rkf_header) == 128, "RkfFileHeader is 128 bytes");
+_Static_assert(sizeof(rkfm_rkf_lod) == 40, "LodFileEntry is 40 bytes");
 _Static_assert(sizeof(rkfm_contact) == 32, "contact record");
 #endif
 

@@ -160,6 +160,14 @@ EDIT = np.dtype([
     ("shape_type", "<u4"), ("strength", "<f4"), ("blend_k", "<f4"), ("falloff", "<u4"), ("material_id", "<u4"),
 ])
 
+RKF_HEADER = np.dtype([("magic", "S4"), ("version", "<u4"), ("lod_count", "<u4"), ("material_count", "<u4"),
+                       ("aabb_min", "<f4", (3,)), ("aabb_max", "<f4", (3,)), ("analytical_type", "<u4"),
+                       ("analytical_params", "<f4", (4,)), ("material_ids", "<u2", (16,)), ("_reserved", "u1", (36,))])
+RKF_LOD = np.dtype([("voxel_size", "<f4"), ("brick_count", "<u4"), ("brick_dims", "<u4", (3,)), ("_pad", "<u4"),
+                    ("data_offset", "<u8"), ("compressed_size", "<u4"), ("uncompressed_size", "<u4")])
+RKF_LOD_SRC = np.dtype([("voxel_size", "<f4"), ("dims", "<u4", (3,)), ("map", "<u8"), ("bricks", "<u8"),
+                        ("brick_count", "<u4"), ("_pad", "<u4")])
+
 BODY_SPHERE, BODY_BOX, BODY_CAPSULE = 0, 1, 2
 MAX_CONTACTS = 16
 BODY = np.dtype([("position", "<f4", (3,)), ("shape", "<u4"), ("rotation", "<f4", (4,)), ("dims", "<f4", (3,)),
@@ -207,6 +215,7 @@ assert POINT_RESULT.itemsize == 24
 assert GBUFFER_DESC.itemsize == 56
 assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
 assert EDIT.itemsize == 64
+assert RKF_HEADER.itemsize == 128 and RKF_LOD.itemsize == 40 and RKF_LOD_SRC.itemsize == 40
 assert BODY.itemsize == 48
 assert CONTACT.itemsize == 32
 assert SERVER_CONFIG.itemsize == 56

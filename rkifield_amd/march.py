@@ -34,7 +34,7 @@ class RkfMarchError(RuntimeError):
         self.code = code
 
 
-ERR = dict(OK=0, INVALID_ARG=-1, OUT_OF_RANGE=-2, OOM=-3, DEVICE=-4, LAYOUT=-5, NOT_READY=-6, BUSY=-7)
+ERR = dict(OK=0, INVALID_ARG=-1, OUT_OF_RANGE=-2, OOM=-3, DEVICE=-4, LAYOUT=-5, NOT_READY=-6, BUSY=-7, FORMAT=-8)
 
 _VP = C.c_void_p
 _U32 = C.c_uint32
@@ -75,6 +75,10 @@ _SIGS = {
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
     "rkfm_sparse_contacts": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
+    "rkfm_rkf_read_header": (C.c_int, [_VP, _U64, _VP, _VP, _U32]),
+    "rkfm_rkf_decode_lod": (C.c_int, [_VP, _U64, _U32, _VP, _VP]),
+    "rkfm_rkf_encode": (C.c_int, [_VP, _VP, _U32, _VP, _U64, _VP]),
+    "rkfm_sparse_from_rkf": (C.c_int, [_VP, _VP, _U64, _U32, _VP, C.POINTER(_VP)]),
     "rkfm_alloc": (C.c_int, [_VP, _U64, C.POINTER(_VP)]),
     "rkfm_free": (C.c_int, [_VP, _VP]),
     "rkfm_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, _U64]),
@@ -451,6 +455,15 @@ class SparseScene:
                                             C.byref(h)))
         return cls(ctx, h.value)
 
+    @classmethod
+    def from_rkf(cls, ctx, data, lod=0, origin=None):
+        """Resident scene from a .rkf v2 asset (bytes); origin None = centred grid."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        o = None if origin is None else np.asarray(origin, np.float32)
+        h = _VP()
+        _check(lib().rkfm_sparse_from_rkf(ctx.h, buf.ctypes.data, buf.size, lod, _p(o), C.byref(h)))
+        return cls(ctx, h.value)
+
     def destroy(self):
         if self.h:
             _check(lib().rkfm_sparse_destroy(self.ctx.h, self.h))
@@ -770,3 +783,57 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def rkf_read_header(data):
+    """(.rkf header record, LOD table) of an asset."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    h = np.zeros(1, L.RKF_HEADER)
+    _check(lib().rkfm_rkf_read_header(buf.ctypes.data, buf.size, h.ctypes.data, None, 0))
+    lods = np.zeros(max(int(h[0]["lod_count"]), 1), L.RKF_LOD)
+    _check(lib().rkfm_rkf_read_header(buf.ctypes.data, buf.size, h.ctypes.data, lods.ctypes.data, lods.size))
+    return h[0], lods[: int(h[0]["lod_count"])]
+
+
+def rkf_decode_lod(data, lod=0):
+    """(dims, brick map, bricks) of one LOD of an asset."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    h, lods = rkf_read_header(data)
+    if lod >= len(lods):
+        m = np.zeros(1, np.uint32)
+        _check(lib().rkfm_rkf_decode_lod(buf.ctypes.data, buf.size, lod, m.ctypes.data, None))
+    e = lods[lod]
+    dims = np.array(e["brick_dims"], np.uint32)
+    m = np.zeros(max(int(np.prod(dims)), 1), np.uint32)
+    b = np.zeros(max(int(e["brick_count"]) * 512, 1), L.VOXEL_SAMPLE)
+    _check(lib().rkfm_rkf_decode_lod(buf.ctypes.data, buf.size, lod, m.ctypes.data, b.ctypes.data))
+    return dims, m[: int(np.prod(dims))], b[: int(e["brick_count"]) * 512]
+
+
+def rkf_encode(levels, aabb_min=(0, 0, 0), aabb_max=(1, 1, 1), material_ids=(), analytical=(0, (0, 0, 0, 0))):
+    """save_object: levels = [(voxel_size, dims, map, bricks)], map holding
+    slots (renumbered in order of first appearance), bricks in that order."""
+    hdr = np.zeros(1, L.RKF_HEADER)
+    hdr["aabb_min"], hdr["aabb_max"] = aabb_min, aabb_max
+    mats = list(material_ids)[:16]
+    hdr["material_count"] = len(mats)
+    hdr[0]["material_ids"][: len(mats)] = mats
+    hdr["analytical_type"] = analytical[0]
+    hdr["analytical_params"] = analytical[1]
+    src = np.zeros(len(levels), L.RKF_LOD_SRC)
+    keep = []
+    for i, (vs, dims, m, b) in enumerate(levels):
+        m = np.ascontiguousarray(m, np.uint32)
+        b = np.ascontiguousarray(b, L.VOXEL_SAMPLE).reshape(-1)
+        keep += [m, b]
+        src[i]["voxel_size"] = vs
+        src[i]["dims"] = dims
+        src[i]["map"] = m.ctypes.data if m.size else 0
+        src[i]["bricks"] = b.ctypes.data if b.size else 0
+        src[i]["brick_count"] = b.size // 512
+    n = np.zeros(1, np.uint64)
+    _check(lib().rkfm_rkf_encode(hdr.ctypes.data, src.ctypes.data, len(levels), None, 0, n.ctypes.data))
+    out = np.zeros(int(n[0]), np.uint8)
+    _check(lib().rkfm_rkf_encode(hdr.ctypes.data, src.ctypes.data, len(levels), out.ctypes.data, out.size,
+                                 n.ctypes.data))
+    return out.tobytes()

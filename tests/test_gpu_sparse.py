@@ -195,7 +195,8 @@ def test_point_queries_parity(ctx, small_scene):
 
 
 def test_from_dense_matches_reference_layout(ctx):
-    """test_cross.rkf streamed into a resident sparse scene at the origin."""
+    """test_cross.rkf streamed into a resident sparse scene at the origin,
+    by the oracle's reader + from_dense and by the product's .rkf loader."""
     import os
     data = open(os.path.join(os.path.dirname(__file__), "golden", "test_cross.rkf"), "rb").read()
     h, m, b = O.rkf_load(data)
@@ -217,6 +218,19 @@ def test_from_dense_matches_reference_layout(ctx):
     assert np.array_equal(gpu["material"], mat)
     assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
     assert np.array_equal(gpu["debug"]["brick_slot"], res["brick_slot"])
+    # the same asset through the product's .rkf loader (centred grid by
+    # default): identical index, pool and march output
+    _say("case: from_rkf test_cross")
+    sr = M.SparseScene.from_rkf(ctx, data)
+    assert np.array_equal(sr.info["origin"], sc.info["origin"])
+    for x, y in zip(sr.download(), sc.download()):
+        assert np.array_equal(x, y)
+    gb2 = M.GBuffer(ctx, w, hh, debug=True)
+    sr.march(cam, su, ext, gb2, variant=0)
+    got = gb2.read()
+    for k in ("position", "normal", "material", "motion", "debug"):
+        assert np.array_equal(got[k].view(np.uint8), gpu[k].view(np.uint8)), k
+    sr.destroy()
     sc.destroy()
 
 

@@ -1,0 +1,135 @@
+"""The product's .rkf v2 reader / writer (host code of librkf_march.so, no
+device needed): the golden fixture decodes exactly as the oracle's
+independent reader, and the reference's own asset tests
+(crates/rkf-core/src/asset_file_tests.rs) hold for save -> load round trips.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from rkifield_amd import layouts as L
+from rkifield_amd import march as M
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _vs(dist, mat):
+    v = np.zeros(512, L.VOXEL_SAMPLE)
+    v["word0"] = int(O.f32_to_f16_bits([dist])[0]) | ((mat & 0x3F) << 16)
+    return v
+
+
+def _small_lod(voxel_size):
+    """asset_file_tests.rs:15-32: 2x2x2 map, slot 10 at (0,0,0), 20 at (1,0,0)."""
+    m = np.full(8, L.EMPTY_SLOT, np.uint32)
+    m[0], m[1] = 10, 20
+    return (voxel_size, (2, 2, 2), m, np.concatenate([_vs(1.0, 1), _vs(2.0, 2)]))
+
+
+def test_golden_fixture_matches_oracle_reader():
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, lods = M.rkf_read_header(data)
+    assert h["magic"] == b"RKF2" and h["version"] == 2 and h["lod_count"] == 1
+    assert np.allclose(h["aabb_max"], (0.96, 0.48, 0.96)) and np.allclose(h["aabb_min"], (-0.96, -0.48, -0.96))
+    assert lods[0]["brick_count"] == 224 and list(lods[0]["brick_dims"]) == [8, 4, 8]
+    dims, m, b = M.rkf_decode_lod(data)
+    oh, om, ob = O.rkf_load(data)
+    assert np.array_equal(m, om) and np.array_equal(b.view(np.uint32), ob.view(np.uint32))
+    # the product's own voxeliser writes the same bricks (gen_test_rkf.rs)
+    vd, vm, vb = M.voxelize_test_cross()
+    assert np.array_equal(vm, m) and np.array_equal(vb.view(np.uint32), b.view(np.uint32))
+
+
+def test_roundtrip_single_lod():
+    """asset_file_tests.rs:65-101."""
+    data = M.rkf_encode([_small_lod(0.02)], material_ids=[1, 2])
+    h, lods = M.rkf_read_header(data)
+    assert len(lods) == 1 and lods[0]["brick_count"] == 2 and list(lods[0]["brick_dims"]) == [2, 2, 2]
+    assert lods[0]["voxel_size"] == np.float32(0.02)
+    assert list(h["material_ids"][: h["material_count"]]) == [1, 2]
+    dims, m, b = M.rkf_decode_lod(data)
+    assert m[0] != L.EMPTY_SLOT and m[1] != L.EMPTY_SLOT and m[0] != m[1]
+    b = b.reshape(2, 512)
+    assert O.f16_bits_to_f32([b[m[0]][0]["word0"] & 0xFFFF])[0] == 1.0 and (b[m[0]][0]["word0"] >> 16) & 0x3F == 1
+    assert O.f16_bits_to_f32([b[m[1]][0]["word0"] & 0xFFFF])[0] == 2.0 and (b[m[1]][0]["word0"] >> 16) & 0x3F == 2
+
+
+def test_roundtrip_multi_lod_sorted_coarsest_first():
+    """asset_file_tests.rs:104-142."""
+    data = M.rkf_encode([_small_lod(0.02), _small_lod(0.08), _small_lod(0.005)])
+    h, lods = M.rkf_read_header(data)
+    assert [float(x) for x in lods["voxel_size"]] == [np.float32(0.08), np.float32(0.02), np.float32(0.005)]
+    for i in range(3):
+        dims, m, b = M.rkf_decode_lod(data, i)
+        assert (m != L.EMPTY_SLOT).sum() == 2 and b.size == 1024
+
+
+def test_header_fields_roundtrip():
+    """asset_file_tests.rs:145-162, 222-310: AABB, material ids and the analytical bound survive."""
+    for kind, params in ((0, (0, 0, 0, 0)), (1, (0.75, 0, 0, 0)), (2, (1, 2, 3, 0)), (3, (0.3, 0.8, 0, 0))):
+        data = M.rkf_encode([_small_lod(0.02)], aabb_min=(-5, -3, 0.5), aabb_max=(5, 3, 1.5),
+                            material_ids=[5, 6, 7], analytical=(kind, params))
+        h, _ = M.rkf_read_header(data)
+        assert np.allclose(h["aabb_min"], (-5, -3, 0.5)) and np.allclose(h["aabb_max"], (5, 3, 1.5))
+        assert list(h["material_ids"][: h["material_count"]]) == [5, 6, 7]
+        assert h["analytical_type"] == kind and np.allclose(h["analytical_params"], params)
+
+
+def test_errors_invalid_magic_and_lod_range():
+    """asset_file_tests.rs:167-193."""
+    data = bytearray(M.rkf_encode([_small_lod(0.02)]))
+    with pytest.raises(M.RkfMarchError) as e:
+        M.rkf_decode_lod(data, 99)
+    assert e.value.code == M.ERR["OUT_OF_RANGE"]
+    data[0:4] = b"XXXX"
+    with pytest.raises(M.RkfMarchError) as e:
+        M.rkf_read_header(bytes(data))
+    assert e.value.code == M.ERR["FORMAT"]
+    with pytest.raises(M.RkfMarchError):
+        M.rkf_read_header(b"RKF2")
+
+
+def test_empty_brick_map():
+    """asset_file_tests.rs:196-218."""
+    m = np.full(64, L.EMPTY_SLOT, np.uint32)
+    data = M.rkf_encode([(0.02, (4, 4, 4), m, np.zeros(0, L.VOXEL_SAMPLE))])
+    h, lods = M.rkf_read_header(data)
+    assert lods[0]["brick_count"] == 0
+    dims, mm, b = M.rkf_decode_lod(data)
+    assert list(dims) == [4, 4, 4] and np.all(mm == L.EMPTY_SLOT) and b.size == 0
+
+
+def test_lz4_cross_decodes_with_the_oracle_and_rejects_corruption():
+    """Blocks written by the product decode with the oracle's independent LZ4
+    reader (and the product's), for the fixture, repetitive and random data;
+    a corrupted block is rejected, not mis-read."""
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    dims, m, b = M.rkf_decode_lod(data)
+    rng = np.random.default_rng(0)
+    rnd = np.zeros(3 * 512, L.VOXEL_SAMPLE)
+    rnd["word0"] = rng.integers(0, 2**32, rnd.size, dtype=np.uint64).astype(np.uint32)
+    rnd["word1"] = rng.integers(0, 2**32, rnd.size, dtype=np.uint64).astype(np.uint32)
+    for lvl in ((0.03, dims, m, b), (0.02, (3, 1, 1), np.array([0, 1, 2], np.uint32), rnd), _small_lod(0.05)):
+        out = M.rkf_encode([lvl])
+        oh, om, ob = O.rkf_load(out)
+        pd, pm, pb = M.rkf_decode_lod(out)
+        assert np.array_equal(om, pm) and np.array_equal(ob.view(np.uint32), pb.view(np.uint32))
+        assert np.array_equal(pb.view(np.uint32), np.ascontiguousarray(lvl[3]).view(np.uint32))
+    out = bytearray(M.rkf_encode([(0.03, dims, m, b)]))
+    h, lods = M.rkf_read_header(bytes(out))
+    start = int(lods[0]["data_offset"]) + 4
+    assert len(out) - start < b.nbytes  # it compressed
+    bad = 0
+    for k in range(20):
+        c = bytearray(out)
+        pos = start + int(rng.integers(0, len(out) - start))
+        c[pos] ^= 0xFF
+        try:
+            _, cm, cb = M.rkf_decode_lod(bytes(c))
+            assert not (np.array_equal(cm, m) and np.array_equal(cb.view(np.uint32), b.view(np.uint32)))
+        except M.RkfMarchError as err:
+            assert err.code == M.ERR["FORMAT"]
+            bad += 1
+    assert bad > 0
