@@ -298,7 +298,8 @@ def run_config4(args, ctx, M, L, scenes):
     cam = M.camera_look_at(eye, tgt, 60.0, W, H)
     su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
     light = np.array([0.5, 1.0, 0.3])
-    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light),
+                      march_flags=0 if args.no_cull else L.MARCH_TILE_CULL)
     rng = scenes.Pcg32(44)
 
     def frame_edits():
@@ -357,7 +358,8 @@ def run_config4(args, ctx, M, L, scenes):
         "config": {"workload": "config4: 1 km^2 fBm terrain (6 octaves, vs 0.05) + 20k smooth-min rocks; 3840x2160 "
                                "primary + 1 shadow ray per hit; 32 sphere add/subtract edits r~U[0.5,3] per frame",
                    "bricks": int(sc.info["n_bricks"]), "scene_gib": round(float(sc.info["device_bytes"]) / 2**30, 2),
-                   "primary_rays": W * H, "shadow_rays": n_shadow, "march_ms": round(ms_march, 4),
+                   "primary_rays": W * H, "shadow_rays": n_shadow, "tile_cull": not args.no_cull,
+                   "march_ms": round(ms_march, 4),
                    "edits_ms": round(ms_edit, 4),
                    "march_algorithmic_gbs": round(nbytes / (ms_march / 1e3) / 1e9, 2)},
     }
@@ -374,6 +376,8 @@ def main():
     ap.add_argument("--requests", type=int, default=4000, help="config3: requests in the arrival trace")
     ap.add_argument("--variant", type=int, default=0,
                     help="sparse kernel: 0 per-pixel, 1 persistent refill, 2 primary + compacted shading pass")
+    ap.add_argument("--no-cull", action="store_true",
+                    help="config1/config4: march without per-tile ray intervals")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -396,6 +400,7 @@ def main():
     if args.workload == "config1":
         width, height = 1920, 1080
         sc, cam, su, ext = build_config1(ctx, M, L, scenes, width, height)
+        ext["march_flags"] = 0 if args.no_cull else L.MARCH_TILE_CULL
         launch = lambda gb: sc.march(cam, su, ext, gb, variant=args.variant)
         workload = ("config1: 4096 sphere/box/capsule prims, size U[0.1,4] m, smin k=0.1 groups of 8, "
                     "256 m cube, vs 0.02 m two-level sparse index; 1920x1080 primary + 1 shadow ray per hit")
@@ -491,7 +496,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded splitmix64/PCG32 scene and camera)",
         "config": {"workload": workload, "kernel": kernel, "width": width, "height": height,
-                   "primary_rays": n_primary, "shadow_rays": n_shadow},
+                   "primary_rays": n_primary, "shadow_rays": n_shadow,
+                   "tile_cull": args.workload == "config1" and not args.no_cull},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,

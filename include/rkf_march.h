@@ -146,11 +146,20 @@ typedef struct rkf_bvh_node {
 #define RKFM_STRATEGY_BVH 2u
 #define RKFM_STRATEGY_BRUTE 3u
 
+/* Sparse scenes: bound every 8x8-pixel tile's primary rays by the occupied
+ * L2 cells its frustum can meet (rkfm_sparse_tile_intervals). A ray starts
+ * at its tile's t_lo (one skip, prev_t = t_lo, prev_dist = 8 vs, like a
+ * free-space skip) and ends as a miss past t_hi; a tile that meets no
+ * occupied cell misses after that one skip. Outside [t_lo, t_hi] the field
+ * along the ray is the free value, so only the path through free space
+ * changes, as with the free-space DDA. */
+#define RKFM_MARCH_TILE_CULL 1u
+
 typedef struct rkfm_scene_ext {
     uint32_t hit_eps_mode;   /* RKFM_EPS_*                                  */
     uint32_t strategy;       /* RKFM_STRATEGY_*                             */
     uint32_t shadow_enable;  /* 1: trace one soft-shadow ray per hit        */
-    uint32_t _reserved0;
+    uint32_t march_flags;    /* RKFM_MARCH_* (sparse scenes)                */
     float light_dir[3];      /* unit vector toward the light                */
     float shadow_max_dist;   /* 0 => 25.0  (shade_common.wgsl:276)          */
     float shadow_k;          /* 0 => 16.0  (shade_common.wgsl:275)          */
@@ -358,6 +367,21 @@ int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
                       const rkfm_gbuffer *out, uint32_t variant,
                       void *hip_stream);
+
+/* Per-tile ray intervals used by RKFM_MARCH_TILE_CULL: tiles_dev receives
+ * ceil(width/8) * ceil(height/8) pairs (t_lo, t_hi) of floats, row-major.
+ * For each L2 cell holding an L1 block, its box widened by 9 vs (by 25 vs on
+ * the sides that lie on the grid boundary, where samples clamp) is projected
+ * with the camera basis; every tile whose pixel rays can meet the box takes
+ * t_lo = min(distance from the camera to the box) and t_hi = max(distance
+ * to the box's farthest corner) over those cells. Boxes beyond
+ * su->max_distance are skipped; a tile that meets none keeps
+ * (+inf, 0). Requires |jitter| <= 0.5. */
+int rkfm_sparse_tile_intervals(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                               const rkf_camera_uniforms *camera,
+                               const rkf_scene_uniforms *su, uint32_t width,
+                               uint32_t height, float *tiles_dev,
+                               void *hip_stream);
 
 /* Point queries (SdfQueryable semantics, sdf_collision.rs:38-56; result
  * fields as SpatialQueryResult automation/types.rs:151-158 plus gradient). */

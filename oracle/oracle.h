@@ -232,6 +232,14 @@ void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       uint64_t p1, orc_ray_result *out);
 void orc_sparse_query(const orc_sparse_scene *s, const float *xyz, uint64_t n,
                       int prec, rkfm_point_result *out);
+/* Tile intervals of RKFM_MARCH_TILE_CULL (same definition as
+ * rkfm_sparse_tile_intervals): out gets (lo, hi) per 8x8 tile, row-major;
+ * prec 32 values are the float results widened. When ext->march_flags has
+ * RKFM_MARCH_TILE_CULL, orc_sparse_trace(_parallel) compute them at the
+ * trace's precision and march every ray within its tile's interval. */
+void orc_sparse_tiles(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
+                      const rkf_scene_uniforms *su, uint32_t width, uint32_t height,
+                      int prec, double *out);
 void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t n, const float L[3],
                        float max_dist, float k, int prec, float *visibility);
 /* Sparse field value at a world point (stepping semantics). */

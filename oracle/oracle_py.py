@@ -319,6 +319,17 @@ def sparse_trace_parallel(sp: SparseInputs, cam, su, ext, width, height, p0=0, p
     return out, used
 
 
+def sparse_tiles(sp: SparseInputs, cam, su, width, height, prec=32):
+    """(t_lo, t_hi) per 8x8 tile, shape (tiles_y, tiles_x, 2); prec 32 as float32."""
+    tx, ty = (width + 7) // 8, (height + 7) // 8
+    out = np.zeros((ty, tx, 2), np.float64)
+    cam = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+    su = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+    lib().orc_sparse_tiles(C.byref(sp.s), _p(cam), _p(su), C.c_uint32(width), C.c_uint32(height), C.c_int(prec),
+                           _p(out))
+    return out.astype(np.float32) if prec == 32 else out
+
+
 def sparse_query(sp: SparseInputs, xyz, prec=32):
     xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
     out = np.zeros(xyz.shape[0], L.POINT_RESULT)

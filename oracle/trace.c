@@ -436,19 +436,53 @@ void orc_sparse_edit(const orc_sparse_scene *s, uint16_t *dist, uint8_t *mat, co
     }
 }
 
+void orc_sparse_tiles(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
+                      uint32_t w, uint32_t h, int prec, double *out) {
+    uint64_t n = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u;
+    if (prec == 64) {
+        sp_tiles_f64(s, cam, (double)su->max_distance, w, h, out);
+        return;
+    }
+    float *t = (float *)malloc(n * sizeof(float));
+    sp_tiles_f32(s, cam, su->max_distance, w, h, t);
+    for (uint64_t i = 0; i < n; i++) out[i] = (double)t[i];
+    free(t);
+}
+
+static int tile_cull(const rkfm_scene_ext *ext) { return ext && (ext->march_flags & RKFM_MARCH_TILE_CULL); }
+
 void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
                       const rkfm_scene_ext *ext, uint32_t w, uint32_t h, int prec, uint64_t p0, uint64_t p1,
                       orc_ray_result *out) {
-    for (uint64_t p = p0; p < p1; p++) {
-        if (prec == 64) sp_pixel_f64(s, cam, su, ext, w, h, p, out + (p - p0));
-        else sp_pixel_f32(s, cam, su, ext, w, h, p, out + (p - p0));
+    uint64_t nt = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u;
+    float *t32 = NULL;
+    double *t64 = NULL;
+    if (tile_cull(ext)) {
+        if (prec == 64) {
+            t64 = (double *)malloc(nt * sizeof(double));
+            sp_tiles_f64(s, cam, (double)su->max_distance, w, h, t64);
+        } else {
+            t32 = (float *)malloc(nt * sizeof(float));
+            sp_tiles_f32(s, cam, su->max_distance, w, h, t32);
+        }
     }
+    for (uint64_t p = p0; p < p1; p++) {
+        if (prec == 64) sp_pixel_f64(s, cam, su, ext, w, h, t64, p, out + (p - p0));
+        else sp_pixel_f32(s, cam, su, ext, w, h, t32, p, out + (p - p0));
+    }
+    free(t32);
+    free(t64);
 }
 
 int orc_sparse_trace_parallel(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                               const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, uint32_t w, uint32_t h,
                               uint64_t p0, uint64_t p1, orc_ray_result *out, int threads) {
     int used = 1;
+    float *t32 = NULL;
+    if (tile_cull(ext)) {
+        t32 = (float *)malloc((uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u * sizeof(float));
+        sp_tiles_f32(s, cam, su->max_distance, w, h, t32);
+    }
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel
@@ -457,12 +491,13 @@ int orc_sparse_trace_parallel(const orc_sparse_scene *s, const rkf_camera_unifor
         used = omp_get_num_threads();
 #pragma omp for schedule(dynamic, 64)
         for (int64_t p = (int64_t)p0; p < (int64_t)p1; p++)
-            sp_pixel_f32(s, cam, su, ext, w, h, (uint64_t)p, out + (p - (int64_t)p0));
+            sp_pixel_f32(s, cam, su, ext, w, h, t32, (uint64_t)p, out + (p - (int64_t)p0));
     }
 #else
     (void)threads;
-    for (uint64_t p = p0; p < p1; p++) sp_pixel_f32(s, cam, su, ext, w, h, p, out + (p - p0));
+    for (uint64_t p = p0; p < p1; p++) sp_pixel_f32(s, cam, su, ext, w, h, t32, p, out + (p - p0));
 #endif
+    free(t32);
     return used;
 }
 

@@ -674,8 +674,122 @@ typedef struct {
 } F(sres);
 
 /* Sparse march: ray_march_tiled's step rules (:905-982) over one field. */
+/* Tile intervals (RKFM_MARCH_TILE_CULL; no reference counterpart). For
+ * every L2 cell holding an L1 block: its box widened by 9 vs (25 vs on sides
+ * on the grid boundary), tn / tf = distance from the camera to the box / to
+ * its farthest corner, and the rectangle of 8x8 tiles whose pixel rays can
+ * meet it (corner projections through the inverted camera basis, 2 pixels
+ * wider; boxes behind the camera or outside one frustum side plane are
+ * dropped, boxes straddling the camera plane cover every tile). Per tile:
+ * lo = min tn, hi = max tf (+inf / 0 when no box). */
+static int F(tile_cell)(const orc_sparse_scene *s, const V3 *tc, R w, R h, R jx, R jy, uint32_t cell, R maxd,
+                        uint32_t tx, uint32_t ty, uint32_t *rect, R *tn, R *tf) {
+    /* tc: o, ia, ib, ic */
+    uint32_t q[3] = {cell % s->cells[0], (cell / s->cells[0]) % s->cells[1], cell / (s->cells[0] * s->cells[1])};
+    R vs = (R)s->voxel_size;
+    R cw = vs * (R)64.0f, mg = vs * (R)9.0f, mb = vs * (R)9.0f + vs * (R)16.0f;
+    R o[3] = {tc[0].x, tc[0].y, tc[0].z};
+    R lo[3], hi[3], dn2 = 0, df2 = 0;
+    for (int a = 0; a < 3; a++) {
+        lo[a] = (R)s->origin[a] + (R)q[a] * cw - (q[a] == 0u ? mb : mg);
+        hi[a] = (R)s->origin[a] + (R)(q[a] + 1u) * cw + (q[a] + 1u == s->cells[a] ? mb : mg);
+        R dl = lo[a] - o[a], dh = o[a] - hi[a];
+        R n = FMAX(FMAX(dl, dh), (R)0);
+        R fa = FMAX(FABS(dl), FABS(dh));
+        dn2 = dn2 + n * n;
+        df2 = df2 + fa * fa;
+    }
+    *tn = SQRT(dn2);
+    *tf = SQRT(df2);
+    if (*tn > maxd) return 0;
+    int all_front = 1, any_front = 0;
+    uint32_t outside = 15u;
+    R nx0 = (R)INFINITY, nx1 = -(R)INFINITY, ny0 = (R)INFINITY, ny1 = -(R)INFINITY;
+    for (uint32_t k = 0; k < 8u; k++) {
+        V3 v = F(sub)(F(mk)((k & 1u) ? hi[0] : lo[0], (k & 2u) ? hi[1] : lo[1], (k & 4u) ? hi[2] : lo[2]), tc[0]);
+        R a = F(dot)(tc[1], v), b = F(dot)(tc[2], v), cc = F(dot)(tc[3], v);
+        R sl = a + FABS(a) * (R)1e-3f;
+        if (!(b > sl)) outside &= ~1u;
+        if (!(-b > sl)) outside &= ~2u;
+        if (!(cc > sl)) outside &= ~4u;
+        if (!(-cc > sl)) outside &= ~8u;
+        if (a > (R)0) {
+            any_front = 1;
+            R nx = b / a, ny = cc / a;
+            nx0 = FMIN(nx0, nx);
+            nx1 = FMAX(nx1, nx);
+            ny0 = FMIN(ny0, ny);
+            ny1 = FMAX(ny1, ny);
+        } else {
+            all_front = 0;
+        }
+    }
+    if (outside) return 0;
+    if (!any_front && *tn > (R)0) return 0;
+    if (!all_front) {
+        rect[0] = 0; rect[1] = tx - 1u; rect[2] = 0; rect[3] = ty - 1u;
+        return 1;
+    }
+    R fx0 = (nx0 + (R)1.0f) * (R)0.5f * w - (R)0.5f - jx - (R)2.0f;
+    R fx1 = (nx1 + (R)1.0f) * (R)0.5f * w - (R)0.5f - jx + (R)2.0f;
+    R fy0 = ((R)1.0f - ny1) * (R)0.5f * h - (R)0.5f - jy - (R)2.0f;
+    R fy1 = ((R)1.0f - ny0) * (R)0.5f * h - (R)0.5f - jy + (R)2.0f;
+    fx0 = FMIN(FMAX(fx0, (R)-16.0f), w + (R)16.0f);
+    fx1 = FMIN(FMAX(fx1, (R)-16.0f), w + (R)16.0f);
+    fy0 = FMIN(FMAX(fy0, (R)-16.0f), h + (R)16.0f);
+    fy1 = FMIN(FMAX(fy1, (R)-16.0f), h + (R)16.0f);
+    int32_t ix0 = (int32_t)FLOOR(fx0 * (R)0.125f), ix1 = (int32_t)FLOOR(fx1 * (R)0.125f);
+    int32_t iy0 = (int32_t)FLOOR(fy0 * (R)0.125f), iy1 = (int32_t)FLOOR(fy1 * (R)0.125f);
+    if (ix1 < 0 || iy1 < 0 || ix0 > (int32_t)tx - 1 || iy0 > (int32_t)ty - 1) return 0;
+    rect[0] = (uint32_t)(ix0 < 0 ? 0 : ix0);
+    rect[1] = (uint32_t)(ix1 > (int32_t)tx - 1 ? (int32_t)tx - 1 : ix1);
+    rect[2] = (uint32_t)(iy0 < 0 ? 0 : iy0);
+    rect[3] = (uint32_t)(iy1 > (int32_t)ty - 1 ? (int32_t)ty - 1 : iy1);
+    return 1;
+}
+
+static V3 F(cross3)(V3 a, V3 b) {
+    return F(mk)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* iv: 2 values (lo, hi) per tile, row-major over ceil(w/8) x ceil(h/8). */
+static void F(sp_tiles)(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, R maxd, uint32_t w, uint32_t h,
+                        R *iv) {
+    uint32_t tx = (w + 7u) / 8u, ty = (h + 7u) / 8u;
+    for (uint64_t i = 0; i < (uint64_t)tx * ty; i++) {
+        iv[2 * i] = (R)INFINITY;
+        iv[2 * i + 1] = 0;
+    }
+    V3 f = F(mk)((R)cam->forward[0], (R)cam->forward[1], (R)cam->forward[2]);
+    V3 r = F(mk)((R)cam->right[0], (R)cam->right[1], (R)cam->right[2]);
+    V3 u = F(mk)((R)cam->up[0], (R)cam->up[1], (R)cam->up[2]);
+    V3 ru = F(cross3)(r, u), uf = F(cross3)(u, f), fr = F(cross3)(f, r);
+    R det = F(dot)(f, ru);
+    V3 tc[4];
+    tc[0] = F(mk)((R)cam->position[0], (R)cam->position[1], (R)cam->position[2]);
+    tc[1] = F(mk)(ru.x / det, ru.y / det, ru.z / det);
+    tc[2] = F(mk)(uf.x / det, uf.y / det, uf.z / det);
+    tc[3] = F(mk)(fr.x / det, fr.y / det, fr.z / det);
+    uint64_t ncell = (uint64_t)s->cells[0] * s->cells[1] * s->cells[2];
+    for (uint64_t cell = 0; cell < ncell; cell++) {
+        if (s->l2[cell] == RKF_EMPTY_SLOT) continue;
+        uint32_t rect[4];
+        R tn, tf;
+        if (!F(tile_cell)(s, tc, (R)w, (R)h, (R)cam->jitter[0], (R)cam->jitter[1], (uint32_t)cell, maxd, tx, ty, rect,
+                          &tn, &tf))
+            continue;
+        for (uint32_t y = rect[2]; y <= rect[3]; y++)
+            for (uint32_t x = rect[0]; x <= rect[1]; x++) {
+                R *t = iv + 2 * ((uint64_t)y * tx + x);
+                t[0] = FMIN(t[0], tn);
+                t[1] = FMAX(t[1], tf);
+            }
+    }
+}
+
+/* iv: the ray's tile (lo, hi) or NULL. */
 static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
-                        F(sres) *r) {
+                        const R *iv, F(sres) *r) {
     R vs = (R)c->s->voxel_size;
     r->hit = 0;
     r->t = K_MAX_FLOAT;
@@ -686,8 +800,20 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
     R nudge = vs * (R)0.00390625f;
     V3 inv = F(mk)(vs / sd.x, vs / sd.y, vs / sd.z);
     uint32_t near = 0;
-    for (uint32_t step = 0; step < K_HARD_MAX; step++) {
-        if (t > (R)su->max_distance || near >= su->max_steps) break;
+    R tmax = (R)su->max_distance;
+    if (iv) {
+        /* outside [lo, hi] the field along the ray is the free value: one
+           skip to lo, a miss past hi */
+        tmax = F(minr)(tmax, iv[1]);
+        if (iv[0] > t) {
+            r->steps = 1;
+            prev_t = iv[0];
+            prev_dist = vs * (R)8.0f;
+            t = iv[0] + nudge;
+        }
+    }
+    for (;;) {
+        if (r->steps >= K_HARD_MAX || t > tmax || near >= su->max_steps) break;
         r->steps++;
         V3 pos = F(add)(o, F(muls)(sd, t));
         R tx;
@@ -762,14 +888,16 @@ static void F(sp_init)(F(sctx) *c, const orc_sparse_scene *s) {
 }
 
 static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, const rkf_scene_uniforms *su,
-                        const rkfm_scene_ext *ext, uint32_t w, uint32_t h, uint64_t p, orc_ray_result *out) {
+                        const rkfm_scene_ext *ext, uint32_t w, uint32_t h, const R *tiles, uint64_t p,
+                        orc_ray_result *out) {
     F(sctx) c;
     F(sp_init)(&c, s);
     uint32_t px = (uint32_t)(p % w), py = (uint32_t)(p / w);
     V3 o, dir;
     F(camera_ray)(cam, w, h, px, py, &o, &dir);
     F(sres) r;
-    F(sp_march)(&c, o, dir, su, ext, &r);
+    const R *iv = tiles ? tiles + 2 * ((uint64_t)(py >> 3) * ((w + 7u) / 8u) + (px >> 3)) : NULL;
+    F(sp_march)(&c, o, dir, su, ext, iv, &r);
     memset(out, 0, sizeof(*out));
     out->steps = r.steps;
     out->brick_slot = RKF_EMPTY_SLOT;

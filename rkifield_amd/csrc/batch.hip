@@ -39,7 +39,7 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
         uint32_t tx = u % B.tiles_x, ty = u / B.tiles_x;
         uint32_t px = tx * B.tile_w + lane % B.tile_w;
         uint32_t py = ty * B.tile_h + lane / B.tile_w;
-        if (px < B.P.g.width && py < B.P.g.height) view_pixel(B.P, px, py);
+        if (px < B.P.g.width && py < B.P.g.height) view_pixel<false>(B.P, px, py);
     } else if (B.kind == RKFM_BATCH_SHADOW) {
         uint32_t i = u * 64u + lane;
         if (i >= B.n) return;
@@ -47,7 +47,7 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
         ShadowRay sr;
         shadow_init(sr, o, mk(B.P.ext.light_dir[0], B.P.ext.light_dir[1], B.P.ext.light_dir[2]), B.P.S.vs);
         float maxd = shadow_maxd(B.P), k = shadow_k(B.P);
-        while (!shadow_step(B.P.S, sr, maxd, k)) {
+        while (!shadow_step<false>(B.P.S, sr, maxd, k)) {
         }
         B.vis[i] = sr.prev_dist;
     } else {
@@ -125,6 +125,9 @@ int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, con
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: scene not resident in this context", i);
         if (it.ext.hit_eps_mode > RKFM_EPS_RELATIVE)
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: hit_eps_mode", i);
+        if (it.ext.march_flags)
+            return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: march_flags 0x%x not supported in batches", i,
+                                 it.ext.march_flags);
         d.P.S = it.scene->dev();
         d.P.cam = it.camera;
         d.P.su = it.su;

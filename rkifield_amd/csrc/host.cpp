@@ -112,6 +112,9 @@ int rkfm_destroy(rkfm_ctx *c) {
     c->batch_counter.release();
     c->march_hits.release();
     c->march_count.release();
+    c->tile_iv.release();
+    c->tile_big.release();
+    c->tile_big_count.release();
     c->edit_ops.release();
     c->edit_list.release();
     c->edit_mod.release();

@@ -27,6 +27,7 @@ struct rkfm_sparse_scene {
     rkfm_ctx *owner = nullptr;
     rkfm_sparse_info info{};
     DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, oct, cellw, bext;
+    DevBuf block_cell;    // per L1 block: its flat L2 cell
     DevBuf edit_touched;  // one bit per slot, all zero between edit calls
     uint64_t n_free_rows = 0;
     rkf::SparseDev dev() const;
@@ -52,6 +53,9 @@ struct rkfm_ctx {
     uint32_t tuned_waves = 0;
     // Split sparse march: compact hit list and its counter.
     DevBuf march_hits, march_count;
+    // Tile intervals of the last culled march: per-tile pairs, the large
+    // rectangle records and their counter.
+    DevBuf tile_iv, tile_big, tile_big_count;
     // Edits: op records (and their pinned staging, reusable once edit_ev
     // has fired), the listed bricks, their modified flags, the list count.
     DevBuf edit_ops, edit_list, edit_mod, edit_count;

@@ -68,6 +68,7 @@ struct SparseDev {
                                // load: free row << 32 | octant extents
     uint64_t n_cells;
     uint32_t n_blocks;         // rows below n_blocks are the L1 blocks
+    const uint32_t *block_cell;  // per L1 block: its flat L2 cell
 };
 
 struct SparseMarchParams {
@@ -77,7 +78,16 @@ struct SparseMarchParams {
     rkfm_scene_ext ext;
     GbufPtrs g;
     uint32_t object_id;
+    uint32_t tiles_x;          // 8x8 tiles per row of `tiles`
+    const uint2 *tiles;        // per tile (t_lo, t_hi) float bits, or null
 };
+
+// A cell whose tile rectangle is too large for one lane (tile binning).
+struct TileBig {
+    uint32_t x0, x1, y0, y1;
+    float tn, tf;
+};
+constexpr uint32_t kTileInlineMax = 16u;  // tiles one lane bins itself
 
 // One batch of a batched launch (batch.hip): the batch's scene and
 // parameters plus its range of 64-lane work units.
@@ -107,4 +117,10 @@ hipError_t rkf_launch_sparse_march_split(const rkf::SparseMarchParams *P, void *
                                          uint32_t shade_waves, hipStream_t stream);
 hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,
                                    hipStream_t stream);
+// Per-tile primary-ray intervals: clear, bin every L1 block's cell, then the
+// cells whose rectangle exceeds kTileInlineMax tiles (big holds up to
+// n_blocks records, big_count one counter).
+hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam, float max_distance,
+                                   uint32_t width, uint32_t height, uint2 *tiles, rkf::TileBig *big,
+                                   uint32_t *big_count, hipStream_t stream);
 }

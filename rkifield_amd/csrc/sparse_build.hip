@@ -447,6 +447,7 @@ rkf::SparseDev rkfm_sparse_scene::dev() const {
     d.cellw = (const uint64_t *)cellw.ptr;
     d.n_cells = (uint64_t)info.cells[0] * info.cells[1] * info.cells[2];
     d.n_blocks = info.n_blocks;
+    d.block_cell = (const uint32_t *)block_cell.ptr;
     return d;
 }
 
@@ -461,6 +462,7 @@ static void release_scene(rkfm_sparse_scene *s) {
     s->oct.release();
     s->cellw.release();
     s->bext.release();
+    s->block_cell.release();
     s->edit_touched.release();
 }
 
@@ -531,7 +533,8 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
         return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "sparse scene: %zu free-mask rows", rows.size());
     TmpBufs tmp;
     DevBuf *d_rows = tmp.make(), *d_any = tmp.make();
-    int rc = put(c->stream, *d_rows, rows.data(), rows.size() * 4);
+    int rc = put(c->stream, s->block_cell, block_cells.data(), block_cells.size() * 4);
+    if (!rc) rc = put(c->stream, *d_rows, rows.data(), rows.size() * 4);
     if (!rc) rc = d_any->reserve(rows.size() * 4 + 4, false);
     if (!rc) rc = s->free_bits.reserve(rows.size() * 64 + 64, false);
     if (!rc) rc = s->free_idx.reserve(ncell * 4, false);
@@ -619,7 +622,7 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
 
 void finish_info(rkfm_sparse_scene *s) {
     s->info.device_bytes = s->l2.cap + s->l1.cap + s->dist.cap + s->mat.cap + s->aux.cap + s->free_idx.cap +
-                           s->free_bits.cap + s->oct.cap + s->cellw.cap + s->bext.cap;
+                           s->free_bits.cap + s->oct.cap + s->cellw.cap + s->bext.cap + s->block_cell.cap;
 }
 
 }  // namespace
@@ -946,6 +949,43 @@ int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t 
     return rc;
 }
 
+// Tile intervals into `tiles` (the context's large-rectangle list as
+// scratch; stream-ordered, so successive calls on one stream may share it).
+static int tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                          const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint2 *tiles, hipStream_t st) {
+    if (!(fabsf(cam->jitter[0]) <= 0.5f && fabsf(cam->jitter[1]) <= 0.5f))
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: |jitter| above 0.5");
+    {
+        // the kernels invert [forward right up]; refuse a degenerate basis
+        double f[3], r[3], u[3];
+        for (int a = 0; a < 3; a++) {
+            f[a] = cam->forward[a];
+            r[a] = cam->right[a];
+            u[a] = cam->up[a];
+        }
+        const double det = f[0] * (r[1] * u[2] - r[2] * u[1]) + f[1] * (r[2] * u[0] - r[0] * u[2]) +
+                           f[2] * (r[0] * u[1] - r[1] * u[0]);
+        if (!std::isfinite(det) || fabs(det) < 1e-12)
+            return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: degenerate camera basis");
+    }
+    int rc = c->tile_big.reserve((uint64_t)s->info.n_blocks * sizeof(rkf::TileBig) + 64, false);
+    if (!rc) rc = c->tile_big_count.reserve(256, false);
+    if (rc) return rc;
+    rkf::SparseDev D = s->dev();
+    return rkf_hip_check(rkf_launch_sparse_tiles(&D, cam, su->max_distance, w, h, tiles,
+                                                 (rkf::TileBig *)c->tile_big.ptr,
+                                                 (uint32_t *)c->tile_big_count.ptr, st),
+                         "tile intervals launch");
+}
+
+int rkfm_sparse_tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                               const rkf_scene_uniforms *su, uint32_t w, uint32_t h, float *tiles, void *stream) {
+    if (!c || !s || !cam || !su || !tiles) return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: null argument");
+    if (w == 0 || h == 0 || w > 32768u || h > 32768u)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: %ux%u", w, h);
+    return tile_intervals(c, s, cam, su, w, h, (uint2 *)tiles, stream ? (hipStream_t)stream : c->stream);
+}
+
 int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, const rkfm_gbuffer *out,
                       uint32_t variant, void *stream) {
@@ -963,6 +1003,16 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
     if (ext) P.ext = *ext;
     P.object_id = 0;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (ext && (ext->march_flags & ~RKFM_MARCH_TILE_CULL))
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_march: march_flags 0x%x", ext->march_flags);
+    if (ext && (ext->march_flags & RKFM_MARCH_TILE_CULL)) {
+        const uint32_t tx = (out->width + 7u) / 8u, ty = (out->height + 7u) / 8u;
+        rc = c->tile_iv.reserve((uint64_t)tx * ty * 8u, false);
+        if (!rc) rc = tile_intervals(c, s, cam, su, out->width, out->height, (uint2 *)c->tile_iv.ptr, st);
+        if (rc) return rc;
+        P.tiles = (const uint2 *)c->tile_iv.ptr;
+        P.tiles_x = tx;
+    }
     if (variant == 1 && !c->tuned_waves) {
         rc = rkf_hip_check(rkf_occupancy_waves(0, &c->tuned_waves), "occupancy");
         if (rc) return rc;

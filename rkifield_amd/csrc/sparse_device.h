@@ -22,6 +22,13 @@
 // cell's extent for the ray's octant, at most 15). The ray jumps to the region's exit t_x,
 // takes prev_t = t_x, prev_dist = 8 vs and resumes at t_x + vs/256. One skip
 // counts as one loop iteration.
+//
+// Tile intervals (optional): a view's 8x8-pixel tiles carry [t_lo, t_hi],
+// the range of ray parameters at which any of the tile's rays can come
+// within 9 vs (25 vs on grid-boundary sides) of an L2 cell holding an L1
+// block; outside it every sample is the free value (a sample reads bricks
+// {b0, b0+1}^3, i.e. points 7.5 vs below to 0.5 vs above a brick, and
+// samples clamped at the grid boundary reach 16 vs further).
 #pragma once
 
 #include "kernels.h"
@@ -32,6 +39,11 @@ struct SpCtx {
     const SparseDev *S;
     uint32_t evals;
 };
+
+// Marks a loaded value as used here: every load before it is issued before
+// any code after it, so a group of independent loads waits once instead of
+// being sunk one by one into the branches that consume them.
+RKF_DEV void loaded(uint32_t &v) { asm volatile("" : "+v"(v)); }
 
 RKF_DEV uint32_t sp_slot(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz) {
     uint32_t bx = (uint32_t)cx >> 3, by = (uint32_t)cy >> 3, bz = (uint32_t)cz >> 3;
@@ -288,6 +300,215 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
     return true;
 }
 
+// The slots of the 2x2x2 bricks b0 + {0,1}^3 that a sample whose lower
+// corner voxel lies in brick b0 reads, kept per ray while its samples stay in
+// b0 (a brick that is not free): a step then needs only the distance loads.
+struct BrickNbhd {
+    uint64_t key;      // cell * 512 + (brick in cell) of b0; ~0 when empty
+    uint32_t slot[8];  // brick b0 + (k & 1, k >> 1 & 1, k >> 2); constant indices only
+};
+
+RKF_DEV void nb_reset(BrickNbhd &n) { n.key = ~0ull; }
+
+// Slot of corner brick (ox, oy, oz) in {0,1}^3 by selects, so the slots stay
+// in registers.
+RKF_DEV uint32_t nb_pick(const BrickNbhd &n, bool ox, bool oy, bool oz) {
+    const uint32_t *s = n.slot;
+    const uint32_t a0 = ox ? s[1] : s[0], a1 = ox ? s[3] : s[2], a2 = ox ? s[5] : s[4], a3 = ox ? s[7] : s[6];
+    const uint32_t b0 = oy ? a1 : a0, b1 = oy ? a3 : a2;
+    return oz ? b1 : b0;
+}
+
+// Corner load address: the voxel's distance word for a live slot, else the
+// pool's first word (the pool always holds one), so the load is issued for
+// every corner.
+RKF_DEV const uint16_t *nb_addr(const SparseDev &S, uint32_t slot, int32_t cx, int32_t cy, int32_t cz) {
+    const uint64_t i = slot < RKF_INTERIOR_SLOT ? (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u +
+                                                      (uint32_t)(cz & 7) * 64u
+                                                : 0u;
+    return S.dist + i;
+}
+
+RKF_DEV float nb_value(uint32_t slot, uint32_t raw, float vs) {
+    return slot < RKF_INTERIOR_SLOT ? h2f((uint16_t)raw) : (slot == RKF_EMPTY_SLOT ? vs * 8.0f : -(vs * 2.0f));
+}
+
+// sp_sample (stepping semantics) for an in-grid point whose lower corner
+// voxel v0 (= floor(g)) lies in the cached brick: same corner values, same
+// arithmetic (the outside-grid distance is +0 here).
+RKF_DEV float nb_sample(const SparseDev &S, const BrickNbhd &n, const float g[3], const int32_t v0[3]) {
+    const float vs = S.vs;
+    const int32_t x0 = v0[0], y0 = v0[1], z0 = v0[2];
+    const float tx = g[0] - (float)x0, ty = g[1] - (float)y0, tz = g[2] - (float)z0;
+    const bool ox = (x0 & 7) == 7, oy = (y0 & 7) == 7, oz = (z0 & 7) == 7;
+    const uint32_t s000 = nb_pick(n, false, false, false), s100 = nb_pick(n, ox, false, false);
+    const uint32_t s010 = nb_pick(n, false, oy, false), s110 = nb_pick(n, ox, oy, false);
+    const uint32_t s001 = nb_pick(n, false, false, oz), s101 = nb_pick(n, ox, false, oz);
+    const uint32_t s011 = nb_pick(n, false, oy, oz), s111 = nb_pick(n, ox, oy, oz);
+    uint32_t r000 = *nb_addr(S, s000, x0, y0, z0), r100 = *nb_addr(S, s100, x0 + 1, y0, z0);
+    uint32_t r010 = *nb_addr(S, s010, x0, y0 + 1, z0), r110 = *nb_addr(S, s110, x0 + 1, y0 + 1, z0);
+    uint32_t r001 = *nb_addr(S, s001, x0, y0, z0 + 1), r101 = *nb_addr(S, s101, x0 + 1, y0, z0 + 1);
+    uint32_t r011 = *nb_addr(S, s011, x0, y0 + 1, z0 + 1), r111 = *nb_addr(S, s111, x0 + 1, y0 + 1, z0 + 1);
+    loaded(r000);
+    loaded(r100);
+    loaded(r010);
+    loaded(r110);
+    loaded(r001);
+    loaded(r101);
+    loaded(r011);
+    loaded(r111);
+    const float c000 = nb_value(s000, r000, vs), c100 = nb_value(s100, r100, vs);
+    const float c010 = nb_value(s010, r010, vs), c110 = nb_value(s110, r110, vs);
+    const float c001 = nb_value(s001, r001, vs), c101 = nb_value(s101, r101, vs);
+    const float c011 = nb_value(s011, r011, vs), c111 = nb_value(s111, r111, vs);
+    const float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
+    const float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
+    const float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
+    return mix(c0, c1, tz) + 0.0f;
+}
+
+// In-grid test of sp_free: g and v0 of the point, false near the grid faces
+// (those points take sp_sample's clamped path).
+RKF_DEV bool sp_grid_pos(const SparseDev &S, f3 pos, float g[3], int32_t v0[3]) {
+    const float vs = S.vs;
+    const float be = vs * 8.0f;
+    const f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
+    const f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
+    const float lo = vs * 0.01f;
+    if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
+    g[0] = gp.x / vs - 0.5f;
+    g[1] = gp.y / vs - 0.5f;
+    g[2] = gp.z / vs - 0.5f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        v0[a] = (int32_t)floorf(g[a]);
+        if (v0[a] < 0 || v0[a] > S.tot[a] - 2) return false;
+    }
+    return true;
+}
+
+RKF_DEV uint64_t nb_key(const SparseDev &S, const int32_t v0[3]) {
+    const uint32_t bx = (uint32_t)v0[0] >> 3, by = (uint32_t)v0[1] >> 3, bz = (uint32_t)v0[2] >> 3;
+    const uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
+                          (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
+    return cell * 512u + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+}
+
+enum : int { kLocFree = 0, kLocCached = 1, kLocEdge = 2 };
+
+// Free-space test + exit parameter (see file header), with the brick
+// neighbourhood cache. A point whose brick b0 is the cached one samples at
+// once (a cached brick is never free). Otherwise one load of the cell word
+// decides cell-level skips, then the brick's extent brick-level skips; a
+// brick that is not free refills the cache with the slots of its 2x2x2
+// neighbourhood. Near the grid faces the point takes the clamped sampling
+// path (kLocEdge).
+RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, BrickNbhd &n, float g[3],
+                      int32_t v0[3]) {
+    if (!sp_grid_pos(S, pos, g, v0)) return kLocEdge;
+    const uint64_t key = nb_key(S, v0);
+    if (key == n.key) return kLocCached;
+    const uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
+    const uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
+    const uint64_t cell = key >> 9;
+    float lo_b[3], hi_b[3];
+    const uint64_t cw = S.cellw[cell];
+    const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
+    const uint32_t e = ((uint32_t)cw >> (4u * octant)) & 15u;
+    if (e > 0) {
+        // the cube of cells q + [0, e) in the travel direction of each axis is free
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((float)q[a] - (float)(e - 1u)) * 64.0f;
+            hi_b[a] = ((float)q[a] + (float)e) * 64.0f;
+        }
+    } else {
+        const uint32_t row = (uint32_t)(cw >> 32);
+        const uint32_t lb = (uint32_t)(key & 511u);
+        const uint32_t eb = (S.bext[((uint64_t)row * 8u + octant) * 256u + (lb >> 1)] >> ((lb & 1u) * 4u)) & 15u;
+        if (eb == 0) {
+            // A brick that is not free: load the slots of its 2x2x2
+            // neighbourhood without branches, so each round's loads go out
+            // together: bricks of the same cell from the cell's L1 block (the
+            // row is the block when below n_blocks), bricks of neighbouring
+            // cells through their L2 entries, then their L1 entries (those
+            // two rounds only when some lane of the wave needs them). Unused
+            // loads read entry 0 (the arrays always hold one); bricks past
+            // the grid end count EMPTY.
+            const bool blk_ok = row < S.n_blocks;
+            uint32_t va[8], vc[8], vb[8], lbk[8];
+            bool inb[8], same[8], far[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++) {
+                const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
+                inb[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2];
+                same[k] = (bx >> 3) == q[0] && (by >> 3) == q[1] && (bz >> 3) == q[2];
+                lbk[k] = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+                va[k] = S.l1[inb[k] && same[k] && blk_ok ? (uint64_t)row * 512u + lbk[k] : 0u];
+                vc[k] = RKF_EMPTY_SLOT;
+            }
+            const bool cross = (b[0] & 7u) == 7u || (b[1] & 7u) == 7u || (b[2] & 7u) == 7u;
+            if (__any(cross)) {
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; k++) {
+                    const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
+                    const uint64_t ck = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
+                                        (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
+                    vc[k] = S.l2[inb[k] && !same[k] ? ck : 0u];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++) {
+                loaded(va[k]);
+                loaded(vc[k]);
+                far[k] = inb[k] && !same[k] && vc[k] != RKF_EMPTY_SLOT;
+                vb[k] = RKF_EMPTY_SLOT;
+            }
+            if (__any(far[0] || far[1] || far[2] || far[3] || far[4] || far[5] || far[6] || far[7])) {
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; k++) vb[k] = S.l1[far[k] ? (uint64_t)vc[k] * 512u + lbk[k] : 0u];
+            }
+            n.key = key;
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++) {
+                loaded(vb[k]);
+                n.slot[k] = !inb[k] ? RKF_EMPTY_SLOT : same[k] ? (blk_ok ? va[k] : RKF_EMPTY_SLOT)
+                                                               : (far[k] ? vb[k] : RKF_EMPTY_SLOT);
+            }
+            return kLocCached;
+        }
+        // the cube of bricks b + [0, eb) in the travel direction of each axis is free
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((float)b[a] - (float)(eb - 1u)) * 8.0f;
+            hi_b[a] = ((float)b[a] + (float)eb) * 8.0f;
+        }
+    }
+    const float dv[3] = {sd.x, sd.y, sd.z};
+    const float iv[3] = {inv.x, inv.y, inv.z};
+    float dt = kMaxFloat;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float bound = dv[a] > 0.0f ? hi_b[a] : lo_b[a];
+        const float da = (bound - g[a]) * iv[a];
+        dt = fminf(dt, da);
+    }
+    t_exit = t + fmaxf(dt, 0.0f);
+    return kLocFree;
+}
+
+// A stepping sample at any point (bisection): from the cache when the point's
+// brick is the cached one, else sp_sample.
+RKF_DEV float sp_sample_nb(SpCtx &c, const BrickNbhd &n, f3 p) {
+    float g[3];
+    int32_t v0[3];
+    if (sp_grid_pos(*c.S, p, g, v0) && nb_key(*c.S, v0) == n.key) {
+        c.evals++;
+        return nb_sample(*c.S, n, g, v0);
+    }
+    return sp_sample(c, p, false, true);
+}
+
 struct SpRes {
     float t, mat_d;
     f3 mat_p;
@@ -302,42 +523,94 @@ struct PrimRay {
     f3 o, sd;  // origin, safe direction (shadow rays: light direction)
     f3 inv;    // vs / sd per axis: free-space exits in voxel units -> t
     float t, prev_t, prev_dist;  // shadow rays keep their visibility in prev_dist
+    float tmax;                  // max_distance, or less with a tile interval
     uint32_t near, steps, evals;
+    BrickNbhd nb;                // slots around the last sampled brick
 };
 enum : int { kRayContinue = 0, kRayHit = 1, kRayMiss = 2 };
 
-RKF_DEV void prim_init(PrimRay &r, f3 o, f3 dir, float vs) {
+RKF_DEV void prim_init(PrimRay &r, f3 o, f3 dir, float vs, float max_distance) {
     r.o = o;
     r.sd = safe_dir(dir);
     r.inv = mk(vs / r.sd.x, vs / r.sd.y, vs / r.sd.z);
     r.t = 0.0f;
     r.prev_t = 0.0f;
     r.prev_dist = kMaxFloat;
+    r.tmax = max_distance;
     r.near = 0;
     r.steps = 0;
     r.evals = 0;
+    nb_reset(r.nb);
 }
 
+// Tile interval (t_lo, t_hi): outside it every sample along the ray reads
+// the free value, so the ray skips to t_lo (one iteration, as a free-space
+// skip) and stops as a miss past t_hi.
+RKF_DEV void prim_cull(PrimRay &r, uint2 iv, float vs) {
+    const float lo = __uint_as_float(iv.x), hi = __uint_as_float(iv.y);
+    r.tmax = fminf(r.tmax, hi);
+    if (lo > r.t) {
+        r.steps = 1;
+        r.prev_t = lo;
+        r.prev_dist = vs * 8.0f;
+        r.t = lo + vs * 0.00390625f;
+    }
+}
+
+// One step's field query: false for a free-space skip (t_exit set), else
+// the stepping sample d. kLowLat: the brick-neighbourhood cache (sp_locate +
+// nb_sample), fewer dependent loads per step, for launches bounded by their
+// slowest rays; otherwise sp_free + sp_sample with the prefetched slot, fewer
+// memory instructions, for launches with enough rays in flight to hide
+// latency. Both give the same values.
+template <bool kLowLat>
+RKF_DEV bool step_field(const SparseDev &S, PrimRay &r, f3 pos, float &t_exit, SpCtx &c, float &d) {
+    if constexpr (kLowLat) {
+        float g[3];
+        int32_t v0[3];
+        const int loc = sp_locate(S, pos, r.sd, r.inv, r.t, t_exit, r.nb, g, v0);
+        if (loc == kLocFree) return false;
+        if (loc == kLocCached) {
+            c.evals++;
+            d = nb_sample(S, r.nb, g, v0);
+        } else {
+            d = sp_sample(c, pos, false, true);
+        }
+        return true;
+    } else {
+        uint32_t slot;
+        if (sp_free(S, pos, r.sd, r.inv, r.t, t_exit, slot)) return false;
+        d = sp_sample(c, pos, false, true, slot);
+        return true;
+    }
+}
+
+template <bool kLowLat>
 RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &out) {
-    if (r.steps >= kHardMax || r.t > su.max_distance || r.near >= su.max_steps) return kRayMiss;
+    if (r.steps >= kHardMax || r.t > r.tmax || r.near >= su.max_steps) return kRayMiss;
     const float vs = S.vs;
     r.steps++;
     f3 pos = add(r.o, muls(r.sd, r.t));
     float tx;
-    uint32_t slot;
-    if (sp_free(S, pos, r.sd, r.inv, r.t, tx, slot)) {
+    SpCtx c{&S, r.evals};
+    float d;
+    if (!step_field<kLowLat>(S, r, pos, tx, c, d)) {
         r.prev_t = tx;
         r.prev_dist = vs * 8.0f;
         r.t = tx + vs * 0.00390625f;
         return kRayContinue;
     }
-    SpCtx c{&S, r.evals};
-    float d = sp_sample(c, pos, false, true, slot);
     if (d < 0.0f && r.prev_dist > hit_eps(eps_mode, su.hit_threshold, r.prev_t)) {
         float lo = r.prev_t, hi = r.t, mt = r.t, md = d;
         for (uint32_t b = 0; b < kBisect; b++) {
             float mid = (lo + hi) * 0.5f;
-            float mdd = sp_sample(c, add(r.o, muls(r.sd, mid)), false, true);
+            const f3 mp = add(r.o, muls(r.sd, mid));
+            float mdd;
+            if constexpr (kLowLat) {
+                mdd = sp_sample_nb(c, r.nb, mp);
+            } else {
+                mdd = sp_sample(c, mp, false, true);
+            }
             if (mdd < 0.0f) {
                 hi = mid;
             } else {
@@ -382,9 +655,11 @@ RKF_DEV void shadow_init(ShadowRay &r, f3 o, f3 L, float vs) {
     r.near = 0;
     r.steps = 0;
     r.evals = 0;
+    nb_reset(r.nb);
 }
 
 // Returns true when finished; r.prev_dist then holds the visibility.
+template <bool kLowLat>
 RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) {
     if (r.steps >= 256u || r.near >= 32u || r.t > maxd) {
         r.prev_dist = clampf(r.prev_dist, 0.0f, 1.0f);
@@ -393,13 +668,12 @@ RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) 
     r.steps++;
     f3 p = add(r.o, muls(r.sd, r.t));
     float tx;
-    uint32_t slot;
-    if (sp_free(S, p, r.sd, r.inv, r.t, tx, slot)) {
+    SpCtx c{&S, r.evals};
+    float d;
+    if (!step_field<kLowLat>(S, r, p, tx, c, d)) {
         r.t = tx + S.vs * 0.00390625f;
         return false;
     }
-    SpCtx c{&S, r.evals};
-    float d = sp_sample(c, p, false, true, slot);
     r.evals = c.evals;
     if (d < 0.005f) {
         r.prev_dist = 0.0f;
@@ -480,22 +754,26 @@ RKF_DEV void shadow_store(const SparseMarchParams &P, uint64_t p, const ShadowRa
 }
 
 // One pixel of a view, marched to completion, then its shadow ray.
+template <bool kLowLat>
 RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
     f3 o, dir;
     camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
     PrimRay r;
-    prim_init(r, o, dir, P.S.vs);
+    prim_init(r, o, dir, P.S.vs, P.su.max_distance);
+    if constexpr (kLowLat) {
+        if (P.tiles) prim_cull(r, P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)], P.S.vs);
+    }
     SpRes res;
     int st;
     do {
-        st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+        st = prim_step<kLowLat>(P.S, r, P.su, P.ext.hit_eps_mode, res);
     } while (st == kRayContinue);
     f3 so;
     if (prim_finish(P, px, py, r, st, res, so)) {
         ShadowRay sr;
         shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
         float maxd = shadow_maxd(P), k = shadow_k(P);
-        while (!shadow_step(P.S, sr, maxd, k)) {
+        while (!shadow_step<kLowLat>(P.S, sr, maxd, k)) {
         }
         shadow_store(P, (uint64_t)py * P.g.width + px, sr);
     }

@@ -10,7 +10,7 @@ __global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= P.g.width || py >= P.g.height) return;
-    view_pixel(P, px, py);
+    view_pixel<true>(P, px, py);
 }
 
 // Tuned variant: persistent wavefronts with per-lane ray refill and no
@@ -55,7 +55,8 @@ __global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, ui
                     if (px < W && py < H) {
                         f3 o, dir;
                         camera_ray(P.cam, W, H, px, py, o, dir);
-                        prim_init(r, o, dir, P.S.vs);
+                        prim_init(r, o, dir, P.S.vs, P.su.max_distance);
+                        if (P.tiles) prim_cull(r, P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)], P.S.vs);
                         kind = 1;
                     }
                 }
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, ui
         for (int it = 0; it < kInnerSteps && kind != 0; it++) {
             if (kind == 1) {
                 SpRes res;
-                int st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+                int st = prim_step<true>(P.S, r, P.su, P.ext.hit_eps_mode, res);
                 if (st != kRayContinue) {
                     f3 so;
                     bool sh = prim_finish(P, px, py, r, st, res, so);
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(64) void sparse_march_tuned(SparseMarchParams P, ui
                     }
                 }
             } else {
-                if (shadow_step(P.S, r, maxd, ksh)) {
+                if (shadow_step<true>(P.S, r, maxd, ksh)) {
                     shadow_store(P, (uint64_t)py * W + px, r);
                     kind = 0;
                 }
@@ -117,9 +118,10 @@ __global__ __launch_bounds__(64) void sparse_march_primary(SparseMarchParams P, 
     if (active) {
         f3 o, dir;
         camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
-        prim_init(r, o, dir, P.S.vs);
+        prim_init(r, o, dir, P.S.vs, P.su.max_distance);
+        if (P.tiles) prim_cull(r, P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)], P.S.vs);
         do {
-            st = prim_step(P.S, r, P.su, P.ext.hit_eps_mode, res);
+            st = prim_step<true>(P.S, r, P.su, P.ext.hit_eps_mode, res);
         } while (st == kRayContinue);
         if (st != kRayHit) {
             f3 so;
@@ -167,10 +169,159 @@ __global__ __launch_bounds__(64) void sparse_march_shade(SparseMarchParams P, co
         if (prim_finish(P, h.px, h.py, r, kRayHit, res, so)) {
             ShadowRay sr;
             shadow_init(sr, so, L, P.S.vs);
-            while (!shadow_step(P.S, sr, maxd, k)) {
+            while (!shadow_step<true>(P.S, sr, maxd, k)) {
             }
             shadow_store(P, (uint64_t)h.py * P.g.width + h.px, sr);
         }
+    }
+}
+
+// Tile intervals. The camera basis is inverted once per lane (rows of
+// [f r u]^-1 = (r x u, u x f, f x r) / det), so a point's coefficients
+// v = a f + b r + c u give its pixel ray coordinates nx = b / a, ny = c / a.
+struct TileCam {
+    f3 o, ia, ib, ic;
+    float w, h, jx, jy;
+};
+
+RKF_DEV f3 cross3(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+RKF_DEV TileCam tile_cam(const rkf_camera_uniforms &cam, uint32_t w, uint32_t h) {
+    const f3 f = mk(cam.forward[0], cam.forward[1], cam.forward[2]);
+    const f3 r = mk(cam.right[0], cam.right[1], cam.right[2]);
+    const f3 u = mk(cam.up[0], cam.up[1], cam.up[2]);
+    const f3 ru = cross3(r, u), uf = cross3(u, f), fr = cross3(f, r);
+    const float det = dot(f, ru);
+    TileCam t;
+    t.o = mk(cam.position[0], cam.position[1], cam.position[2]);
+    t.ia = mk(ru.x / det, ru.y / det, ru.z / det);
+    t.ib = mk(uf.x / det, uf.y / det, uf.z / det);
+    t.ic = mk(fr.x / det, fr.y / det, fr.z / det);
+    t.w = (float)w;
+    t.h = (float)h;
+    t.jx = cam.jitter[0];
+    t.jy = cam.jitter[1];
+    return t;
+}
+
+// One occupied cell: its widened box, the distances from the camera to the
+// box (tn) and to its farthest corner (tf), and the rectangle of tiles whose
+// rays can meet it. Returns false when no pixel ray can meet it within
+// max_distance.
+RKF_DEV bool tile_cell(const SparseDev &S, const TileCam &tc, uint32_t cell, float max_distance, uint32_t tiles_x,
+                       uint32_t tiles_y, uint32_t &x0, uint32_t &x1, uint32_t &y0, uint32_t &y1, float &tn,
+                       float &tf) {
+    const uint32_t q[3] = {cell % S.cells[0], (cell / S.cells[0]) % S.cells[1], cell / (S.cells[0] * S.cells[1])};
+    const float vs = S.vs;
+    const float cw = vs * 64.0f, mg = vs * 9.0f, mb = vs * 9.0f + vs * 16.0f;
+    const float o[3] = {tc.o.x, tc.o.y, tc.o.z};
+    float lo[3], hi[3];
+    float dn2 = 0.0f, df2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = S.origin[a] + (float)q[a] * cw - (q[a] == 0u ? mb : mg);
+        hi[a] = S.origin[a] + (float)(q[a] + 1u) * cw + (q[a] + 1u == S.cells[a] ? mb : mg);
+        const float dl = lo[a] - o[a], dh = o[a] - hi[a];
+        const float n = fmaxf(fmaxf(dl, dh), 0.0f);
+        const float fa = fmaxf(fabsf(dl), fabsf(dh));
+        dn2 = dn2 + n * n;
+        df2 = df2 + fa * fa;
+    }
+    tn = __builtin_sqrtf(dn2);
+    tf = __builtin_sqrtf(df2);
+    if (tn > max_distance) return false;
+    bool all_front = true, any_front = false;
+    uint32_t outside = 15u;  // right, left, top, bottom frustum planes
+    float nx0 = __builtin_inff(), nx1 = -__builtin_inff(), ny0 = __builtin_inff(), ny1 = -__builtin_inff();
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const f3 v = sub(mk((k & 1u) ? hi[0] : lo[0], (k & 2u) ? hi[1] : lo[1], (k & 4u) ? hi[2] : lo[2]), tc.o);
+        const float a = dot(tc.ia, v), b = dot(tc.ib, v), c = dot(tc.ic, v);
+        const float sl = a + fabsf(a) * 1e-3f;
+        if (!(b > sl)) outside &= ~1u;
+        if (!(-b > sl)) outside &= ~2u;
+        if (!(c > sl)) outside &= ~4u;
+        if (!(-c > sl)) outside &= ~8u;
+        if (a > 0.0f) {
+            any_front = true;
+            const float nx = b / a, ny = c / a;
+            nx0 = fminf(nx0, nx);
+            nx1 = fmaxf(nx1, nx);
+            ny0 = fminf(ny0, ny);
+            ny1 = fmaxf(ny1, ny);
+        } else {
+            all_front = false;
+        }
+    }
+    if (outside != 0u) return false;
+    if (!any_front && tn > 0.0f) return false;
+    if (!all_front) {
+        x0 = 0u;
+        y0 = 0u;
+        x1 = tiles_x - 1u;
+        y1 = tiles_y - 1u;
+        return true;
+    }
+    // pixel coordinates of the rectangle (camera_ray inverted), 2 pixels wider
+    float fx0 = (nx0 + 1.0f) * 0.5f * tc.w - 0.5f - tc.jx - 2.0f;
+    float fx1 = (nx1 + 1.0f) * 0.5f * tc.w - 0.5f - tc.jx + 2.0f;
+    float fy0 = (1.0f - ny1) * 0.5f * tc.h - 0.5f - tc.jy - 2.0f;
+    float fy1 = (1.0f - ny0) * 0.5f * tc.h - 0.5f - tc.jy + 2.0f;
+    fx0 = fminf(fmaxf(fx0, -16.0f), tc.w + 16.0f);
+    fx1 = fminf(fmaxf(fx1, -16.0f), tc.w + 16.0f);
+    fy0 = fminf(fmaxf(fy0, -16.0f), tc.h + 16.0f);
+    fy1 = fminf(fmaxf(fy1, -16.0f), tc.h + 16.0f);
+    const int32_t ix0 = (int32_t)floorf(fx0 * 0.125f), ix1 = (int32_t)floorf(fx1 * 0.125f);
+    const int32_t iy0 = (int32_t)floorf(fy0 * 0.125f), iy1 = (int32_t)floorf(fy1 * 0.125f);
+    if (ix1 < 0 || iy1 < 0 || ix0 > (int32_t)tiles_x - 1 || iy0 > (int32_t)tiles_y - 1) return false;
+    x0 = (uint32_t)max(ix0, 0);
+    y0 = (uint32_t)max(iy0, 0);
+    x1 = (uint32_t)min(ix1, (int32_t)tiles_x - 1);
+    y1 = (uint32_t)min(iy1, (int32_t)tiles_y - 1);
+    return true;
+}
+
+RKF_DEV void tile_merge(uint2 *tiles, uint64_t i, float tn, float tf) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(tiles + i);
+    atomicMin(w, __float_as_uint(tn));
+    atomicMax(w + 1, __float_as_uint(tf));
+}
+
+__global__ __launch_bounds__(256) void tiles_clear_kernel(uint2 *tiles, uint32_t n, uint32_t *big_count) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0u) *big_count = 0u;
+    if (i < n) tiles[i] = make_uint2(0x7F800000u, 0u);
+}
+
+// One lane per L1 block; rectangles above kTileInlineMax tiles go to `big`.
+__global__ __launch_bounds__(256) void tiles_bin_kernel(SparseDev S, rkf_camera_uniforms cam, float max_distance,
+                                                        uint32_t w, uint32_t h, uint2 *tiles, TileBig *big,
+                                                        uint32_t *big_count) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= S.n_blocks) return;
+    const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
+    const TileCam tc = tile_cam(cam, w, h);
+    uint32_t x0, x1, y0, y1;
+    float tn, tf;
+    if (!tile_cell(S, tc, S.block_cell[i], max_distance, tiles_x, tiles_y, x0, x1, y0, y1, tn, tf)) return;
+    if ((x1 - x0 + 1u) * (y1 - y0 + 1u) <= kTileInlineMax) {
+        for (uint32_t y = y0; y <= y1; y++)
+            for (uint32_t x = x0; x <= x1; x++) tile_merge(tiles, (uint64_t)y * tiles_x + x, tn, tf);
+        return;
+    }
+    const uint32_t k = atomicAdd(big_count, 1u);
+    big[k] = TileBig{x0, x1, y0, y1, tn, tf};
+}
+
+// Large rectangles: one workgroup per record, grid-stride over the records.
+__global__ __launch_bounds__(256) void tiles_big_kernel(const TileBig *big, const uint32_t *big_count,
+                                                        uint32_t tiles_x, uint2 *tiles) {
+    const uint32_t n = *big_count;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const TileBig b = big[k];
+        const uint32_t rw = b.x1 - b.x0 + 1u, area = rw * (b.y1 - b.y0 + 1u);
+        for (uint32_t j = threadIdx.x; j < area; j += 256u)
+            tile_merge(tiles, (uint64_t)(b.y0 + j / rw) * tiles_x + (b.x0 + j % rw), b.tn, b.tf);
     }
 }
 
@@ -290,5 +441,20 @@ extern "C" hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const flo
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(rkf::sparse_query_kernel, dim3((n + 255u) / 256u, 1, 1), dim3(256, 1, 1), 0, stream, *S, xyz,
                        n, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam,
+                                              float max_distance, uint32_t width, uint32_t height, uint2 *tiles,
+                                              rkf::TileBig *big, uint32_t *big_count, hipStream_t stream) {
+    const uint32_t n_tiles = ((width + 7u) / 8u) * ((height + 7u) / 8u);
+    hipLaunchKernelGGL(rkf::tiles_clear_kernel, dim3((n_tiles + 255u) / 256u), dim3(256), 0, stream, tiles, n_tiles,
+                       big_count);
+    if (S->n_blocks) {
+        hipLaunchKernelGGL(rkf::tiles_bin_kernel, dim3((S->n_blocks + 255u) / 256u), dim3(256), 0, stream, *S, *cam,
+                           max_distance, width, height, tiles, big, big_count);
+        hipLaunchKernelGGL(rkf::tiles_big_kernel, dim3(512), dim3(256), 0, stream, (const rkf::TileBig *)big,
+                           (const uint32_t *)big_count, (width + 7u) / 8u, tiles);
+    }
     return hipGetLastError();
 }

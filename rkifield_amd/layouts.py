@@ -21,6 +21,7 @@ SDF_TYPE_NONE, SDF_TYPE_ANALYTICAL, SDF_TYPE_VOXELIZED = 0, 1, 2
 PRIM_SPHERE, PRIM_BOX, PRIM_CAPSULE, PRIM_TORUS, PRIM_CYLINDER, PRIM_PLANE = range(6)
 EPS_ABSOLUTE, EPS_RELATIVE = 0, 1
 STRATEGY_AUTO, STRATEGY_TILED, STRATEGY_BVH, STRATEGY_BRUTE = 0, 1, 2, 3
+MARCH_TILE_CULL = 1
 MAX_FLOAT = np.float32(3.402823e38)
 
 VOXEL_SAMPLE = np.dtype([("word0", "<u4"), ("word1", "<u4")])
@@ -82,7 +83,7 @@ SCENE_EXT = np.dtype([
     ("hit_eps_mode", "<u4"),
     ("strategy", "<u4"),
     ("shadow_enable", "<u4"),
-    ("_reserved0", "<u4"),
+    ("march_flags", "<u4"),
     ("light_dir", "<f4", (3,)),
     ("shadow_max_dist", "<f4"),
     ("shadow_k", "<f4"),
@@ -235,11 +236,12 @@ def scene_uniforms(num_objects=0, max_steps=128, max_distance=100.0, hit_thresho
 
 
 def scene_ext(hit_eps_mode=EPS_ABSOLUTE, strategy=STRATEGY_AUTO, shadow_enable=0,
-              light_dir=(0.0, 1.0, 0.0), shadow_max_dist=25.0, shadow_k=16.0):
+              light_dir=(0.0, 1.0, 0.0), shadow_max_dist=25.0, shadow_k=16.0, march_flags=0):
     e = np.zeros((), SCENE_EXT)
     e["hit_eps_mode"] = hit_eps_mode
     e["strategy"] = strategy
     e["shadow_enable"] = shadow_enable
+    e["march_flags"] = march_flags
     e["light_dir"] = light_dir
     e["shadow_max_dist"] = shadow_max_dist
     e["shadow_k"] = shadow_k

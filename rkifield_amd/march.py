@@ -73,6 +73,7 @@ _SIGS = {
     "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
+    "rkfm_sparse_tile_intervals": (C.c_int, [_VP, _VP, _VP, _VP, _U32, _U32, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
     "rkfm_sparse_contacts": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
     "rkfm_rkf_read_header": (C.c_int, [_VP, _U64, _VP, _VP, _U32]),
@@ -498,6 +499,17 @@ class SparseScene:
         e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
         _check(lib().rkfm_sparse_march(self.ctx.h, self.h, _p(c), _p(s), _p(e), gbuffer.desc.ctypes.data,
                                        variant, stream))
+
+    def tile_intervals(self, cam, su, width, height):
+        """(t_lo, t_hi) per 8x8 tile (rkfm_sparse_tile_intervals), shape
+        (tiles_y, tiles_x, 2) float32."""
+        c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+        s = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+        tx, ty = (width + 7) // 8, (height + 7) // 8
+        d = self.ctx.array((ty, tx, 2), np.float32)
+        _check(lib().rkfm_sparse_tile_intervals(self.ctx.h, self.h, _p(c), _p(s), width, height, d.ptr, None))
+        self.ctx.sync()
+        return d.download()
 
     def edit(self, ops, report=True, stream=None):
         """Applies CSG edits in order (rkfm_sparse_edit); with report=True

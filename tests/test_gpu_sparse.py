@@ -141,16 +141,47 @@ def test_free_masks_match_oracle(small_scene):
                     assert notfree == (0 if sp.brick_free(bx, by, bz) else 1)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
-def test_sparse_march_parity(ctx, small_scene, variant):
+def test_tile_intervals_match_oracle(ctx, small_scene):
+    """rkfm_sparse_tile_intervals equals the oracle's FP32 intervals bit for
+    bit: camera outside the grid, inside it, inside an occupied cell, with
+    jitter, odd image sizes."""
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    l2 = sp.l2
+    cells = [int(c) for c in sc.info["cells"]]
+    c0 = int(np.flatnonzero(l2 != L.EMPTY_SLOT)[len(np.flatnonzero(l2 != L.EMPTY_SLOT)) // 2])
+    cw = float(sc.info["voxel_size"]) * 64
+    org = np.asarray(sc.info["origin"], np.float64)
+    in_occ = org + (np.array([c0 % cells[0], (c0 // cells[0]) % cells[1], c0 // (cells[0] * cells[1])]) + 0.5) * cw
+    su = L.scene_uniforms(num_objects=1, max_distance=50.0)
+    views = [((6.0, 3.0, 7.0), (0, 0, 0), 96, 64, (0.0, 0.0)), ((0.5, 0.2, -0.3), (2.0, 1.0, 3.0), 101, 37, (0.3, -0.2)),
+             (tuple(in_occ), (0.0, 0.0, 0.0), 64, 64, (-0.5, 0.5)),
+             ((20.0, 10.0, 24.0), (40.0, 20.0, 48.0), 80, 48, (0.1, 0.1))]
+    nonempty = empty = 0
+    for eye, tgt, w, h, jit in views:
+        cam = M.camera_look_at(eye, tgt, 60.0, w, h)
+        cam["jitter"][:2] = jit
+        _say(f"case: tile intervals {w}x{h} eye {eye}")
+        got = sc.tile_intervals(cam, su, w, h)
+        want = O.sparse_tiles(sp, cam, su, w, h, prec=32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        fin = np.isfinite(got[..., 0])
+        nonempty += int(fin.sum())
+        empty += int((~fin).sum())
+    assert nonempty > 50 and empty > 10
+
+
+@pytest.mark.parametrize("variant,cull", [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1), (2, 1)])
+def test_sparse_march_parity(ctx, small_scene, variant, cull):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)
     w, h = 96, 64
     cam = M.camera_look_at((6.0, 3.0, 7.0), (0, 0, 0), 60.0, w, h)
     su = L.scene_uniforms(num_objects=1)
     ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1,
-                      light_dir=(np.array([0.5, 1.0, 0.3]) / np.linalg.norm([0.5, 1.0, 0.3])))
-    _say(f"case: sparse march variant {variant} {w}x{h}")
+                      light_dir=(np.array([0.5, 1.0, 0.3]) / np.linalg.norm([0.5, 1.0, 0.3])),
+                      march_flags=L.MARCH_TILE_CULL if cull else 0)
+    _say(f"case: sparse march variant {variant} cull {cull} {w}x{h}")
     gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
     sc.march(cam, su, ext, gb, variant=variant)
     gpu = gb.read()

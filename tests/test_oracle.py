@@ -583,3 +583,139 @@ def test_contact_sample_points_match_shapes():
     corners = rel[1][np.argsort(ct[1]["sample"][: n[1]])][:8]
     assert np.allclose(np.abs(corners), [0.1, 0.2, 0.3], atol=1e-6)
     assert np.isclose(rel[2][:, 1].max(), 0.35, atol=1e-6) and np.isclose(rel[2][:, 1].min(), -0.35, atol=1e-6)
+
+
+# ---------------- tile intervals (RKFM_MARCH_TILE_CULL) ----------------
+
+def _camera(eye, target, w, h, jitter=(0.0, 0.0)):
+    from rkifield_amd import march as M
+    cam = M.camera_look_at(eye, target, 60.0, w, h)
+    cam["jitter"][:2] = jitter
+    return cam
+
+
+def _ray_dirs(cam, w, h):
+    px, py = np.meshgrid(np.arange(w), np.arange(h))
+    ux = (px + 0.5 + cam["jitter"][0]) / w
+    uy = (py + 0.5 + cam["jitter"][1]) / h
+    nx, ny = ux * 2 - 1, 1 - uy * 2
+    f, r, u = (np.asarray(cam[k][:3], np.float64) for k in ("forward", "right", "up"))
+    d = f + nx[..., None] * r + ny[..., None] * u
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def _inside_scene(seed, cells, n_occ, bricks_per_cell, vs):
+    """Random occupied cells (boundary cells included), each with a few
+    random bricks filled with -vs: a sample any stored brick reaches reads
+    below the free value 8 vs."""
+    rng = np.random.default_rng(seed)
+    nc = cells[0] * cells[1] * cells[2]
+    l2 = np.full(nc, L.EMPTY_SLOT, np.uint32)
+    occ = rng.choice(nc, size=n_occ, replace=False)
+    occ[0] = 0
+    occ[1] = nc - 1
+    l1 = np.full(n_occ * 512, L.EMPTY_SLOT, np.uint32)
+    nb = 0
+    for k, c in enumerate(occ):
+        l2[c] = k
+        lbs = rng.choice(512, size=bricks_per_cell, replace=False)
+        lbs[0] = 0 if k % 2 else 511
+        for lb in lbs:
+            l1[k * 512 + lb] = nb
+            nb += 1
+    dist = np.full(nb * 512, O.f32_to_f16_bits([-vs])[0], np.uint16)
+    return O.SparseInputs((-1.0, -0.5, -2.0), vs, cells, l2, l1, dist, np.zeros(nb * 512, np.uint8))
+
+
+def test_tile_intervals_are_sound():
+    """Outside its tile's [t_lo, t_hi] every sample along every pixel ray
+    (step vs/2 up to max_distance) reads at least the free value 8 vs, for
+    cameras outside the grid, inside it, inside an occupied cell and with
+    sub-pixel jitter; intervals are non-trivial (empty tiles, samples below
+    8 vs inside the interval). FP32 and FP64 intervals bracket the same
+    samples."""
+    vs = 0.05
+    cells = (5, 3, 4)
+    sp = _inside_scene(3, cells, 9, 6, vs)
+    grid = np.asarray(cells, np.float64) * 64 * vs
+    origin = np.array([-1.0, -0.5, -2.0])
+    occ = np.flatnonzero(sp.l2 != L.EMPTY_SLOT)
+    c0 = occ[2]
+    inside_occ = origin + (np.array([c0 % cells[0], (c0 // cells[0]) % cells[1], c0 // (cells[0] * cells[1])]) + 0.5) * 64 * vs
+    w, h = 56, 40
+    su = L.scene_uniforms(num_objects=1, max_distance=40.0)
+    views = [((-8.0, 9.0, 14.0), origin + grid / 2, (0.0, 0.0)),
+             (tuple(origin + grid * 0.4), tuple(origin + grid), (0.3, -0.4)),
+             (tuple(inside_occ), tuple(origin), (-0.5, 0.5)),
+             (tuple(origin + grid * np.array([0.5, 1.02, 0.5])), tuple(origin + grid * np.array([0.2, 0.5, 0.9])),
+              (0.0, 0.25))]
+    total_out = total_in_low = empty = 0
+    for eye, tgt, jit in views:
+        cam = _camera(eye, tgt, w, h, jit)
+        iv = O.sparse_tiles(sp, cam, su, w, h, prec=32)
+        iv64 = O.sparse_tiles(sp, cam, su, w, h, prec=64)
+        assert np.array_equal(np.isinf(iv[..., 0]), np.isinf(iv64[..., 0]))
+        fin = ~np.isinf(iv[..., 0])
+        assert np.allclose(iv[fin], iv64[fin], rtol=1e-5, atol=1e-5)
+        empty += int((~fin).sum())
+        dirs = _ray_dirs(cam, w, h)
+        o = np.asarray(cam["position"][:3], np.float64)
+        ts = np.arange(0.0, float(su["max_distance"]), vs * 0.5)
+        for py in range(0, h, 3):
+            for px in range(0, w, 3):
+                lo, hi = (float(x) for x in iv[py >> 3, px >> 3])
+                for t in ts:
+                    d = O.sparse_sample(sp, (o + dirs[py, px] * t).astype(np.float32))
+                    if t < lo or t > hi:
+                        assert d >= 8 * vs * (1 - 1e-5), (eye, px, py, t, lo, hi, d)
+                        total_out += 1
+                    elif d < 8 * vs:
+                        total_in_low += 1
+    assert empty > 0 and total_out > 100000 and total_in_low > 100
+
+
+def _cross_grid_scene():
+    """Copies of test_cross.rkf's 8x4x8 bricks at four places of a 40x16x40
+    brick grid (5x2x5 cells): corner and boundary cells, empty space between."""
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    cd = [int(x) for x in h["dims"]]
+    dims = (40, 16, 40)
+    big = np.full(dims[0] * dims[1] * dims[2], L.EMPTY_SLOT, np.uint32)
+    bricks = []
+    for k, (ox, oy, oz) in enumerate(((0, 0, 0), (16, 8, 16), (32, 4, 8), (8, 12, 32))):
+        for bz in range(cd[2]):
+            for by in range(cd[1]):
+                for bx in range(cd[0]):
+                    e = m[bx + by * cd[0] + bz * cd[0] * cd[1]]
+                    if e == L.EMPTY_SLOT:
+                        continue
+                    big[(ox + bx) + (oy + by) * dims[0] + (oz + bz) * dims[0] * dims[1]] = len(bricks)
+                    bricks.append(b[e * 512:(e + 1) * 512])
+    vs = np.float32(h["voxel_size"])
+    return _sparse_from_dense_host(dims, big, np.concatenate(bricks), vs, (-1.0, -0.5, -1.5)), float(vs)
+
+
+def test_tile_culled_march_agrees_with_unculled():
+    """The culled march (FP32 twin and FP64 oracle) finds the same surfaces
+    as the plain march: same hit mask, depths within the hit band (the
+    relative hit threshold), same materials, fewer loop iterations."""
+    sp, vs = _cross_grid_scene()
+    w, h = 64, 48
+    su = L.scene_uniforms(num_objects=1, max_distance=30.0)
+    for eye, tgt in (((3.5, 2.2, 5.5), (1.5, 0.8, 1.5)), ((0.3, 0.2, 1.0), (-1.0, 0.0, -1.0))):
+        cam = _camera(eye, tgt, w, h)
+        plain = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE)
+        cull = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, march_flags=L.MARCH_TILE_CULL)
+        for prec in (32, 64):
+            a = O.sparse_trace(sp, cam, su, plain, w, h, prec=prec)
+            b = O.sparse_trace(sp, cam, su, cull, w, h, prec=prec)
+            assert a["hit"].sum() > 200
+            assert np.array_equal(a["hit"], b["hit"])
+            hit = a["hit"] == 1
+            # a march stops anywhere its sample drops below the hit threshold
+            # (relative: 1e-3 t), so two step sequences agree to that band
+            rel = np.abs(a["t"][hit] - b["t"][hit]) / a["t"][hit]
+            assert rel.max() <= float(su["hit_threshold"]) * 1.01 and np.median(rel) <= 1e-6
+            assert np.array_equal(a["material_id"], b["material_id"])
+            assert b["steps"].sum() < a["steps"].sum()
