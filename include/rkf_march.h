@@ -581,6 +581,11 @@ typedef struct rkfm_batch_item {
 
 #define RKFM_SCHEDULE_GRID 0u       /* one wavefront per work unit           */
 #define RKFM_SCHEDULE_PERSISTENT 1u /* persistent wavefronts pull work units */
+/* One wavefront per work unit with the per-ray code of the single-view
+ * kernels (each ray keeps the slots of its current brick neighbourhood:
+ * fewer dependent loads per step, more memory instructions): for batch sets
+ * whose time is set by their slowest rays rather than by their total work. */
+#define RKFM_SCHEDULE_GRID_LOWLAT 2u
 
 /* Launch shape: views are cut into tile_w x tile_h pixel packets
  * (tile_w * tile_h == 64, one packet per wavefront); shadow and point batches
@@ -639,7 +644,8 @@ typedef struct rkfm_server_config {
                                /* round always takes at least one request)  */
     uint32_t max_round_requests; /* requests per round (0 = 512)             */
     uint32_t policy;           /* RKFM_SERVE_*                               */
-    rkfm_batch_tuning tuning;  /* batched launch shape (zero = grid 8x8)     */
+    rkfm_batch_tuning tuning;  /* batched launch shape (zero = low-latency   */
+                               /* grid, 8x8)                                 */
     uint32_t max_pending;      /* submitted, not yet admitted (0 = 65536)    */
     uint32_t _reserved;
 } rkfm_server_config;

@@ -10,8 +10,10 @@
 //
 // Schedules: GRID launches one wavefront per unit and lets the hardware
 // dispatcher balance; PERSISTENT launches waves_per_cu x CUs wavefronts that
-// pull units from a counter (one atomic per 64 rays). The autotuner times both
-// and the packet shapes on the actual batch set.
+// pull units from a counter (one atomic per 64 rays); GRID_LOWLAT is GRID
+// with the single-view kernels' per-ray code (brick-neighbourhood cache),
+// which shortens the slowest rays at the price of more memory instructions.
+// The autotuner times them and the packet shapes on the actual batch set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,12 +36,13 @@ RKF_DEV uint32_t find_batch(const uint32_t *begins, uint32_t n_items, uint32_t u
     return lo;
 }
 
+template <bool kLowLat>
 RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
     if (B.kind == RKFM_BATCH_VIEW) {
         uint32_t tx = u % B.tiles_x, ty = u / B.tiles_x;
         uint32_t px = tx * B.tile_w + lane % B.tile_w;
         uint32_t py = ty * B.tile_h + lane / B.tile_w;
-        if (px < B.P.g.width && py < B.P.g.height) view_pixel<false>(B.P, px, py);
+        if (px < B.P.g.width && py < B.P.g.height) view_pixel<kLowLat>(B.P, px, py);
     } else if (B.kind == RKFM_BATCH_SHADOW) {
         uint32_t i = u * 64u + lane;
         if (i >= B.n) return;
@@ -47,7 +50,7 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
         ShadowRay sr;
         shadow_init(sr, o, mk(B.P.ext.light_dir[0], B.P.ext.light_dir[1], B.P.ext.light_dir[2]), B.P.S.vs);
         float maxd = shadow_maxd(B.P), k = shadow_k(B.P);
-        while (!shadow_step<false>(B.P.S, sr, maxd, k)) {
+        while (!shadow_step<kLowLat>(B.P.S, sr, maxd, k)) {
         }
         B.vis[i] = sr.prev_dist;
     } else {
@@ -58,13 +61,14 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
     }
 }
 
+template <bool kLowLat>
 __global__ __launch_bounds__(64) void batch_grid_kernel(const BatchDev *items, const uint32_t *begins,
                                                         uint32_t n_items, uint32_t total) {
     uint32_t unit = blockIdx.y * gridDim.x + blockIdx.x;
     if (unit >= total) return;
     uint32_t b = find_batch(begins, n_items, unit);
     const BatchDev &B = items[b];
-    run_unit(B, unit - B.unit_begin, threadIdx.x);
+    run_unit<kLowLat>(B, unit - B.unit_begin, threadIdx.x);
 }
 
 __global__ __launch_bounds__(64) void batch_persistent_kernel(const BatchDev *items, const uint32_t *begins,
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(64) void batch_persistent_kernel(const BatchDev *it
         if (unit >= total) break;
         uint32_t b = find_batch(begins, n_items, unit);
         const BatchDev &B = items[b];
-        run_unit(B, unit - B.unit_begin, lane);
+        run_unit<false>(B, unit - B.unit_begin, lane);
     }
 }
 
@@ -101,7 +105,7 @@ bool owned(const rkfm_ctx *c, const rkfm_sparse_scene *s) {
 }  // namespace
 
 bool rkf_batch_valid_shape(const rkfm_batch_tuning &t) {
-    if (t.schedule > RKFM_SCHEDULE_PERSISTENT) return false;
+    if (t.schedule > RKFM_SCHEDULE_GRID_LOWLAT) return false;
     if (t.tile_w == 0 || t.tile_h == 0 || t.tile_w * t.tile_h != 64u) return false;
     if (t.schedule == RKFM_SCHEDULE_PERSISTENT && (t.waves_per_cu == 0 || t.waves_per_cu > 64)) return false;
     return true;
@@ -184,10 +188,13 @@ int rkf_device_waves(uint32_t per_cu, uint32_t &waves) {
 int rkf_batch_enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32_t *begins, uint32_t n,
             uint32_t total, uint32_t *counter, uint32_t waves, hipStream_t st) {
     if (total == 0) return RKFM_OK;
-    if (t.schedule == RKFM_SCHEDULE_GRID) {
+    if (t.schedule == RKFM_SCHEDULE_GRID || t.schedule == RKFM_SCHEDULE_GRID_LOWLAT) {
         uint32_t gx = total < 65536u ? total : 65536u;
         uint32_t gy = (total + gx - 1) / gx;
-        hipLaunchKernelGGL(rkf::batch_grid_kernel, dim3(gx, gy), dim3(64), 0, st, items, begins, n, total);
+        if (t.schedule == RKFM_SCHEDULE_GRID)
+            hipLaunchKernelGGL(rkf::batch_grid_kernel<false>, dim3(gx, gy), dim3(64), 0, st, items, begins, n, total);
+        else
+            hipLaunchKernelGGL(rkf::batch_grid_kernel<true>, dim3(gx, gy), dim3(64), 0, st, items, begins, n, total);
     } else {
         if (waves > total) waves = total;
         if (waves == 0) waves = 1;
@@ -261,6 +268,7 @@ int rkfm_batch_autotune(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, u
     static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}};
     std::vector<rkfm_batch_tuning> cands;
     for (auto &s : shapes) cands.push_back({RKFM_SCHEDULE_GRID, s[0], s[1], 0});
+    for (int k = 0; k < 2; k++) cands.push_back({RKFM_SCHEDULE_GRID_LOWLAT, shapes[k][0], shapes[k][1], 0});
     for (uint32_t w : {8u, 16u, 24u})
         for (int k = 0; k < 2; k++) cands.push_back({RKFM_SCHEDULE_PERSISTENT, shapes[k][0], shapes[k][1], w});
     float best_t = 3.4e38f;

@@ -501,7 +501,7 @@ int rkfm_server_create(rkfm_ctx *c, const rkfm_sparse_scene *const *scenes, uint
     if (k.max_round_requests == 0) k.max_round_requests = 512;
     if (k.max_pending == 0) k.max_pending = 65536;
     rkfm_batch_tuning t = k.tuning;
-    if (t.tile_w == 0 && t.tile_h == 0) t = rkf_batch_default_tuning(c);
+    if (t.tile_w == 0 && t.tile_h == 0) t = rkfm_batch_tuning{RKFM_SCHEDULE_GRID_LOWLAT, 8u, 8u, 0u};
     if (!rkf_batch_valid_shape(t)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "server_create: invalid launch shape");
     k.tuning = t;
     int rc = rkf_hip_check(hipSetDevice(c->device), "hipSetDevice");

@@ -141,7 +141,7 @@ GBUFFER_DESC = np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<
                          ("debug", "<u8"), ("shadow", "<u8"), ("width", "<u4"), ("height", "<u4")])
 
 BATCH_VIEW, BATCH_SHADOW, BATCH_POINTS = 0, 1, 2
-SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
+SCHEDULE_GRID, SCHEDULE_PERSISTENT, SCHEDULE_GRID_LOWLAT = 0, 1, 2
 
 BATCH_ITEM = np.dtype([
     ("kind", "<u4"), ("n", "<u4"), ("scene", "<u8"),

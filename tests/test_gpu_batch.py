@@ -13,7 +13,8 @@ from rkifield_amd import scenes
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [(L.SCHEDULE_GRID, 8, 8, 0), (L.SCHEDULE_GRID, 32, 2, 0), (L.SCHEDULE_GRID, 4, 16, 0),
-           (L.SCHEDULE_PERSISTENT, 16, 4, 16), (L.SCHEDULE_PERSISTENT, 8, 8, 4)]
+           (L.SCHEDULE_PERSISTENT, 16, 4, 16), (L.SCHEDULE_PERSISTENT, 8, 8, 4),
+           (L.SCHEDULE_GRID_LOWLAT, 8, 8, 0), (L.SCHEDULE_GRID_LOWLAT, 16, 4, 0)]
 
 
 def _say(msg):
@@ -122,7 +123,7 @@ def test_graph_replays_and_autotune(ctx, two_scenes):
     b.run(tuning=TUNINGS[0])
     ctx.sync()
     ref_sh = [s[2].download() for s in shadows]
-    for t in (TUNINGS[0], TUNINGS[3]):
+    for t in (TUNINGS[0], TUNINGS[3], TUNINGS[5]):
         _say(f"case: graph tuning {t}")
         g = b.graph(tuning=t)
         for rep in range(3):
@@ -151,3 +152,5 @@ def test_batch_errors(ctx, two_scenes):
     b.add_view(two_scenes[0], M.camera_look_at((5, 5, 5), (0, 0, 0), 60.0, 8, 8), L.scene_uniforms(), _ext(False), gb)
     with pytest.raises(M.RkfMarchError):
         b.run(tuning=(L.SCHEDULE_GRID, 8, 4, 0))  # 32 lanes: invalid packet
+    with pytest.raises(M.RkfMarchError):
+        b.run(tuning=(3, 8, 8, 0))  # no such schedule
