@@ -26,7 +26,7 @@ struct DevBuf {
 struct rkfm_sparse_scene {
     rkfm_ctx *owner = nullptr;
     rkfm_sparse_info info{};
-    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, oct, cellw, bext;
+    DevBuf l2, l1, dist, mat, aux, free_idx, free_bits, oct, cellw, roww;
     DevBuf block_cell;    // per L1 block: its flat L2 cell
     DevBuf edit_touched;  // one bit per slot, all zero between edit calls
     uint64_t n_free_rows = 0;

@@ -62,8 +62,8 @@ struct SparseDev {
     const uint32_t *free_idx;  // per L2 cell: kFreeCell or row of free_bits
     const uint64_t *free_bits; // 8 words per row: bit set = brick NOT free
     const uint32_t *oct;       // per L2 cell: 8 octant extents, 4 bits each
-    const uint8_t *bext;       // per row: 8 octant planes of 512 brick extents,
-                               // 4 bits each ((row * 8 + o) * 256 + lb / 2)
+    const uint64_t *roww;      // per row brick: octant extents (4 bits each) << 32
+                               // | L1 slot (EMPTY in rows past n_blocks)
     const uint64_t *cellw;     // per L2 cell, what one DDA step needs in one
                                // load: free row << 32 | octant extents
     uint64_t n_cells;

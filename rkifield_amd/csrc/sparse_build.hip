@@ -315,7 +315,7 @@ constexpr uint32_t kOctPasses = 16u;  // >= 15, even: the result lands in the fi
 // which is at least 1. Every brick of the cube b + [0, E_o(b)) in o's
 // directions is then free. Jacobi passes from "free = 15" over packed u32
 // (8 nibbles per brick) reach the fixed point in 15 passes; the march reads
-// per-octant nibble planes.
+// them from the row words.
 struct BextArgs {
     uint32_t cells[3];
     uint32_t nb[3];
@@ -403,16 +403,16 @@ __global__ __launch_bounds__(512) void bext_relax_kernel(BextArgs A, const uint3
     nxt[self] = out;
 }
 
-// Packed u32 per brick -> per-octant nibble planes (brick lb in the low
-// nibble when lb is even).
-__global__ __launch_bounds__(256) void bext_planes_kernel(uint64_t n_rows, const uint32_t *packed, uint8_t *planes) {
+// Row word per brick: packed octant extents << 32 | L1 slot (EMPTY in rows
+// past n_blocks, which hold no block): one load gives a DDA step both the
+// brick's extent and the slot it samples.
+__global__ __launch_bounds__(256) void rowword_kernel(uint64_t n_rows, uint32_t n_blocks, const uint32_t *packed,
+                                                      const uint32_t *l1, uint64_t *roww) {
     uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x + (uint64_t)blockIdx.y * gridDim.x * 256u;
-    if (i >= n_rows * 2048u) return;
-    const uint64_t row = i / 2048u;
-    const uint32_t o = (uint32_t)((i / 256u) % 8u), j = (uint32_t)(i % 256u);
-    const uint32_t lo = (packed[row * 512u + 2u * j] >> (4u * o)) & 15u;
-    const uint32_t hi = (packed[row * 512u + 2u * j + 1u] >> (4u * o)) & 15u;
-    planes[i] = (uint8_t)(lo | (hi << 4));
+    if (i >= n_rows * 512u) return;
+    const uint64_t row = i / 512u;
+    const uint32_t slot = row < n_blocks ? l1[i] : RKF_EMPTY_SLOT;
+    roww[i] = ((uint64_t)packed[i] << 32) | slot;
 }
 
 // cellw = free row << 32 | octant extents (row kFreeCell: every brick free).
@@ -443,7 +443,7 @@ rkf::SparseDev rkfm_sparse_scene::dev() const {
     d.free_idx = (const uint32_t *)free_idx.ptr;
     d.free_bits = (const uint64_t *)free_bits.ptr;
     d.oct = (const uint32_t *)oct.ptr;
-    d.bext = (const uint8_t *)bext.ptr;
+    d.roww = (const uint64_t *)roww.ptr;
     d.cellw = (const uint64_t *)cellw.ptr;
     d.n_cells = (uint64_t)info.cells[0] * info.cells[1] * info.cells[2];
     d.n_blocks = info.n_blocks;
@@ -461,7 +461,7 @@ static void release_scene(rkfm_sparse_scene *s) {
     s->free_bits.release();
     s->oct.release();
     s->cellw.release();
-    s->bext.release();
+    s->roww.release();
     s->block_cell.release();
     s->edit_touched.release();
 }
@@ -585,12 +585,12 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
     if (rc) return rc;
     // brick extents inside rows
     const uint64_t nrows = rows.size();
-    if (!rc) rc = s->bext.reserve(nrows * 2048 + 64, false);
+    if (!rc) rc = s->roww.reserve(nrows * 4096 + 64, false);
     DevBuf *pa = tmp.make(), *pb = tmp.make();
     if (!rc) rc = pa->reserve(nrows * 2048 + 64, false);
     if (!rc) rc = pb->reserve(nrows * 2048 + 64, false);
     if (rc) return rc;
-    s->bext.size = nrows * 2048;
+    s->roww.size = nrows * 4096;
     if (nrows) {
         rkf::BextArgs A;
         for (int a = 0; a < 3; a++) {
@@ -609,10 +609,11 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
             hipLaunchKernelGGL(rkf::bext_relax_kernel, rkf::item_grid(nrows), dim3(512), 0, c->stream, A, bc, bn);
             std::swap(bc, bn);
         }
-        const uint64_t n_pl = nrows * 2048u;
-        const uint64_t blocks = (n_pl + 255u) / 256u;
+        const uint64_t n_w = nrows * 512u;
+        const uint64_t blocks = (n_w + 255u) / 256u;
         dim3 pg((unsigned)(blocks < 65535u ? blocks : 65535u), (unsigned)((blocks + 65534u) / 65535u));
-        hipLaunchKernelGGL(rkf::bext_planes_kernel, pg, dim3(256), 0, c->stream, nrows, bc, (uint8_t *)s->bext.ptr);
+        hipLaunchKernelGGL(rkf::rowword_kernel, pg, dim3(256), 0, c->stream, nrows, s->info.n_blocks, bc,
+                           (const uint32_t *)s->l1.ptr, (uint64_t *)s->roww.ptr);
         rc = rkf_hip_check(hipGetLastError(), "brick extent launch");
         if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "brick extent sync");
     }
@@ -622,7 +623,7 @@ int build_free_masks(rkfm_ctx *c, rkfm_sparse_scene *s, const std::vector<uint32
 
 void finish_info(rkfm_sparse_scene *s) {
     s->info.device_bytes = s->l2.cap + s->l1.cap + s->dist.cap + s->mat.cap + s->aux.cap + s->free_idx.cap +
-                           s->free_bits.cap + s->oct.cap + s->cellw.cap + s->bext.cap + s->block_cell.cap;
+                           s->free_bits.cap + s->oct.cap + s->cellw.cap + s->roww.cap + s->block_cell.cap;
 }
 
 }  // namespace
@@ -944,8 +945,18 @@ int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t 
     if (!rc && free_bits && s->free_bits.size)
         rc = rkf_hip_check(hipMemcpy(free_bits, s->free_bits.ptr, s->free_bits.size, hipMemcpyDeviceToHost), "free bits");
     if (!rc && oct) rc = rkf_hip_check(hipMemcpy(oct, s->oct.ptr, s->oct.size, hipMemcpyDeviceToHost), "octants");
-    if (!rc && brick_ext && s->bext.size)
-        rc = rkf_hip_check(hipMemcpy(brick_ext, s->bext.ptr, s->bext.size, hipMemcpyDeviceToHost), "brick extents");
+    if (!rc && brick_ext && s->roww.size) {
+        // octant planes (row * 8 + o) * 256 + lb / 2, low nibble for even lb
+        std::vector<uint64_t> w(s->n_free_rows * 512u);
+        rc = rkf_hip_check(hipMemcpy(w.data(), s->roww.ptr, w.size() * 8, hipMemcpyDeviceToHost), "brick extents");
+        for (uint64_t r = 0; !rc && r < s->n_free_rows; r++)
+            for (uint32_t o = 0; o < 8u; o++)
+                for (uint32_t j = 0; j < 256u; j++) {
+                    const uint32_t lo = (uint32_t)(w[r * 512u + 2u * j] >> (32u + 4u * o)) & 15u;
+                    const uint32_t hi = (uint32_t)(w[r * 512u + 2u * j + 1u] >> (32u + 4u * o)) & 15u;
+                    brick_ext[(r * 8u + o) * 256u + j] = (uint8_t)(lo | (hi << 4));
+                }
+    }
     return rc;
 }
 

@@ -237,10 +237,9 @@ RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
 }
 
 // Free-space test + exit parameter (see file header). One load of the cell
-// word decides cell-level skips; otherwise the brick's free bit and its L1
-// slot (the row is the L1 block when below n_blocks) are loaded side by side,
-// so a step that must sample has its slot (returned in `slot`, else kNoSlot)
-// after two dependent loads.
+// word decides cell-level skips; otherwise one load of the brick's row word
+// gives its extent and its L1 slot, so a step that must sample has its slot
+// (returned in `slot`, else kNoSlot) after two dependent loads.
 RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, uint32_t &slot) {
     slot = kNoSlot;
     float vs = S.vs;
@@ -273,11 +272,10 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
     } else {
         uint32_t row = (uint32_t)(cw >> 32);
         uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
-        uint32_t eb = S.bext[((uint64_t)row * 8u + octant) * 256u + (lb >> 1)];
-        uint32_t sl = row < S.n_blocks ? S.l1[(uint64_t)row * 512u + lb] : RKF_EMPTY_SLOT;
-        eb = (eb >> ((lb & 1u) * 4u)) & 15u;
+        const uint64_t rw = S.roww[(uint64_t)row * 512u + lb];
+        const uint32_t eb = (uint32_t)(rw >> (32u + 4u * octant)) & 15u;
         if (eb == 0) {
-            slot = sl;
+            slot = (uint32_t)rw;
             return false;
         }
         // the cube of bricks b + [0, eb) in the travel direction of each axis is free
@@ -306,9 +304,14 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
 struct BrickNbhd {
     uint64_t key;      // cell * 512 + (brick in cell) of b0; ~0 when empty
     uint32_t slot[8];  // brick b0 + (k & 1, k >> 1 & 1, k >> 2); constant indices only
+    uint32_t cell;     // last cell met that is not free for this ray's octant, ~0 when none
+    uint32_t row;      // its free-mask row
 };
 
-RKF_DEV void nb_reset(BrickNbhd &n) { n.key = ~0ull; }
+RKF_DEV void nb_reset(BrickNbhd &n) {
+    n.key = ~0ull;
+    n.cell = ~0u;
+}
 
 // Slot of corner brick (ox, oy, oz) in {0,1}^3 by selects, so the slots stay
 // in registers.
@@ -412,9 +415,16 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
     const uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     const uint64_t cell = key >> 9;
     float lo_b[3], hi_b[3];
-    const uint64_t cw = S.cellw[cell];
     const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
-    const uint32_t e = ((uint32_t)cw >> (4u * octant)) & 15u;
+    uint32_t e = 0u;
+    if ((uint32_t)cell != n.cell) {
+        const uint64_t cw = S.cellw[cell];
+        e = ((uint32_t)cw >> (4u * octant)) & 15u;
+        if (e == 0) {
+            n.cell = (uint32_t)cell;
+            n.row = (uint32_t)(cw >> 32);
+        }
+    }
     if (e > 0) {
         // the cube of cells q + [0, e) in the travel direction of each axis is free
 #pragma unroll
@@ -423,30 +433,35 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
             hi_b[a] = ((float)q[a] + (float)e) * 64.0f;
         }
     } else {
-        const uint32_t row = (uint32_t)(cw >> 32);
+        // The brick's row word (extent and slot) and, side by side, the
+        // slots of the rest of its 2x2x2 neighbourhood that lie in the same
+        // cell, without branches so the loads go out together; for a brick
+        // that is not free, bricks of neighbouring cells follow through
+        // their L2 and L1 entries (only when some lane of the wave needs
+        // them). Unused loads read entry 0 (the arrays always hold one);
+        // bricks past the grid end count EMPTY.
+        const uint32_t row = n.row;
         const uint32_t lb = (uint32_t)(key & 511u);
-        const uint32_t eb = (S.bext[((uint64_t)row * 8u + octant) * 256u + (lb >> 1)] >> ((lb & 1u) * 4u)) & 15u;
-        if (eb == 0) {
-            // A brick that is not free: load the slots of its 2x2x2
-            // neighbourhood without branches, so each round's loads go out
-            // together: bricks of the same cell from the cell's L1 block (the
-            // row is the block when below n_blocks), bricks of neighbouring
-            // cells through their L2 entries, then their L1 entries (those
-            // two rounds only when some lane of the wave needs them). Unused
-            // loads read entry 0 (the arrays always hold one); bricks past
-            // the grid end count EMPTY.
-            const bool blk_ok = row < S.n_blocks;
-            uint32_t va[8], vc[8], vb[8], lbk[8];
-            bool inb[8], same[8], far[8];
+        uint64_t rw = S.roww[(uint64_t)row * 512u + lb];
+        uint32_t va[8], vc[8], vb[8], lbk[8];
+        bool inb[8], same[8], far[8];
 #pragma unroll
-            for (uint32_t k = 0; k < 8u; k++) {
-                const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
-                inb[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2];
-                same[k] = (bx >> 3) == q[0] && (by >> 3) == q[1] && (bz >> 3) == q[2];
-                lbk[k] = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
-                va[k] = S.l1[inb[k] && same[k] && blk_ok ? (uint64_t)row * 512u + lbk[k] : 0u];
-                vc[k] = RKF_EMPTY_SLOT;
-            }
+        for (uint32_t k = 0; k < 8u; k++) {
+            const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
+            inb[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2];
+            same[k] = (bx >> 3) == q[0] && (by >> 3) == q[1] && (bz >> 3) == q[2];
+            lbk[k] = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+            va[k] = k == 0u ? 0u : (uint32_t)S.roww[inb[k] && same[k] ? (uint64_t)row * 512u + lbk[k] : 0u];
+            vc[k] = RKF_EMPTY_SLOT;
+        }
+        uint32_t rw_lo = (uint32_t)rw, rw_hi = (uint32_t)(rw >> 32);
+        loaded(rw_lo);
+        loaded(rw_hi);
+#pragma unroll
+        for (uint32_t k = 1; k < 8u; k++) loaded(va[k]);
+        va[0] = rw_lo;
+        const uint32_t eb = (rw_hi >> (4u * octant)) & 15u;
+        if (eb == 0) {
             const bool cross = (b[0] & 7u) == 7u || (b[1] & 7u) == 7u || (b[2] & 7u) == 7u;
             if (__any(cross)) {
 #pragma unroll
@@ -459,7 +474,6 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
             }
 #pragma unroll
             for (uint32_t k = 0; k < 8u; k++) {
-                loaded(va[k]);
                 loaded(vc[k]);
                 far[k] = inb[k] && !same[k] && vc[k] != RKF_EMPTY_SLOT;
                 vb[k] = RKF_EMPTY_SLOT;
@@ -472,8 +486,7 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
 #pragma unroll
             for (uint32_t k = 0; k < 8u; k++) {
                 loaded(vb[k]);
-                n.slot[k] = !inb[k] ? RKF_EMPTY_SLOT : same[k] ? (blk_ok ? va[k] : RKF_EMPTY_SLOT)
-                                                               : (far[k] ? vb[k] : RKF_EMPTY_SLOT);
+                n.slot[k] = !inb[k] ? RKF_EMPTY_SLOT : same[k] ? va[k] : (far[k] ? vb[k] : RKF_EMPTY_SLOT);
             }
             return kLocCached;
         }
