@@ -352,55 +352,71 @@ __global__ __launch_bounds__(512) void bext_init_kernel(BextArgs A, uint32_t *cu
     cur[item * 512u + t] = nf ? 0u : 0xFFFFFFFFu;
 }
 
+// One Jacobi pass, one workgroup per row: the row's 8x8x8 bricks and the
+// one-brick shell around them (from neighbouring rows, the free-cell bound
+// of free neighbouring cells, or 15 past the grid end) are staged in LDS, so
+// each brick's 26 neighbours are LDS reads.
+constexpr uint32_t kOutOfGrid = 0xFFFFFFFEu;
+
 __global__ __launch_bounds__(512) void bext_relax_kernel(BextArgs A, const uint32_t *cur, uint32_t *nxt) {
-    uint64_t item = item_index();
+    __shared__ uint32_t tile[1000];  // 10 x 10 x 10 bricks: the cell and its shell
+    __shared__ uint32_t nrow[27];    // rows of the 3 x 3 x 3 cells around (kFreeCell / kOutOfGrid)
+    const uint64_t item = item_index();
     if (item >= A.n_rows) return;
     const uint32_t t = threadIdx.x;
-    const uint64_t self = item * 512u + t;
-    if (cur[self] == 0u) {
-        nxt[self] = 0u;
+    const uint32_t cell = A.row_cell[item];
+    const int64_t cq[3] = {(int64_t)(cell % A.cells[0]), (int64_t)((cell / A.cells[0]) % A.cells[1]),
+                           (int64_t)(cell / (A.cells[0] * A.cells[1]))};
+    if (t < 27u) {
+        const int64_t x = cq[0] + (int64_t)(t % 3u) - 1, y = cq[1] + (int64_t)((t / 3u) % 3u) - 1,
+                      z = cq[2] + (int64_t)(t / 9u) - 1;
+        nrow[t] = (x < 0 || y < 0 || z < 0 || x >= A.cells[0] || y >= A.cells[1] || z >= A.cells[2])
+                      ? kOutOfGrid
+                      : (t == 13u ? (uint32_t)item
+                                  : A.free_idx[(uint64_t)x + (uint64_t)y * A.cells[0] + (uint64_t)z * A.cells[0] * A.cells[1]]);
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < 1000u; i += 512u) {
+        const uint32_t h[3] = {i % 10u, (i / 10u) % 10u, i / 100u};
+        int64_t bb[3];
+        uint32_t nc = 0u;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            bb[a] = cq[a] * 8 - 1 + (int64_t)h[a];
+            nc += (h[a] == 0u ? 0u : (h[a] == 9u ? 2u : 1u)) * (a == 0 ? 1u : (a == 1 ? 3u : 9u));
+        }
+        const uint32_t r2 = nrow[nc];
+        uint32_t v;
+        if (r2 == kOutOfGrid) {
+            v = 0xFFFFFFFFu;
+        } else if (r2 == kFreeCell) {
+            const uint32_t c2 = (uint32_t)((bb[0] >> 3) + (bb[1] >> 3) * A.cells[0] + (bb[2] >> 3) * A.cells[0] * A.cells[1]);
+            v = free_cell_bound(A, c2, bb[0], bb[1], bb[2]);
+        } else {
+            v = cur[(uint64_t)r2 * 512u + (uint32_t)((bb[0] & 7) + (bb[1] & 7) * 8 + (bb[2] & 7) * 64)];
+        }
+        tile[i] = v;
+    }
+    __syncthreads();
+    const uint32_t x = t & 7u, y = (t >> 3) & 7u, z = t >> 6;
+    const uint32_t c = (z + 1u) * 100u + (y + 1u) * 10u + (x + 1u);
+    if (tile[c] == 0u) {
+        nxt[item * 512u + t] = 0u;
         return;
     }
-    const uint32_t cell = A.row_cell[item];
-    const int64_t bx = (int64_t)(cell % A.cells[0]) * 8 + (t & 7u);
-    const int64_t by = (int64_t)((cell / A.cells[0]) % A.cells[1]) * 8 + ((t >> 3) & 7u);
-    const int64_t bz = (int64_t)(cell / (A.cells[0] * A.cells[1])) * 8 + (t >> 6);
-    uint32_t nbv[27];
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int k = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                const int64_t x = bx + dx, y = by + dy, z = bz + dz;
-                if (k == 13) {
-                    nbv[k] = 0;
-                    continue;
-                }
-                if (x < 0 || y < 0 || z < 0 || x >= A.nb[0] || y >= A.nb[1] || z >= A.nb[2]) {
-                    nbv[k] = 0xFFFFFFFFu;
-                    continue;
-                }
-                const uint32_t c2 = (uint32_t)((x >> 3) + (y >> 3) * A.cells[0] + (z >> 3) * A.cells[0] * A.cells[1]);
-                const uint32_t lb = (uint32_t)((x & 7) + (y & 7) * 8 + (z & 7) * 64);
-                if (c2 == cell) {
-                    nbv[k] = cur[item * 512u + lb];
-                } else {
-                    const uint32_t r2 = A.free_idx[c2];
-                    nbv[k] = r2 != kFreeCell ? cur[(uint64_t)r2 * 512u + lb] : free_cell_bound(A, c2, x, y, z);
-                }
-            }
     uint32_t out = 0;
     for (int o = 0; o < 8; o++) {
         const int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
         uint32_t m = kOctCap;
         for (int k = 1; k < 8; k++) {
             const int ox = (k & 1) * sx, oy = ((k >> 1) & 1) * sy, oz = (k >> 2) * sz;
-            const uint32_t v = (nbv[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] >> (4 * o)) & 15u;
+            const uint32_t v = (tile[(int)c + ox + 10 * oy + 100 * oz] >> (4 * o)) & 15u;
             m = v < m ? v : m;
         }
         const uint32_t e = m + 1u < kOctCap ? m + 1u : kOctCap;
         out |= e << (4 * o);
     }
-    nxt[self] = out;
+    nxt[item * 512u + t] = out;
 }
 
 // Row word per brick: packed octant extents << 32 | L1 slot (EMPTY in rows
