@@ -98,7 +98,7 @@ RKF_DEV float terrain_vnoise(uint32_t seed, float u, float v) {
     float b = c01 + (c11 - c01) * su;
     return a + (b - a) * sv;
 }
-RKF_DEV float terrain_sdf(const rkfm_prim &pr, float x, float y, float z) {
+RKF_DEV float terrain_height(const rkfm_prim &pr, float x, float z) {
     float u = x - pr.centre[0], w = z - pr.centre[2];
     float h = 0.0f, amp = 1.0f, freq = pr.params[1];
     for (uint32_t i = 0; i < 6u; i++) {
@@ -106,7 +106,13 @@ RKF_DEV float terrain_sdf(const rkfm_prim &pr, float x, float y, float z) {
         amp = amp * 0.5f;
         freq = freq * 2.0f;
     }
+    return h;
+}
+RKF_DEV float terrain_from_height(const rkfm_prim &pr, float y, float h) {
     return pr.params[2] * ((y - pr.centre[1]) - pr.params[0] * h);
+}
+RKF_DEV float terrain_sdf(const rkfm_prim &pr, float x, float y, float z) {
+    return terrain_from_height(pr, y, terrain_height(pr, x, z));
 }
 
 RKF_DEV float prim_eval(const rkfm_prim &pr, float x, float y, float z) {
@@ -139,6 +145,26 @@ RKF_DEV float scene_field(const BuildDev &B, uint32_t cand, float x, float y, fl
         float r = prim_eval(B.prims[B.group_prims[s]], x, y, z);
         for (uint32_t j = 1; j < n; j++) {
             float dj = prim_eval(B.prims[B.group_prims[s + j]], x, y, z);
+            r = B.k > 0.0f ? smin(r, dj, B.k) : fminf(r, dj);
+        }
+        d = fminf(d, r);
+    }
+    return d;
+}
+
+// scene_field with one terrain primitive's height given (prim index tp,
+// height th at this point's column): the same operations as prim_eval.
+RKF_DEV float scene_field_th(const BuildDev &B, uint32_t cand, float x, float y, float z, uint32_t tp, float th) {
+    float d = INFINITY;
+    uint32_t g0 = B.cand_off[cand], g1 = B.cand_off[cand + 1];
+    for (uint32_t gi = g0; gi < g1; gi++) {
+        uint32_t g = B.cand_groups[gi];
+        uint32_t s = B.group_start[g], n = B.group_count[g];
+        uint32_t pi = B.group_prims[s];
+        float r = pi == tp ? terrain_from_height(B.prims[pi], y, th) : prim_eval(B.prims[pi], x, y, z);
+        for (uint32_t j = 1; j < n; j++) {
+            pi = B.group_prims[s + j];
+            float dj = pi == tp ? terrain_from_height(B.prims[pi], y, th) : prim_eval(B.prims[pi], x, y, z);
             r = B.k > 0.0f ? smin(r, dj, B.k) : fminf(r, dj);
         }
         d = fminf(d, r);
@@ -200,9 +226,14 @@ __global__ __launch_bounds__(512) void assign_kernel(BuildDev B, uint32_t ncand,
     if (t == 0) l2[B.cand_cell[cand]] = blk;
 }
 
-// One block per allocated brick, one lane per voxel (populate_brick).
+// One block per allocated brick, one lane per voxel (populate_brick). The
+// first terrain primitive of the cell's groups has its height computed once
+// per voxel column (64 per brick) and shared through LDS: the height does not
+// depend on y.
 __global__ __launch_bounds__(512) void fill_kernel(BuildDev B, uint64_t nbricks, const uint32_t *slot_cand,
                                                    const uint32_t *slot_brick, uint16_t *dist, uint8_t *mat) {
+    __shared__ float hcol[64];
+    __shared__ uint32_t tprim;
     uint64_t item = item_index();
     if (item >= nbricks) return;
     uint32_t slot = (uint32_t)item, v = threadIdx.x;
@@ -215,7 +246,30 @@ __global__ __launch_bounds__(512) void fill_kernel(BuildDev B, uint64_t nbricks,
     float px = (B.origin[0] + (float)gbx * bws) + ((float)vx * vs + half);
     float py = (B.origin[1] + (float)gby * bws) + ((float)vy * vs + half);
     float pz = (B.origin[2] + (float)gbz * bws) + ((float)vz * vs + half);
-    float d = scene_field(B, cand, px, py, pz);
+    if (v == 0u) {
+        uint32_t tp = ~0u;
+        for (uint32_t gi = B.cand_off[cand]; gi < B.cand_off[cand + 1] && tp == ~0u; gi++) {
+            const uint32_t g = B.cand_groups[gi];
+            for (uint32_t j = 0; j < B.group_count[g]; j++) {
+                const uint32_t pi = B.group_prims[B.group_start[g] + j];
+                if (B.prims[pi].type == 3u) {
+                    tp = pi;
+                    break;
+                }
+            }
+        }
+        tprim = tp;
+    }
+    __syncthreads();
+    const uint32_t tp = tprim;
+    if (tp != ~0u && v < 64u) {
+        // column (v & 7, v >> 3): the x and z of its voxels, as above
+        const float cxw = (B.origin[0] + (float)gbx * bws) + ((float)(v & 7u) * vs + half);
+        const float czw = (B.origin[2] + (float)gbz * bws) + ((float)(v >> 3) * vs + half);
+        hcol[v] = terrain_height(B.prims[tp], cxw, czw);
+    }
+    __syncthreads();
+    float d = tp != ~0u ? scene_field_th(B, cand, px, py, pz, tp, hcol[vx + 8u * vz]) : scene_field(B, cand, px, py, pz);
     uint64_t idx = (uint64_t)slot * 512u + v;
     dist[idx] = (uint16_t)f2h(d);
     uint32_t m = 0;
