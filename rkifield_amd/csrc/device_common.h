@@ -26,6 +26,11 @@ constexpr float kCoarseNear = 0.5f;         // ray_march.wgsl:863
 struct f3 {
     float x, y, z;
 };
+// Marks a loaded value as used here: every load before it is issued before
+// any code after it, so a group of independent loads waits once instead of
+// being sunk one by one into the branches that consume them.
+RKF_DEV void loaded(uint32_t &v) { asm volatile("" : "+v"(v)); }
+
 RKF_DEV f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 RKF_DEV f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 RKF_DEV f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }

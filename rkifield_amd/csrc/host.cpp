@@ -365,6 +365,10 @@ int rkfm_dispatch(rkfm_ctx *c, const rkfm_gbuffer *out, void *stream) {
             return rkf_set_error(RKFM_ERR_LAYOUT, "dispatch: tile grid %ux%u does not match %ux%u pixels",
                                  c->tiles_x, c->tiles_y, out->width, out->height);
     }
+    // the kernel reads entry 0 of both for corners it does not use
+    rc = c->pool.reserve(64, true);
+    if (!rc) rc = c->maps.reserve(64, true);
+    if (rc) return rc;
     P.pool = (const rkf_voxel_sample *)c->pool.ptr;
     P.n_pool_voxels = c->n_pool_voxels;
     P.maps = (const uint32_t *)c->maps.ptr;

@@ -29,22 +29,6 @@ RKF_DEV uint32_t map_entry(const RefParams &P, uint64_t i) { return i < P.n_maps
 RKF_DEV uint32_t pool_w0(const RefParams &P, uint64_t i) { return i < P.n_pool_voxels ? P.pool[i].word0 : 0u; }
 RKF_DEV uint32_t pool_w1(const RefParams &P, uint64_t i) { return i < P.n_pool_voxels ? P.pool[i].word1 : 0u; }
 
-// ray_march.wgsl:270-288 (grad = false) and :300-315 (grad = true).
-RKF_DEV float voxel_at(const RefParams &P, uint32_t off, int32_t cx, int32_t cy, int32_t cz, const uint32_t *dims,
-                       const int32_t *tot, float vs, bool grad) {
-    cx = clampi(cx, 0, tot[0] - 1);
-    cy = clampi(cy, 0, tot[1] - 1);
-    cz = clampi(cz, 0, tot[2] - 1);
-    uint32_t bx = (uint32_t)(cx / 8), by = (uint32_t)(cy / 8), bz = (uint32_t)(cz / 8);
-    uint32_t lx = (uint32_t)(cx % 8), ly = (uint32_t)(cy % 8), lz = (uint32_t)(cz % 8);
-    uint32_t flat = bx + by * dims[0] + bz * dims[0] * dims[1];
-    uint32_t slot = map_entry(P, (uint64_t)off + flat);
-    if (slot == RKF_EMPTY_SLOT) return vs * 8.0f;
-    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * 8.0f) : -(vs * 2.0f);
-    uint64_t idx = (uint64_t)slot * 512u + lx + ly * 8u + lz * 64u;
-    return h2f(pool_w0(P, idx));
-}
-
 // ray_march.wgsl:353-413 / :463-498.
 RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool grad) {
     float vs = o.voxel_size;
@@ -72,14 +56,47 @@ RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool g
     uint32_t off = o.brick_map_offset;
     const RefParams &P = *c.P;
     if (!grad) c.evals++;
-    float c000 = voxel_at(P, off, x0, y0, z0, dims, tot, vs, grad);
-    float c100 = voxel_at(P, off, x0 + 1, y0, z0, dims, tot, vs, grad);
-    float c010 = voxel_at(P, off, x0, y0 + 1, z0, dims, tot, vs, grad);
-    float c110 = voxel_at(P, off, x0 + 1, y0 + 1, z0, dims, tot, vs, grad);
-    float c001 = voxel_at(P, off, x0, y0, z0 + 1, dims, tot, vs, grad);
-    float c101 = voxel_at(P, off, x0 + 1, y0, z0 + 1, dims, tot, vs, grad);
-    float c011 = voxel_at(P, off, x0, y0 + 1, z0 + 1, dims, tot, vs, grad);
-    float c111 = voxel_at(P, off, x0 + 1, y0 + 1, z0 + 1, dims, tot, vs, grad);
+    // sample_voxel_at (:270-288; gradient variant :300-315) for the 8
+    // corners: clamped to the grid, EMPTY -> +8 vs, INTERIOR -> -2 vs (-8 vs
+    // for gradients), else the pool word's binary16. Two rounds of eight
+    // independent loads (map entries, then pool words), so each round waits
+    // once. Unused loads read entry 0 (the dispatch keeps both buffers
+    // non-empty).
+    const int32_t X[2] = {clampi(x0, 0, tot[0] - 1), clampi(x0 + 1, 0, tot[0] - 1)};
+    const int32_t Y[2] = {clampi(y0, 0, tot[1] - 1), clampi(y0 + 1, 0, tot[1] - 1)};
+    const int32_t Z[2] = {clampi(z0, 0, tot[2] - 1), clampi(z0 + 1, 0, tot[2] - 1)};
+    uint32_t ent[8], raw[8];
+    bool inm[8], ok[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const uint32_t cx = (uint32_t)X[k & 1u], cy = (uint32_t)Y[(k >> 1) & 1u], cz = (uint32_t)Z[k >> 2];
+        const uint64_t i = (uint64_t)off + ((cx >> 3) + (cy >> 3) * dims[0] + (cz >> 3) * dims[0] * dims[1]);
+        inm[k] = i < P.n_maps;
+        ent[k] = P.maps[inm[k] ? i : 0u];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        loaded(ent[k]);
+        ent[k] = inm[k] ? ent[k] : RKF_EMPTY_SLOT;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const uint32_t cx = (uint32_t)X[k & 1u], cy = (uint32_t)Y[(k >> 1) & 1u], cz = (uint32_t)Z[k >> 2];
+        const uint64_t idx = (uint64_t)ent[k] * 512u + (cx & 7u) + (cy & 7u) * 8u + (cz & 7u) * 64u;
+        ok[k] = ent[k] < RKF_INTERIOR_SLOT && idx < P.n_pool_voxels;
+        raw[k] = P.pool[ok[k] ? idx : 0u].word0;
+    }
+    float cv[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        loaded(raw[k]);
+        // pool_w0: words past the pool read 0
+        cv[k] = ent[k] == RKF_EMPTY_SLOT      ? vs * 8.0f
+                : ent[k] == RKF_INTERIOR_SLOT ? (grad ? -(vs * 8.0f) : -(vs * 2.0f))
+                                              : h2f((uint16_t)(ok[k] ? raw[k] : 0u));
+    }
+    const float c000 = cv[0], c100 = cv[1], c010 = cv[2], c110 = cv[3];
+    const float c001 = cv[4], c101 = cv[5], c011 = cv[6], c111 = cv[7];
     float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
     float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
     float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);

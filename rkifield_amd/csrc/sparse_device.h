@@ -40,10 +40,6 @@ struct SpCtx {
     uint32_t evals;
 };
 
-// Marks a loaded value as used here: every load before it is issued before
-// any code after it, so a group of independent loads waits once instead of
-// being sunk one by one into the branches that consume them.
-RKF_DEV void loaded(uint32_t &v) { asm volatile("" : "+v"(v)); }
 
 RKF_DEV uint32_t sp_slot(const SparseDev &S, int32_t cx, int32_t cy, int32_t cz) {
     uint32_t bx = (uint32_t)cx >> 3, by = (uint32_t)cy >> 3, bz = (uint32_t)cz >> 3;
