@@ -395,30 +395,80 @@ RKF_DEV uint64_t nb_key(const SparseDev &S, const int32_t v0[3]) {
 
 enum : int { kLocFree = 0, kLocCached = 1, kLocEdge = 2 };
 
+// Low dword of a row word: the brick's L1 slot.
+RKF_DEV const uint32_t *roww_slots(const SparseDev &S) { return reinterpret_cast<const uint32_t *>(S.roww); }
+
+// Row of the cell containing corner brick k of a neighbourhood whose lower
+// brick lies on the cell's +x / +y / +z face when fx / fy / fz: rows[0] is
+// the cell's own, rows[j] the neighbour at offset (j & 1, j >> 1 & 1, j >> 2).
+// Selects only, so the rows stay in registers.
+RKF_DEV uint32_t nb_row_of(const uint32_t rows[8], uint32_t k, bool fx, bool fy, bool fz) {
+    const bool ox = (k & 1u) && fx, oy = ((k >> 1) & 1u) && fy, oz = (k >> 2) && fz;
+    const uint32_t a0 = ox ? rows[1] : rows[0], a1 = ox ? rows[3] : rows[2];
+    const uint32_t a2 = ox ? rows[5] : rows[4], a3 = ox ? rows[7] : rows[6];
+    const uint32_t b0 = oy ? a1 : a0, b1 = oy ? a3 : a2;
+    return oz ? b1 : b0;
+}
+
 // Free-space test + exit parameter (see file header), with the brick
 // neighbourhood cache. A point whose brick b0 is the cached one samples at
-// once (a cached brick is never free). Otherwise one load of the cell word
-// decides cell-level skips, then the brick's extent brick-level skips; a
-// brick that is not free refills the cache with the slots of its 2x2x2
-// neighbourhood. Near the grid faces the point takes the clamped sampling
-// path (kLocEdge).
+// once (a cached brick is never free). Otherwise the cell word decides
+// cell-level skips (one load when the cell is not the ray's cached one,
+// together with the L2 entries of the seven cells after it on +x, +y, +z),
+// then the brick's row word brick-level skips; a brick that is not free
+// refills the cache with the slots of its 2x2x2 neighbourhood, read from
+// the row words of its own cell and of the neighbouring cells in one round.
+// Near the grid faces the point takes the clamped sampling path (kLocEdge).
+//
+// The neighbour rows (L1 block, or EMPTY when the cell holds none) are kept
+// per lane in LDS, tagged with their cell: they depend on the cell only, so
+// any ray of the lane may reuse them. Kernels that call this run 64-lane
+// workgroups.
 RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, BrickNbhd &n, float g[3],
                       int32_t v0[3]) {
+    __shared__ uint32_t lds_cell[64];
+    __shared__ uint32_t lds_rows[7][64];
     if (!sp_grid_pos(S, pos, g, v0)) return kLocEdge;
     const uint64_t key = nb_key(S, v0);
     if (key == n.key) return kLocCached;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
     const uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
     const uint64_t cell = key >> 9;
+    const uint64_t cxy = (uint64_t)S.cells[0] * S.cells[1];
     float lo_b[3], hi_b[3];
     const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
     uint32_t e = 0u;
+    uint32_t rows[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; j++) rows[j] = RKF_EMPTY_SLOT;
+    bool have = false;
     if ((uint32_t)cell != n.cell) {
         const uint64_t cw = S.cellw[cell];
-        e = ((uint32_t)cw >> (4u * octant)) & 15u;
+        uint32_t nl[8];
+        bool ing[8];
+#pragma unroll
+        for (uint32_t j = 1; j < 8u; j++) {
+            const uint32_t dx = j & 1u, dy = (j >> 1) & 1u, dz = j >> 2;
+            ing[j] = q[0] + dx < S.cells[0] && q[1] + dy < S.cells[1] && q[2] + dz < S.cells[2];
+            nl[j] = S.l2[ing[j] ? cell + dx + (uint64_t)dy * S.cells[0] + (uint64_t)dz * cxy : 0u];
+        }
+        uint32_t cw_lo = (uint32_t)cw, cw_hi = (uint32_t)(cw >> 32);
+        loaded(cw_lo);
+        loaded(cw_hi);
+#pragma unroll
+        for (uint32_t j = 1; j < 8u; j++) loaded(nl[j]);
+        e = (cw_lo >> (4u * octant)) & 15u;
         if (e == 0) {
             n.cell = (uint32_t)cell;
-            n.row = (uint32_t)(cw >> 32);
+            n.row = cw_hi;
+            lds_cell[lane] = (uint32_t)cell;
+#pragma unroll
+            for (uint32_t j = 1; j < 8u; j++) {
+                rows[j] = ing[j] ? nl[j] : RKF_EMPTY_SLOT;
+                lds_rows[j - 1u][lane] = rows[j];
+            }
+            have = true;
         }
     }
     if (e > 0) {
@@ -430,60 +480,57 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
         }
     } else {
         // The brick's row word (extent and slot) and, side by side, the
-        // slots of the rest of its 2x2x2 neighbourhood that lie in the same
-        // cell, without branches so the loads go out together; for a brick
-        // that is not free, bricks of neighbouring cells follow through
-        // their L2 and L1 entries (only when some lane of the wave needs
-        // them). Unused loads read entry 0 (the arrays always hold one);
+        // slots of the rest of its 2x2x2 neighbourhood from the row words of
+        // the cells they lie in, without branches so the loads go out
+        // together. Unused loads read entry 0 (the arrays always hold one);
         // bricks past the grid end count EMPTY.
-        const uint32_t row = n.row;
-        const uint32_t lb = (uint32_t)(key & 511u);
-        uint64_t rw = S.roww[(uint64_t)row * 512u + lb];
-        uint32_t va[8], vc[8], vb[8], lbk[8];
-        bool inb[8], same[8], far[8];
+        const bool fx = (b[0] & 7u) == 7u, fy = (b[1] & 7u) == 7u, fz = (b[2] & 7u) == 7u;
+        if (!have && (fx || fy || fz)) {
+            if (lds_cell[lane] == (uint32_t)cell) {
 #pragma unroll
-        for (uint32_t k = 0; k < 8u; k++) {
+                for (uint32_t j = 1; j < 8u; j++) rows[j] = lds_rows[j - 1u][lane];
+            } else {
+                uint32_t nl[8];
+                bool ing[8];
+#pragma unroll
+                for (uint32_t j = 1; j < 8u; j++) {
+                    const uint32_t dx = j & 1u, dy = (j >> 1) & 1u, dz = j >> 2;
+                    ing[j] = q[0] + dx < S.cells[0] && q[1] + dy < S.cells[1] && q[2] + dz < S.cells[2];
+                    nl[j] = S.l2[ing[j] ? cell + dx + (uint64_t)dy * S.cells[0] + (uint64_t)dz * cxy : 0u];
+                }
+                lds_cell[lane] = (uint32_t)cell;
+#pragma unroll
+                for (uint32_t j = 1; j < 8u; j++) {
+                    loaded(nl[j]);
+                    rows[j] = ing[j] ? nl[j] : RKF_EMPTY_SLOT;
+                    lds_rows[j - 1u][lane] = rows[j];
+                }
+            }
+        }
+        rows[0] = n.row;
+        const uint32_t lb = (uint32_t)(key & 511u);
+        const uint64_t rw = S.roww[(uint64_t)rows[0] * 512u + lb];
+        uint32_t va[8];
+        bool ok[8];
+#pragma unroll
+        for (uint32_t k = 1; k < 8u; k++) {
             const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
-            inb[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2];
-            same[k] = (bx >> 3) == q[0] && (by >> 3) == q[1] && (bz >> 3) == q[2];
-            lbk[k] = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
-            va[k] = k == 0u ? 0u : (uint32_t)S.roww[inb[k] && same[k] ? (uint64_t)row * 512u + lbk[k] : 0u];
-            vc[k] = RKF_EMPTY_SLOT;
+            const uint32_t rk = nb_row_of(rows, k, fx, fy, fz);
+            ok[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2] && rk != RKF_EMPTY_SLOT;
+            const uint32_t lbk = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+            va[k] = roww_slots(S)[ok[k] ? ((uint64_t)rk * 512u + lbk) * 2u : 0u];
         }
         uint32_t rw_lo = (uint32_t)rw, rw_hi = (uint32_t)(rw >> 32);
         loaded(rw_lo);
         loaded(rw_hi);
 #pragma unroll
         for (uint32_t k = 1; k < 8u; k++) loaded(va[k]);
-        va[0] = rw_lo;
         const uint32_t eb = (rw_hi >> (4u * octant)) & 15u;
         if (eb == 0) {
-            const bool cross = (b[0] & 7u) == 7u || (b[1] & 7u) == 7u || (b[2] & 7u) == 7u;
-            if (__any(cross)) {
-#pragma unroll
-                for (uint32_t k = 0; k < 8u; k++) {
-                    const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
-                    const uint64_t ck = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
-                                        (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
-                    vc[k] = S.l2[inb[k] && !same[k] ? ck : 0u];
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < 8u; k++) {
-                loaded(vc[k]);
-                far[k] = inb[k] && !same[k] && vc[k] != RKF_EMPTY_SLOT;
-                vb[k] = RKF_EMPTY_SLOT;
-            }
-            if (__any(far[0] || far[1] || far[2] || far[3] || far[4] || far[5] || far[6] || far[7])) {
-#pragma unroll
-                for (uint32_t k = 0; k < 8u; k++) vb[k] = S.l1[far[k] ? (uint64_t)vc[k] * 512u + lbk[k] : 0u];
-            }
             n.key = key;
+            n.slot[0] = rw_lo;
 #pragma unroll
-            for (uint32_t k = 0; k < 8u; k++) {
-                loaded(vb[k]);
-                n.slot[k] = !inb[k] ? RKF_EMPTY_SLOT : same[k] ? va[k] : (far[k] ? vb[k] : RKF_EMPTY_SLOT);
-            }
+            for (uint32_t k = 1; k < 8u; k++) n.slot[k] = ok[k] ? va[k] : RKF_EMPTY_SLOT;
             return kLocCached;
         }
         // the cube of bricks b + [0, eb) in the travel direction of each axis is free
