@@ -318,50 +318,39 @@ RKF_DEV uint32_t nb_pick(const BrickNbhd &n, bool ox, bool oy, bool oz) {
     return oz ? b1 : b0;
 }
 
-// Corner load address: the voxel's distance word for a live slot, else the
-// pool's first word (the pool always holds one), so the load is issued for
-// every corner.
-RKF_DEV const uint16_t *nb_addr(const SparseDev &S, uint32_t slot, int32_t cx, int32_t cy, int32_t cz) {
-    const uint64_t i = slot < RKF_INTERIOR_SLOT ? (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u +
-                                                      (uint32_t)(cz & 7) * 64u
-                                                : 0u;
-    return S.dist + i;
-}
-
-RKF_DEV float nb_value(uint32_t slot, uint32_t raw, float vs) {
-    return slot < RKF_INTERIOR_SLOT ? h2f((uint16_t)raw) : (slot == RKF_EMPTY_SLOT ? vs * 8.0f : -(vs * 2.0f));
-}
-
 // sp_sample (stepping semantics) for an in-grid point whose lower corner
 // voxel v0 (= floor(g)) lies in the cached brick: same corner values, same
-// arithmetic (the outside-grid distance is +0 here).
+// arithmetic (the outside-grid distance is +0 here). Branch-free: every
+// corner's load is issued (a sentinel slot reads the pool's first word, the
+// pool always holds one) and the sentinel values are selected afterwards.
 RKF_DEV float nb_sample(const SparseDev &S, const BrickNbhd &n, const float g[3], const int32_t v0[3]) {
     const float vs = S.vs;
+    const float v_empty = vs * 8.0f, v_interior = -(vs * 2.0f);
     const int32_t x0 = v0[0], y0 = v0[1], z0 = v0[2];
     const float tx = g[0] - (float)x0, ty = g[1] - (float)y0, tz = g[2] - (float)z0;
     const bool ox = (x0 & 7) == 7, oy = (y0 & 7) == 7, oz = (z0 & 7) == 7;
-    const uint32_t s000 = nb_pick(n, false, false, false), s100 = nb_pick(n, ox, false, false);
-    const uint32_t s010 = nb_pick(n, false, oy, false), s110 = nb_pick(n, ox, oy, false);
-    const uint32_t s001 = nb_pick(n, false, false, oz), s101 = nb_pick(n, ox, false, oz);
-    const uint32_t s011 = nb_pick(n, false, oy, oz), s111 = nb_pick(n, ox, oy, oz);
-    uint32_t r000 = *nb_addr(S, s000, x0, y0, z0), r100 = *nb_addr(S, s100, x0 + 1, y0, z0);
-    uint32_t r010 = *nb_addr(S, s010, x0, y0 + 1, z0), r110 = *nb_addr(S, s110, x0 + 1, y0 + 1, z0);
-    uint32_t r001 = *nb_addr(S, s001, x0, y0, z0 + 1), r101 = *nb_addr(S, s101, x0 + 1, y0, z0 + 1);
-    uint32_t r011 = *nb_addr(S, s011, x0, y0 + 1, z0 + 1), r111 = *nb_addr(S, s111, x0 + 1, y0 + 1, z0 + 1);
-    loaded(r000);
-    loaded(r100);
-    loaded(r010);
-    loaded(r110);
-    loaded(r001);
-    loaded(r101);
-    loaded(r011);
-    loaded(r111);
-    const float c000 = nb_value(s000, r000, vs), c100 = nb_value(s100, r100, vs);
-    const float c010 = nb_value(s010, r010, vs), c110 = nb_value(s110, r110, vs);
-    const float c001 = nb_value(s001, r001, vs), c101 = nb_value(s101, r101, vs);
-    const float c011 = nb_value(s011, r011, vs), c111 = nb_value(s111, r111, vs);
-    const float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
-    const float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
+    const uint32_t lx[2] = {(uint32_t)(x0 & 7), (uint32_t)((x0 + 1) & 7)};
+    const uint32_t ly[2] = {(uint32_t)(y0 & 7) * 8u, (uint32_t)((y0 + 1) & 7) * 8u};
+    const uint32_t lz[2] = {(uint32_t)(z0 & 7) * 64u, (uint32_t)((z0 + 1) & 7) * 64u};
+    uint32_t sl[8], raw[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
+        sl[k] = nb_pick(n, kx && ox, ky && oy, kz && oz);
+        const bool live = sl[k] < RKF_INTERIOR_SLOT;
+        const uint32_t loc = lx[kx] + ly[ky] + lz[kz];
+        raw[k] = S.dist[(uint64_t)(live ? sl[k] : 0u) * 512u + (live ? loc : 0u)];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) loaded(raw[k]);
+    float c[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const uint32_t sent = __float_as_uint(sl[k] == RKF_EMPTY_SLOT ? v_empty : v_interior);
+        c[k] = __uint_as_float(sl[k] < RKF_INTERIOR_SLOT ? __float_as_uint(h2f(raw[k])) : sent);
+    }
+    const float c00 = mix(c[0], c[1], tx), c10 = mix(c[2], c[3], tx);
+    const float c01 = mix(c[4], c[5], tx), c11 = mix(c[6], c[7], tx);
     const float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
     return mix(c0, c1, tz) + 0.0f;
 }
