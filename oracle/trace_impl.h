@@ -515,11 +515,15 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
 /* ================================================================== */
 /*
  * Field: identical to sample_voxelized for one object whose dense brick map is
- * the expansion of the index (grid_pos = p - origin, no geometry early-out).
+ * the expansion of the index (grid_pos = p - origin, no geometry early-out),
+ * except that grid coordinates multiply by 1/vs (rounded once to the working
+ * precision) where the WGSL divides by vs: WGSL's f32 division is only
+ * specified to 2.5 ULP, and a multiply costs one instruction on the GPU
+ * instead of a correctly rounded division's dozen.
  * L2 cell = 8^3 bricks; l2[cell] = L1 block or EMPTY; l1[block*512 + brick in
  * cell] = slot / EMPTY / INTERIOR; dist[slot*512 + voxel] binary16.
  *
- * Free-space DDA: let g = (p - origin)/vs - 0.5 (the trilinear coordinate) and
+ * Free-space DDA: let g = (p - origin)*(1/vs) - 0.5 (the trilinear coordinate) and
  * b0 = floor(floor(g)/8). A sample's 8 corners lie in bricks {b0, b0+1}^3, so
  * if all of those are EMPTY (b0 "free") the field is the EMPTY constant
  * 8 vs over the whole region floor(g/8) == b0; if every brick of an L2 cell is
@@ -573,7 +577,8 @@ static R F(sp_sample)(F(sctx) *c, V3 p, int grad, int count) {
     V3 cl = F(mk)(F(clampr)(gp.x, lo, gs.x - lo), F(clampr)(gp.y, lo, gs.y - lo), F(clampr)(gp.z, lo, gs.z - lo));
     R outside = F(length)(F(sub)(gp, cl));
     if (outside > be * (R)2.0f) return outside;
-    V3 vc = F(mk)(cl.x / vs - (R)0.5f, cl.y / vs - (R)0.5f, cl.z / vs - (R)0.5f);
+    R ivs = (R)1.0f / vs;
+    V3 vc = F(mk)(cl.x * ivs - (R)0.5f, cl.y * ivs - (R)0.5f, cl.z * ivs - (R)0.5f);
     int32_t x0 = (int32_t)FLOOR(vc.x), y0 = (int32_t)FLOOR(vc.y), z0 = (int32_t)FLOOR(vc.z);
     R tx = vc.x - (R)x0, ty = vc.y - (R)y0, tz = vc.z - (R)z0;
     if (count) c->evals++;
@@ -598,9 +603,10 @@ static int64_t F(sp_nearest)(const F(sctx) *c, V3 p, uint32_t *slot_o, uint32_t 
     *vox_o = 0xFFFFFFFFu;
     *cell_o = 0xFFFFFFFFu;
     if (gp.x < (R)0 || gp.y < (R)0 || gp.z < (R)0 || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) return -1;
-    int32_t cx = F(clampi)((int32_t)FLOOR(gp.x / vs), 0, c->tot[0] - 1);
-    int32_t cy = F(clampi)((int32_t)FLOOR(gp.y / vs), 0, c->tot[1] - 1);
-    int32_t cz = F(clampi)((int32_t)FLOOR(gp.z / vs), 0, c->tot[2] - 1);
+    R ivs = (R)1.0f / vs;
+    int32_t cx = F(clampi)((int32_t)FLOOR(gp.x * ivs), 0, c->tot[0] - 1);
+    int32_t cy = F(clampi)((int32_t)FLOOR(gp.y * ivs), 0, c->tot[1] - 1);
+    int32_t cz = F(clampi)((int32_t)FLOOR(gp.z * ivs), 0, c->tot[2] - 1);
     uint32_t slot = F(sp_slot)(c, cx, cy, cz);
     *cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * s->cells[0] + ((uint32_t)cz >> 6) * s->cells[0] * s->cells[1];
     *slot_o = slot;
@@ -626,7 +632,8 @@ static int F(sp_free)(const F(sctx) *c, V3 pos, V3 sd, V3 inv, R t, R *t_exit) {
     V3 gp = F(sub)(pos, F(mk)((R)s->origin[0], (R)s->origin[1], (R)s->origin[2]));
     R lo = vs * (R)0.01f;
     if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return 0;
-    R g[3] = {gp.x / vs - (R)0.5f, gp.y / vs - (R)0.5f, gp.z / vs - (R)0.5f};
+    R ivs = (R)1.0f / vs;
+    R g[3] = {gp.x * ivs - (R)0.5f, gp.y * ivs - (R)0.5f, gp.z * ivs - (R)0.5f};
     int32_t v0[3];
     for (int a = 0; a < 3; a++) {
         v0[a] = (int32_t)FLOOR(g[a]);

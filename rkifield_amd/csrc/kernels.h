@@ -51,6 +51,7 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFDu;  // "slot not prefetched"
 struct SparseDev {
     float origin[3];
     float vs;
+    float inv_vs;     // 1 / vs rounded to float: g = (p - origin) * inv_vs - 0.5
     uint32_t cells[3];
     uint32_t nb[3];   // bricks per axis = cells * 8
     int32_t tot[3];   // voxels per axis = cells * 64

@@ -505,6 +505,7 @@ rkf::SparseDev rkfm_sparse_scene::dev() const {
         d.tot[a] = (int32_t)(info.cells[a] * 64u);
     }
     d.vs = info.voxel_size;
+    d.inv_vs = 1.0f / info.voxel_size;
     d.l2 = (const uint32_t *)l2.ptr;
     d.l1 = (const uint32_t *)l1.ptr;
     d.dist = (const uint16_t *)dist.ptr;

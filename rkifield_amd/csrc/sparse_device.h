@@ -3,7 +3,10 @@
 //
 // Field: the reference's sample_voxelized (ray_march.wgsl:353-413) for one
 // world-space object whose dense brick map is the expansion of the two-level
-// index: 8 corner voxels, EMPTY -> +8 vs, INTERIOR -> -2 vs (stepping) or
+// index, with grid coordinates multiplied by inv_vs (1/vs rounded to float)
+// where the WGSL divides by vs (its f32 division is specified to 2.5 ULP;
+// the multiply is one instruction, a correctly rounded division a dozen):
+// 8 corner voxels, EMPTY -> +8 vs, INTERIOR -> -2 vs (stepping) or
 // -8 vs (gradient, :300-315), trilinear mix x->y->z, plus the outside-grid
 // distance. Step rules: ray_march_tiled (:905-982): near-step budget
 // (d < 0.02), MIN_STEP, 8-step bisection on overstep, hit threshold
@@ -12,7 +15,7 @@
 // (shade_common_shading.wgsl:9-33) from shade_pbr.wgsl:40's origin.
 //
 // Multi-level free-space DDA (no reference counterpart): with
-// g = (p - origin)/vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
+// g = (p - origin) * inv_vs - 0.5 and b0 = floor(floor(g)/8), a sample's corners lie
 // in bricks {b0, b0+1}^3. If all of them are EMPTY the field is the constant
 // 8 vs over the region floor(g/8) == b0, and over the cube of free bricks
 // that has b0 at its corner and extends E_b bricks in the directions the ray
@@ -73,7 +76,7 @@ RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count, uint32_t pre = kN
     f3 cl = mk(clampf(gp.x, lo, gs.x - lo), clampf(gp.y, lo, gs.y - lo), clampf(gp.z, lo, gs.z - lo));
     float outside = length(sub(gp, cl));
     if (outside > be * 2.0f) return outside;
-    f3 vc = mk(cl.x / vs - 0.5f, cl.y / vs - 0.5f, cl.z / vs - 0.5f);
+    f3 vc = mk(cl.x * S.inv_vs - 0.5f, cl.y * S.inv_vs - 0.5f, cl.z * S.inv_vs - 0.5f);
     int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
     float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
     if (count) c.evals++;
@@ -137,7 +140,7 @@ RKF_DEV uint32_t sp_brick_slot(const SparseDev &S, uint32_t bx, uint32_t by, uin
 RKF_DEV void bc_init(const SparseDev &S, f3 p, int32_t reach, BrickCache &bc) {
     const float vs = S.vs;
     f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
-    float g[3] = {gp.x / vs - 0.5f, gp.y / vs - 0.5f, gp.z / vs - 0.5f};
+    float g[3] = {gp.x * S.inv_vs - 0.5f, gp.y * S.inv_vs - 0.5f, gp.z * S.inv_vs - 0.5f};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         // clampf keeps the float -> int conversion in range for far points
@@ -189,7 +192,7 @@ RKF_DEV float sp_sample_bc(const SparseDev &S, const BrickCache &bc, f3 p, bool 
     f3 cl = mk(clampf(gp.x, lo, gs.x - lo), clampf(gp.y, lo, gs.y - lo), clampf(gp.z, lo, gs.z - lo));
     float outside = length(sub(gp, cl));
     if (outside > be * 2.0f) return outside;
-    f3 vc = mk(cl.x / vs - 0.5f, cl.y / vs - 0.5f, cl.z / vs - 0.5f);
+    f3 vc = mk(cl.x * S.inv_vs - 0.5f, cl.y * S.inv_vs - 0.5f, cl.z * S.inv_vs - 0.5f);
     int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
     float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
     float c000 = bc_voxel(S, bc, x0, y0, z0, vs, grad), c100 = bc_voxel(S, bc, x0 + 1, y0, z0, vs, grad);
@@ -203,7 +206,7 @@ RKF_DEV float sp_sample_bc(const SparseDev &S, const BrickCache &bc, f3 p, bool 
     return mix(c0, c1, tz) + outside;
 }
 
-// Nearest voxel floor(grid_pos / vs) as ray_march.wgsl:540-596. Returns the
+// Nearest voxel floor(grid_pos * inv_vs) as ray_march.wgsl:540-596. Returns the
 // pool index or ~0.
 RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o) {
     float vs = S.vs;
@@ -214,9 +217,9 @@ RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t
     vox_o = 0xFFFFFFFFu;
     cell_o = 0xFFFFFFFFu;
     if (gp.x < 0.0f || gp.y < 0.0f || gp.z < 0.0f || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) return ~0ull;
-    int32_t cx = clampi((int32_t)floorf(gp.x / vs), 0, S.tot[0] - 1);
-    int32_t cy = clampi((int32_t)floorf(gp.y / vs), 0, S.tot[1] - 1);
-    int32_t cz = clampi((int32_t)floorf(gp.z / vs), 0, S.tot[2] - 1);
+    int32_t cx = clampi((int32_t)floorf(gp.x * S.inv_vs), 0, S.tot[0] - 1);
+    int32_t cy = clampi((int32_t)floorf(gp.y * S.inv_vs), 0, S.tot[1] - 1);
+    int32_t cz = clampi((int32_t)floorf(gp.z * S.inv_vs), 0, S.tot[2] - 1);
     uint32_t slot = sp_slot(S, cx, cy, cz);
     cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * S.cells[0] + ((uint32_t)cz >> 6) * S.cells[0] * S.cells[1];
     slot_o = slot;
@@ -244,7 +247,7 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
     f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
     float lo = vs * 0.01f;
     if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
-    float g[3] = {gp.x / vs - 0.5f, gp.y / vs - 0.5f, gp.z / vs - 0.5f};
+    float g[3] = {gp.x * S.inv_vs - 0.5f, gp.y * S.inv_vs - 0.5f, gp.z * S.inv_vs - 0.5f};
     int32_t v0[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -364,9 +367,9 @@ RKF_DEV bool sp_grid_pos(const SparseDev &S, f3 pos, float g[3], int32_t v0[3]) 
     const f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
     const float lo = vs * 0.01f;
     if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
-    g[0] = gp.x / vs - 0.5f;
-    g[1] = gp.y / vs - 0.5f;
-    g[2] = gp.z / vs - 0.5f;
+    g[0] = gp.x * S.inv_vs - 0.5f;
+    g[1] = gp.y * S.inv_vs - 0.5f;
+    g[2] = gp.z * S.inv_vs - 0.5f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         v0[a] = (int32_t)floorf(g[a]);
