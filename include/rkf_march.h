@@ -333,13 +333,15 @@ typedef struct rkfm_sparse_info {
     uint64_t device_bytes;  /* HBM held by the scene                       */
 } rkfm_sparse_info;
 
-/* Voxelise primitives on the GPU into a new resident sparse scene. */
+/* Voxelise primitives on the GPU into a new resident sparse scene (at most
+ * 32768 L2 cells per axis). */
 int rkfm_sparse_build(rkfm_ctx *ctx, const rkfm_prim *prims, uint32_t n_prims,
                       const rkfm_sparse_params *params,
                       rkfm_sparse_scene **out);
 /* Build a sparse scene from a reference dense brick map + AoS bricks (one
  * object, translation only: `origin` = world position of the map's grid
- * corner). Used to stream .rkf assets and fixtures into resident scenes. */
+ * corner; at most 262144 bricks per axis). Used to stream .rkf assets and
+ * fixtures into resident scenes. */
 int rkfm_sparse_from_dense(rkfm_ctx *ctx, const uint32_t *map,
                            const uint32_t dims[3],
                            const rkf_voxel_sample *bricks, uint64_t n_bricks,

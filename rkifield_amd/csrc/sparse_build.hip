@@ -920,6 +920,8 @@ int rkfm_sparse_from_dense(rkfm_ctx *c, const uint32_t *map, const uint32_t dims
     *out = nullptr;
     if (!(vs > 0.0f) || !dims[0] || !dims[1] || !dims[2])
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_from_dense: voxel size / dims");
+    if (dims[0] > 8u * 32768u || dims[1] > 8u * 32768u || dims[2] > 8u * 32768u)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_from_dense: grid too large");
     uint32_t cells[3] = {(dims[0] + 7u) / 8u, (dims[1] + 7u) / 8u, (dims[2] + 7u) / 8u};
     uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
     std::vector<uint32_t> l2(ncell, RKF_EMPTY_SLOT), l1, block_cells;

@@ -301,7 +301,8 @@ RKF_DEV bool sp_free(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &
 // corner voxel lies in brick b0 reads, kept per ray while its samples stay in
 // b0 (a brick that is not free): a step then needs only the distance loads.
 struct BrickNbhd {
-    uint64_t key;      // cell * 512 + (brick in cell) of b0; ~0 when empty
+    uint64_t key;      // b0's brick coordinates, 21 bits each (grids hold at most 2^18
+                       // bricks per axis); ~0 when empty
     uint32_t slot[8];  // brick b0 + (k & 1, k >> 1 & 1, k >> 2); constant indices only
     uint32_t cell;     // last cell met that is not free for this ray's octant, ~0 when none
     uint32_t row;      // its free-mask row
@@ -366,23 +367,24 @@ RKF_DEV bool sp_grid_pos(const SparseDev &S, f3 pos, float g[3], int32_t v0[3]) 
     const f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
     const f3 gp = sub(pos, mk(S.origin[0], S.origin[1], S.origin[2]));
     const float lo = vs * 0.01f;
-    if (gp.x < lo || gp.y < lo || gp.z < lo || gp.x > gs.x - lo || gp.y > gs.y - lo || gp.z > gs.z - lo) return false;
+    // one combined test (no early exits, so no nested branches)
+    bool in = !(gp.x < lo) & !(gp.y < lo) & !(gp.z < lo) & !(gp.x > gs.x - lo) & !(gp.y > gs.y - lo) &
+              !(gp.z > gs.z - lo);
     g[0] = gp.x * S.inv_vs - 0.5f;
     g[1] = gp.y * S.inv_vs - 0.5f;
     g[2] = gp.z * S.inv_vs - 0.5f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         v0[a] = (int32_t)floorf(g[a]);
-        if (v0[a] < 0 || v0[a] > S.tot[a] - 2) return false;
+        in = in & (v0[a] >= 0) & (v0[a] <= S.tot[a] - 2);
     }
-    return true;
+    return in;
 }
 
 RKF_DEV uint64_t nb_key(const SparseDev &S, const int32_t v0[3]) {
-    const uint32_t bx = (uint32_t)v0[0] >> 3, by = (uint32_t)v0[1] >> 3, bz = (uint32_t)v0[2] >> 3;
-    const uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
-                          (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
-    return cell * 512u + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+    // brick coordinates of v0 packed 21 bits each (shifts only, no multiply)
+    const uint64_t bx = (uint32_t)v0[0] >> 3, by = (uint32_t)v0[1] >> 3, bz = (uint32_t)v0[2] >> 3;
+    return bx | (by << 21) | (bz << 42);
 }
 
 enum : int { kLocFree = 0, kLocCached = 1, kLocEdge = 2 };
@@ -426,7 +428,7 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
     const uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
-    const uint64_t cell = key >> 9;
+    const uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
     const uint64_t cxy = (uint64_t)S.cells[0] * S.cells[1];
     float lo_b[3], hi_b[3];
     const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
@@ -500,7 +502,7 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
             }
         }
         rows[0] = n.row;
-        const uint32_t lb = (uint32_t)(key & 511u);
+        const uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
         const uint64_t rw = S.roww[(uint64_t)rows[0] * 512u + lb];
         uint32_t va[8];
         bool ok[8];
