@@ -562,7 +562,7 @@ struct TmpBufs {
 };
 
 int put(hipStream_t st, DevBuf &b, const void *src, uint64_t bytes) {
-    int rc = b.reserve(bytes ? bytes : 4, false);
+    int rc = b.reserve(std::max<uint64_t>(bytes, 1024), false);  // kernels may read the first KiB
     if (rc) return rc;
     b.size = bytes;
     if (bytes) rc = rkf_hip_check(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st), "upload");
@@ -873,7 +873,7 @@ int rkfm_sparse_build(rkfm_ctx *c, const rkfm_prim *prims, uint32_t n, const rkf
                                   (unsigned long long)need));
     rc = s->l2.reserve(ncell * 4, false);
     if (!rc) rc = s->l1.reserve((uint64_t)nblocks * 2048 + 4, false);
-    if (!rc) rc = s->dist.reserve(nbricks * 1024 + 4, false);
+    if (!rc) rc = s->dist.reserve(nbricks * 1024 + 1024, false);  // the first brick is read by unused loads
     if (!rc) rc = s->mat.reserve(nbricks * 512 + 4, false);
     if (!rc) rc = put(c->stream, *d_blk, blk_of.data(), (uint64_t)ncand * 4);
     if (!rc) rc = put(c->stream, *d_base, base.data(), (uint64_t)ncand * 4);

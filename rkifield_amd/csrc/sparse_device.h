@@ -324,9 +324,10 @@ RKF_DEV uint32_t nb_pick(const BrickNbhd &n, bool ox, bool oy, bool oz) {
 
 // sp_sample (stepping semantics) for an in-grid point whose lower corner
 // voxel v0 (= floor(g)) lies in the cached brick: same corner values, same
-// arithmetic (the outside-grid distance is +0 here). Branch-free: every
-// corner's load is issued (a sentinel slot reads the pool's first word, the
-// pool always holds one) and the sentinel values are selected afterwards.
+// arithmetic (the outside-grid distance is +0 here). Branch-free per lane:
+// every corner's load is issued (a sentinel slot reads the pool's first
+// brick, the pool always holds one) and the sentinel values are selected
+// afterwards.
 RKF_DEV float nb_sample(const SparseDev &S, const BrickNbhd &n, const float g[3], const int32_t v0[3]) {
     const float vs = S.vs;
     const float v_empty = vs * 8.0f, v_interior = -(vs * 2.0f);
@@ -337,13 +338,27 @@ RKF_DEV float nb_sample(const SparseDev &S, const BrickNbhd &n, const float g[3]
     const uint32_t ly[2] = {(uint32_t)(y0 & 7) * 8u, (uint32_t)((y0 + 1) & 7) * 8u};
     const uint32_t lz[2] = {(uint32_t)(z0 & 7) * 64u, (uint32_t)((z0 + 1) & 7) * 64u};
     uint32_t sl[8], raw[8];
+    if (!__any(ox || oy || oz)) {
+        // every active lane's corners lie in its brick b0: one address, the
+        // eight loads at fixed offsets from it
+        const uint32_t s0 = n.slot[0];
+        const bool live = s0 < RKF_INTERIOR_SLOT;
+        const uint16_t *pg = S.dist + (uint64_t)(live ? s0 : 0u) * 512u + (live ? lx[0] + ly[0] + lz[0] : 0u);
+        const uint32_t off[8] = {0u, 1u, 8u, 9u, 64u, 65u, 72u, 73u};
 #pragma unroll
-    for (uint32_t k = 0; k < 8u; k++) {
-        const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
-        sl[k] = nb_pick(n, kx && ox, ky && oy, kz && oz);
-        const bool live = sl[k] < RKF_INTERIOR_SLOT;
-        const uint32_t loc = lx[kx] + ly[ky] + lz[kz];
-        raw[k] = S.dist[(uint64_t)(live ? sl[k] : 0u) * 512u + (live ? loc : 0u)];
+        for (uint32_t k = 0; k < 8u; k++) {
+            sl[k] = s0;
+            raw[k] = pg[off[k]];
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; k++) {
+            const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
+            sl[k] = nb_pick(n, kx && ox, ky && oy, kz && oz);
+            const bool live = sl[k] < RKF_INTERIOR_SLOT;
+            const uint32_t loc = lx[kx] + ly[ky] + lz[kz];
+            raw[k] = S.dist[(uint64_t)(live ? sl[k] : 0u) * 512u + (live ? loc : 0u)];
+        }
     }
 #pragma unroll
     for (uint32_t k = 0; k < 8u; k++) loaded(raw[k]);
