@@ -518,16 +518,28 @@ RKF_DEV int sp_locate(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float 
         }
         rows[0] = n.row;
         const uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
-        const uint64_t rw = S.roww[(uint64_t)rows[0] * 512u + lb];
+        const uint64_t *pr = S.roww + (uint64_t)rows[0] * 512u + lb;
+        const uint64_t rw = pr[0];
         uint32_t va[8];
         bool ok[8];
+        if (!__any(fx || fy || fz)) {
+            // no active lane's neighbourhood leaves its cell: all eight row
+            // words at fixed offsets from the brick's own
+            const uint32_t dk[8] = {0u, 1u, 8u, 9u, 64u, 65u, 72u, 73u};
 #pragma unroll
-        for (uint32_t k = 1; k < 8u; k++) {
-            const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
-            const uint32_t rk = nb_row_of(rows, k, fx, fy, fz);
-            ok[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2] && rk != RKF_EMPTY_SLOT;
-            const uint32_t lbk = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
-            va[k] = roww_slots(S)[ok[k] ? ((uint64_t)rk * 512u + lbk) * 2u : 0u];
+            for (uint32_t k = 1; k < 8u; k++) {
+                ok[k] = true;
+                va[k] = *reinterpret_cast<const uint32_t *>(pr + dk[k]);
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 1; k < 8u; k++) {
+                const uint32_t bx = b[0] + (k & 1u), by = b[1] + ((k >> 1) & 1u), bz = b[2] + (k >> 2);
+                const uint32_t rk = nb_row_of(rows, k, fx, fy, fz);
+                ok[k] = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2] && rk != RKF_EMPTY_SLOT;
+                const uint32_t lbk = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+                va[k] = roww_slots(S)[ok[k] ? ((uint64_t)rk * 512u + lbk) * 2u : 0u];
+            }
         }
         uint32_t rw_lo = (uint32_t)rw, rw_hi = (uint32_t)(rw >> 32);
         loaded(rw_lo);
