@@ -189,8 +189,10 @@ def run_config3(args, ctx, M, L, scenes):
     admitted by a 4 GiB memory budget. The same trace is replayed by a native
     client against the batched server and against the per-request policy
     (one plain launch per request), first at its arrival times (latency), then
-    all at once (saturation throughput). Point results are copied to pinned
-    host memory; view G-buffers stay in HBM for their GPU consumer and are
+    all at once (saturation throughput). Both use the server's default shape
+    (4 rounds in flight, rounds capped at 2^20 rays + queries, a round of one
+    request on its plain kernel). Point results are copied to pinned host
+    memory; view G-buffers stay in HBM for their GPU consumer and are
     released on completion."""
     n_scenes = 8
     resident = []
@@ -232,8 +234,8 @@ def run_config3(args, ctx, M, L, scenes):
         f"per request, offered {offered:.1f} Mrays/s")
     results = {}
     for name, policy in (("served", L.SERVE_BATCHED), ("plain", L.SERVE_PER_REQUEST)):
-        srv = M.Server(ctx, resident, memory_budget=4 << 30, max_requests=ring_n, rounds_in_flight=2,
-                       policy=policy, max_pending=len(trace) + 16)
+        srv = M.Server(ctx, resident, memory_budget=4 << 30, max_requests=ring_n, policy=policy,
+                       max_pending=len(trace) + 16)
         srv.replay(reqs[:64], np.zeros(64))  # warm-up
         lat, wall = srv.replay(reqs, arrivals)
         st = srv.stats()

@@ -632,7 +632,9 @@ uint64_t rkfm_batch_work_items(const rkfm_batch_item *items, uint32_t n);
 /* ------------------------------------------------------------------------ */
 typedef struct rkfm_server rkfm_server;
 
-#define RKFM_SERVE_BATCHED 0u     /* one batched launch per round           */
+#define RKFM_SERVE_BATCHED 0u     /* one batched launch per round (with the */
+                                  /* default tuning, a round of one request */
+                                  /* takes that request's plain launch)     */
 #define RKFM_SERVE_PER_REQUEST 1u /* one plain launch per request, one at a */
                                   /* time (the per-request baseline)        */
 
@@ -641,9 +643,10 @@ typedef struct rkfm_server_config {
                                /* allocated once (0 = the context's budget,  */
                                /* else 1 GiB)                               */
     uint32_t max_requests;     /* admitted-request slots (0 = 1024)          */
-    uint32_t rounds_in_flight; /* 1..4 (0 = 2); PER_REQUEST forces 1         */
-    uint64_t max_round_work;   /* rays + queries per round (0 = no cap; a    */
-                               /* round always takes at least one request)  */
+    uint32_t rounds_in_flight; /* 1..4 (0 = 4); PER_REQUEST forces 1         */
+    uint64_t max_round_work;   /* rays + queries per round (0 = 2^20; a      */
+                               /* round always takes at least one request;  */
+                               /* UINT64_MAX = no cap)                      */
     uint32_t max_round_requests; /* requests per round (0 = 512)             */
     uint32_t policy;           /* RKFM_SERVE_*                               */
     rkfm_batch_tuning tuning;  /* batched launch shape (zero = 8x8 grid,     */

@@ -388,7 +388,10 @@ int rkfm_server::launch_plain(Slot &sl) {
 int rkfm_server::launch(Slot &sl) {
     int rc = rkf_hip_check(hipEventRecord(sl.ev_start, sl.stream), "server event");
     if (!rc) rc = copies_in(sl);
-    if (!rc) rc = cfg.policy == RKFM_SERVE_PER_REQUEST ? launch_plain(sl) : launch_batched(sl);
+    // with the default shape, a round of one request takes the plain
+    // kernel: no descriptor copies, the single-batch kernels' launch shapes
+    const bool plain = cfg.policy == RKFM_SERVE_PER_REQUEST || (adaptive && sl.reqs.size() == 1);
+    if (!rc) rc = plain ? launch_plain(sl) : launch_batched(sl);
     if (!rc) rc = copies_out(sl);
     hipError_t e = hipEventRecord(sl.ev_end, sl.stream);
     if (!rc && e != hipSuccess) rc = rkf_hip_check(e, "server event");
@@ -502,7 +505,8 @@ int rkfm_server_create(rkfm_ctx *c, const rkfm_sparse_scene *const *scenes, uint
     if (k.policy > RKFM_SERVE_PER_REQUEST) return rkf_set_error(RKFM_ERR_INVALID_ARG, "server_create: policy %u", k.policy);
     if (k.memory_budget == 0) k.memory_budget = c->memory_budget ? c->memory_budget : (1ull << 30);
     if (k.max_requests == 0) k.max_requests = 1024;
-    if (k.rounds_in_flight == 0) k.rounds_in_flight = 2;
+    if (k.rounds_in_flight == 0) k.rounds_in_flight = 4;
+    if (k.max_round_work == 0) k.max_round_work = 1ull << 20;  // a 1080p view runs in a round of its own
     if (k.rounds_in_flight > kMaxRounds)
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "server_create: rounds_in_flight %u > %u", k.rounds_in_flight,
                              kMaxRounds);
