@@ -61,26 +61,44 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
     }
 }
 
+// The batch's scene pointers as global pointers (they come from the
+// descriptor array, not from kernel arguments).
+RKF_DEV void global_scene(SparseDev &S) {
+    S.l2 = global_ptr(S.l2);
+    S.l1 = global_ptr(S.l1);
+    S.dist = global_ptr(S.dist);
+    S.mat = global_ptr(S.mat);
+    S.aux = global_ptr(S.aux);
+    S.free_idx = global_ptr(S.free_idx);
+    S.free_bits = global_ptr(S.free_bits);
+    S.oct = global_ptr(S.oct);
+    S.roww = global_ptr(S.roww);
+    S.cellw = global_ptr(S.cellw);
+    S.block_cell = global_ptr(S.block_cell);
+}
+
 template <bool kLowLat>
-__global__ __launch_bounds__(64) void batch_grid_kernel(const BatchDev *items, const uint32_t *begins,
+__global__ __launch_bounds__(64) void batch_grid_kernel(const BatchDev *__restrict__ items, const uint32_t *__restrict__ begins,
                                                         uint32_t n_items, uint32_t total) {
     uint32_t unit = blockIdx.y * gridDim.x + blockIdx.x;
     if (unit >= total) return;
     uint32_t b = find_batch(begins, n_items, unit);
-    const BatchDev &B = items[b];
+    BatchDev B = items[b];
+    global_scene(B.P.S);
     run_unit<kLowLat>(B, unit - B.unit_begin, threadIdx.x);
 }
 
-__global__ __launch_bounds__(64) void batch_persistent_kernel(const BatchDev *items, const uint32_t *begins,
+__global__ __launch_bounds__(64) void batch_persistent_kernel(const BatchDev *__restrict__ items, const uint32_t *__restrict__ begins,
                                                               uint32_t n_items, uint32_t total, uint32_t *counter) {
     const uint32_t lane = threadIdx.x;
     while (true) {
         uint32_t unit = 0;
         if (lane == 0) unit = atomicAdd(counter, 1u);
-        unit = (uint32_t)__shfl((int)unit, 0);
+        unit = __builtin_amdgcn_readfirstlane(unit);  // lane 0's (all lanes are active here)
         if (unit >= total) break;
         uint32_t b = find_batch(begins, n_items, unit);
-        const BatchDev &B = items[b];
+        BatchDev B = items[b];
+        global_scene(B.P.S);
         run_unit<false>(B, unit - B.unit_begin, lane);
     }
 }

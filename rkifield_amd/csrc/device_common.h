@@ -31,6 +31,17 @@ struct f3 {
 // being sunk one by one into the branches that consume them.
 RKF_DEV void loaded(uint32_t &v) { asm volatile("" : "+v"(v)); }
 
+// A pointer loaded from memory (not a kernel argument) that points to global
+// memory, re-typed so its loads and stores compile to global_* instead of
+// flat_* instructions (flat ones also count against lgkmcnt and check the
+// address aperture). The empty asm keeps the compiler from folding the
+// address-space round trip away; the value stays in SGPRs.
+template <class T> RKF_DEV T *global_ptr(T *p) {
+    __attribute__((address_space(1))) T *g = (__attribute__((address_space(1))) T *)p;
+    asm("" : "+s"(g));
+    return (T *)g;
+}
+
 RKF_DEV f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 RKF_DEV f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 RKF_DEV f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
