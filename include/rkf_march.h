@@ -650,7 +650,7 @@ typedef struct rkfm_server_config {
     uint32_t max_round_requests; /* requests per round (0 = 512)             */
     uint32_t policy;           /* RKFM_SERVE_*                               */
     rkfm_batch_tuning tuning;  /* batched launch shape (zero = 8x8 grid,     */
-                               /* low-latency for rounds under 65,536 units) */
+                               /* low-latency step code)                    */
     uint32_t max_pending;      /* submitted, not yet admitted (0 = 65536)    */
     uint32_t _reserved;
 } rkfm_server_config;

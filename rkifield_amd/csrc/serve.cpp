@@ -36,7 +36,6 @@ constexpr uint32_t kMaxRounds = 4;
 // Rounds below this many 64-lane work units (4 M rays) take the low-latency
 // schedule under the default shape: their time is set by their slowest rays
 // (one 1080p view is 32,400 units); larger rounds are bound by total work.
-constexpr uint32_t kLowLatUnits = 65536;
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -133,7 +132,7 @@ struct rkfm_server {
     std::vector<const rkfm_sparse_scene *> scenes;
     rkfm_server_config cfg{};
     rkfm_batch_tuning tuning{};
-    bool adaptive = false;  // zero tuning given: schedule chosen per round
+    bool adaptive = false;  // zero tuning given: the default shape
     uint32_t waves = 0;
     unsigned char *arena_base = nullptr;
     Arena arena;
@@ -353,10 +352,9 @@ int rkfm_server::launch_batched(Slot &sl) {
             hipMemcpyAsync(sl.d_begins.ptr, sl.h_begins, begins.size() * 4, hipMemcpyHostToDevice, sl.stream),
             "server begins");
     if (rc) return rc;
-    // Default shape: a round small enough that its slowest rays set its
-    // time takes the low-latency per-ray code, a large one the plain grid.
-    rkfm_batch_tuning t = tuning;
-    if (adaptive) t.schedule = total < kLowLatUnits ? RKFM_SCHEDULE_GRID_LOWLAT : RKFM_SCHEDULE_GRID;
+    // Default shape (set at creation): the 8x8 grid with the low-latency
+    // per-ray code, the faster schedule for single views and mixed sets alike.
+    const rkfm_batch_tuning &t = tuning;
     return rkf_batch_enqueue(t, (const rkf::BatchDev *)sl.d_items.ptr, (const uint32_t *)sl.d_begins.ptr,
                              (uint32_t)items.size(), total, (uint32_t *)sl.d_counter.ptr, waves, sl.stream);
 }
@@ -515,7 +513,7 @@ int rkfm_server_create(rkfm_ctx *c, const rkfm_sparse_scene *const *scenes, uint
     if (k.max_pending == 0) k.max_pending = 65536;
     rkfm_batch_tuning t = k.tuning;
     const bool adaptive = t.tile_w == 0 && t.tile_h == 0;
-    if (adaptive) t = rkfm_batch_tuning{RKFM_SCHEDULE_GRID, 8u, 8u, 0u};
+    if (adaptive) t = rkfm_batch_tuning{RKFM_SCHEDULE_GRID_LOWLAT, 8u, 8u, 0u};
     if (!rkf_batch_valid_shape(t)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "server_create: invalid launch shape");
     k.tuning = t;
     int rc = rkf_hip_check(hipSetDevice(c->device), "hipSetDevice");
