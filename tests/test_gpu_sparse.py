@@ -273,3 +273,8 @@ def test_sparse_errors(ctx):
     assert e.value.code == M.ERR["OOM"]
     with pytest.raises(M.RkfMarchError):
         M.SparseScene.build(ctx, prims, -1.0, lo, hi)
+    # brick grids are limited to 2^18 bricks per axis (packed per-ray cache keys)
+    with pytest.raises(M.RkfMarchError) as e:
+        M.SparseScene.from_dense(ctx, (8 * 32768 + 8, 1, 1), np.full(8 * 32768 + 8, 0xFFFFFFFF, np.uint32),
+                                 np.zeros(512, L.VOXEL_SAMPLE), 0.05, (0.0, 0.0, 0.0))
+    assert e.value.code == M.ERR["INVALID_ARG"]
