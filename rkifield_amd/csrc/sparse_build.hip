@@ -562,7 +562,7 @@ struct TmpBufs {
 };
 
 int put(hipStream_t st, DevBuf &b, const void *src, uint64_t bytes) {
-    int rc = b.reserve(std::max<uint64_t>(bytes, 1024), false);  // kernels may read the first KiB
+    int rc = b.reserve(bytes + 1024, false);  // kernels read the first KiB and 4-byte words at the end
     if (rc) return rc;
     b.size = bytes;
     if (bytes) rc = rkf_hip_check(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st), "upload");
