@@ -154,12 +154,11 @@ RKF_DEV float analytical(f3 p, const rkf_gpu_object &o) {
     }
 }
 
-// ray_march.wgsl:600-634
-RKF_DEV float eval_object(RefCtx &c, f3 wp, uint32_t oi, uint32_t &mat) {
+// ray_march.wgsl:600-634, for object record o (index oi).
+RKF_DEV float eval_obj(RefCtx &c, f3 wp, const rkf_gpu_object &o, uint32_t oi, uint32_t &mat) {
     const RefParams &P = *c.P;
     mat = 0;
     if (oi >= P.su.num_objects) return kMaxFloat;
-    const rkf_gpu_object &o = P.objects[oi];
     uint32_t st = o.sdf_type;
     if (st == RKF_SDF_TYPE_NONE) return kMaxFloat;
     f3 lp = xform_point(o.inverse_world, wp);
@@ -177,6 +176,11 @@ RKF_DEV float eval_object(RefCtx &c, f3 wp, uint32_t oi, uint32_t &mat) {
     }
     float ms = fminf(o.accumulated_scale[0], fminf(o.accumulated_scale[1], o.accumulated_scale[2]));
     return d * ms;
+}
+
+RKF_DEV float eval_object(RefCtx &c, f3 wp, uint32_t oi, uint32_t &mat) {
+    const RefParams &P = *c.P;
+    return eval_obj(c, wp, P.objects[oi < P.su.num_objects ? oi : 0u], oi, mat);
 }
 
 // ray_march.wgsl:228-235, software trilinear.
@@ -215,14 +219,14 @@ struct MarchRes {
     uint32_t mat, obj_id, obj_idx, steps;
 };
 
-RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, uint32_t oi, uint32_t best_mat, uint32_t best_id,
-                    MarchRes &r) {
+RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, const rkf_gpu_object &obj, uint32_t oi,
+                    uint32_t best_mat, uint32_t best_id, MarchRes &r) {
     uint32_t bm = best_mat;
     for (uint32_t b = 0; b < kBisect; b++) {
         float mid = (lo + hi) * 0.5f;
         f3 mp = add(o, muls(sd, mid));
         uint32_t mm;
-        float md = eval_object(c, mp, oi, mm);
+        float md = eval_obj(c, mp, obj, oi, mm);
         if (md < 0.0f) {
             hi = mid;
         } else {
@@ -237,21 +241,18 @@ RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, uint32_t oi, uin
     r.obj_idx = oi;
 }
 
-// ray_march.wgsl:641-985. `tile` is wave-uniform (an 8x8 pixel tile lies in
-// one 16x16 object tile), so the per-tile list and objects come through the
-// scalar cache.
-RKF_DEV void march(RefCtx &c, f3 o, f3 dir, uint32_t tile, MarchRes &r) {
+// ray_march.wgsl:641-985. Tiled mode: the tile's `count` object records and
+// indices, staged in LDS by the workgroup (an 8x8 pixel workgroup lies in one
+// 16x16 object tile), so a step's object loop reads no global memory.
+RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
+                   MarchRes &r) {
     const RefParams &P = *c.P;
     r.t = kMaxFloat;
     r.hit = false;
     r.mat = r.obj_id = r.obj_idx = 0;
     r.steps = 0;
-    uint32_t count = 0, base = 0;
     if (P.mode == 0) {
-        count = tile < P.n_tiles ? P.tile_counts[tile] : 0u;
-        if (count > RKF_TILE_MAX_OBJECTS) count = RKF_TILE_MAX_OBJECTS;
         if (count == 0) return;
-        base = tile * RKF_TILE_MAX_OBJECTS;
     } else if (P.mode == 1) {
         if (P.su.num_objects == 0) return;
     }
@@ -272,17 +273,18 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, uint32_t tile, MarchRes &r) {
             }
         }
         float min_dist = kMaxFloat;
-        uint32_t best_mat = 0, best_id = 0, best_idx = 0;
+        uint32_t best_mat = 0, best_id = 0, best_idx = 0, best_j = 0;
         if (P.mode == 0) {
             for (uint32_t i = 0; i < count; i++) {
-                uint32_t oi = P.tile_idx[base + i];
+                const uint32_t oi = toi[i];
                 uint32_t m;
-                float d = eval_object(c, pos, oi, m);
+                float d = eval_obj(c, pos, tobj[i], oi, m);
                 if (d < min_dist) {
                     min_dist = d;
                     best_mat = m;
-                    best_id = oi < P.su.num_objects ? P.objects[oi].object_id : 0u;
+                    best_id = oi < P.su.num_objects ? tobj[i].object_id : 0u;
                     best_idx = oi;
+                    best_j = i;
                 }
             }
         } else if (P.mode == 1) {
@@ -327,7 +329,9 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, uint32_t tile, MarchRes &r) {
             }
         }
         if (min_dist < 0.0f && prev_dist > hit_eps(P.eps_mode, P.su.hit_threshold, prev_t)) {
-            bisect(c, o, sd, prev_t, t, best_idx, best_mat, best_id, r);
+            const rkf_gpu_object &bo =
+                P.mode == 0 ? tobj[best_j] : P.objects[best_idx < P.su.num_objects ? best_idx : 0u];
+            bisect(c, o, sd, prev_t, t, bo, best_idx, best_mat, best_id, r);
             return;
         }
         if (min_dist < hit_eps(P.eps_mode, P.su.hit_threshold, t)) {
@@ -373,18 +377,45 @@ RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t oi) {
 
 // ray_march.wgsl:1036-1111. Block = 64 lanes = 8x8 pixels.
 __global__ __launch_bounds__(64) void ray_march_ref_kernel(RefParams P) {
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
+    __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
+    const uint32_t lane = threadIdx.x;
+    uint32_t px = blockIdx.x * 8u + (lane & 7u);
+    uint32_t py = blockIdx.y * 8u + (lane >> 3);
     const uint32_t W = P.g.width, H = P.g.height;
+    uint32_t count = 0;
+    if (P.mode == 0) {
+        // Stage the tile's object list and records (64 lanes, one dword of a
+        // 256-byte record each; records past num_objects are never read).
+        const uint32_t ntx = (W + 15u) / 16u;
+        const uint32_t tile = ((blockIdx.y * 8u) / 16u) * ntx + (blockIdx.x * 8u) / 16u;
+        count = tile < P.n_tiles ? P.tile_counts[tile] : 0u;
+        if (count > RKF_TILE_MAX_OBJECTS) count = RKF_TILE_MAX_OBJECTS;
+        if (lane < count) s_oi[lane] = P.tile_idx[(uint64_t)tile * RKF_TILE_MAX_OBJECTS + lane];
+        __syncthreads();
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(P.objects);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(s_obj);
+        for (uint32_t j0 = 0; j0 < count; j0 += 8u) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++) {
+                const uint32_t j = j0 + k;
+                const uint32_t oi = j < count ? s_oi[j] : ~0u;
+                v[k] = oi < P.su.num_objects ? src[(uint64_t)oi * 64u + lane] : 0u;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; k++)
+                if (j0 + k < count) dst[(j0 + k) * 64u + lane] = v[k];
+        }
+        __syncthreads();
+    }
     if (px >= W || py >= H) return;
     uint64_t p = (uint64_t)py * W + px;
     RefCtx c{&P, 0u};
     f3 o, dir;
     camera_ray(P.cam, W, H, px, py, o, dir);
-    uint32_t ntx = (W + 15u) / 16u;
-    uint32_t tile = ((blockIdx.y * 8u) / 16u) * ntx + (blockIdx.x * 8u) / 16u;
     MarchRes r;
-    march(c, o, dir, tile, r);
+    march(c, o, dir, s_obj, s_oi, count, r);
     if (!r.hit) {
         store_miss(P.g, p);
         store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
