@@ -29,8 +29,11 @@ RKF_DEV uint32_t map_entry(const RefParams &P, uint64_t i) { return i < P.n_maps
 RKF_DEV uint32_t pool_w0(const RefParams &P, uint64_t i) { return i < P.n_pool_voxels ? P.pool[i].word0 : 0u; }
 RKF_DEV uint32_t pool_w1(const RefParams &P, uint64_t i) { return i < P.n_pool_voxels ? P.pool[i].word1 : 0u; }
 
-// ray_march.wgsl:353-413 / :463-498.
-RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool grad) {
+// ray_march.wgsl:353-413 / :463-498. mat (optional): set to the material
+// of the voxel nearest_voxel would read (:540-566) when that voxel is one of
+// the 8 corners just loaded (the point lies inside the clamp range, so
+// floor(gp / vs) is x0 or x0 + 1 per axis), else left unchanged.
+RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool grad, uint32_t *mat = nullptr) {
     float vs = o.voxel_size;
     float be = vs * 8.0f;
     uint32_t dims[3] = {o.brick_map_dims[0], o.brick_map_dims[1], o.brick_map_dims[2]};
@@ -94,6 +97,20 @@ RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool g
         cv[k] = ent[k] == RKF_EMPTY_SLOT      ? vs * 8.0f
                 : ent[k] == RKF_INTERIOR_SLOT ? (grad ? -(vs * 8.0f) : -(vs * 2.0f))
                                               : h2f((uint16_t)(ok[k] ? raw[k] : 0u));
+    }
+    if (mat && cl.x == gp.x && cl.y == gp.y && cl.z == gp.z) {
+        // nearest_voxel: floor(gp / vs) clamped to the grid, EMPTY / INTERIOR -> 0
+        const uint32_t k = ((int32_t)floorf(cl.x / vs) != x0 ? 1u : 0u) |
+                           ((int32_t)floorf(cl.y / vs) != y0 ? 2u : 0u) |
+                           ((int32_t)floorf(cl.z / vs) != z0 ? 4u : 0u);
+        uint32_t e = ent[0], w = raw[0];
+#pragma unroll
+        for (uint32_t j = 1; j < 8u; j++) {
+            e = k == j ? ent[j] : e;
+            w = k == j ? (ok[j] ? raw[j] : 0u) : w;
+        }
+        if (k == 0u) w = ok[0] ? raw[0] : 0u;
+        *mat = e >= RKF_INTERIOR_SLOT ? 0u : ((w >> 16) & 0x3Fu);
     }
     const float c000 = cv[0], c100 = cv[1], c010 = cv[2], c110 = cv[3];
     const float c001 = cv[4], c101 = cv[5], c011 = cv[6], c111 = cv[7];
@@ -167,11 +184,16 @@ RKF_DEV float eval_obj(RefCtx &c, f3 wp, const rkf_gpu_object &o, uint32_t oi, u
         d = analytical(lp, o);
         mat = o.material_id;
     } else {
-        d = sample_voxelized(c, lp, o, false);
+        uint32_t near_mat = ~0u;
+        d = sample_voxelized(c, lp, o, false, &near_mat);
         if (d < o.voxel_size * 16.0f) {
-            uint32_t sl, vx;
-            uint64_t i = nearest_voxel(P, lp, o, sl, vx);
-            mat = i == ~0ull ? 0u : ((pool_w0(P, i) >> 16) & 0x3Fu);
+            if (near_mat != ~0u) {
+                mat = near_mat;  // the nearest voxel was a loaded corner
+            } else {
+                uint32_t sl, vx;
+                uint64_t i = nearest_voxel(P, lp, o, sl, vx);
+                mat = i == ~0ull ? 0u : ((pool_w0(P, i) >> 16) & 0x3Fu);
+            }
         }
     }
     float ms = fminf(o.accumulated_scale[0], fminf(o.accumulated_scale[1], o.accumulated_scale[2]));
