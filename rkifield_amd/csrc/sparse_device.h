@@ -138,7 +138,6 @@ RKF_DEV uint32_t sp_brick_slot(const SparseDev &S, uint32_t bx, uint32_t by, uin
 }
 
 RKF_DEV void bc_init(const SparseDev &S, f3 p, int32_t reach, BrickCache &bc) {
-    const float vs = S.vs;
     f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
     float g[3] = {gp.x * S.inv_vs - 0.5f, gp.y * S.inv_vs - 0.5f, gp.z * S.inv_vs - 0.5f};
 #pragma unroll
