@@ -121,7 +121,7 @@ RKF_DEV float sp_sample(SpCtx &c, f3 p, bool grad, bool count, uint32_t pre = kN
 // Slots of the (at most 2 x 2 x 2) bricks holding every corner voxel of the
 // samples within `reach` voxels of p on each axis, looked up once so that the
 // shading samples around a hit (6 gradient taps at +-2 vs, the value at the
-// hit) share one index walk. bc_voxel falls back to the index for a corner
+// hit) share one index walk. sp_sample_bc falls back to the index for a corner
 // outside the cached bricks, so results equal sp_voxel's exactly.
 struct BrickCache {
     int32_t b0[3];
@@ -149,39 +149,50 @@ RKF_DEV void bc_init(const SparseDev &S, f3 p, int32_t reach, BrickCache &bc) {
         bc.b0[a] = lo >> 3;
         bc.span[a] = (uint32_t)((hi >> 3) - (lo >> 3)) + 1u;
     }
+    // sp_brick_slot for each cached brick: the L2 entries in one round, the
+    // L1 slots in a second (unused loads read entry 0)
+    uint32_t blk[8], lb[8];
+    bool need[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8; k++) {
-        uint32_t dx = k & 1u, dy = (k >> 1) & 1u, dz = k >> 2;
-        bool need = dx < bc.span[0] && dy < bc.span[1] && dz < bc.span[2];
-        bc.slot[k] = need ? sp_brick_slot(S, (uint32_t)bc.b0[0] + dx, (uint32_t)bc.b0[1] + dy, (uint32_t)bc.b0[2] + dz)
-                          : RKF_EMPTY_SLOT;
+        const uint32_t dx = k & 1u, dy = (k >> 1) & 1u, dz = k >> 2;
+        need[k] = dx < bc.span[0] && dy < bc.span[1] && dz < bc.span[2];
+        const uint32_t bx = (uint32_t)bc.b0[0] + dx, by = (uint32_t)bc.b0[1] + dy, bz = (uint32_t)bc.b0[2] + dz;
+        const uint64_t cell = (uint64_t)(bx >> 3) + (uint64_t)(by >> 3) * S.cells[0] +
+                              (uint64_t)(bz >> 3) * S.cells[0] * S.cells[1];
+        lb[k] = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+        blk[k] = S.l2[need[k] ? cell : 0u];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        loaded(blk[k]);
+        need[k] = need[k] && blk[k] != RKF_EMPTY_SLOT;
+        bc.slot[k] = S.l1[need[k] ? (uint64_t)blk[k] * 512u + lb[k] : 0u];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        loaded(bc.slot[k]);
+        bc.slot[k] = need[k] ? bc.slot[k] : RKF_EMPTY_SLOT;
     }
 }
 
-RKF_DEV float bc_voxel(const SparseDev &S, const BrickCache &bc, int32_t cx, int32_t cy, int32_t cz, float vs,
-                       bool grad) {
-    cx = clampi(cx, 0, S.tot[0] - 1);
-    cy = clampi(cy, 0, S.tot[1] - 1);
-    cz = clampi(cz, 0, S.tot[2] - 1);
-    uint32_t dx = (uint32_t)((cx >> 3) - bc.b0[0]), dy = (uint32_t)((cy >> 3) - bc.b0[1]),
-             dz = (uint32_t)((cz >> 3) - bc.b0[2]);
-    uint32_t slot;
-    if (dx < bc.span[0] && dy < bc.span[1] && dz < bc.span[2]) {
-        // selects rather than a dynamic index, so the slots stay in registers
-        const uint32_t *s = bc.slot;
-        uint32_t a0 = dx ? s[1] : s[0], a1 = dx ? s[3] : s[2], a2 = dx ? s[5] : s[4], a3 = dx ? s[7] : s[6];
-        uint32_t b0 = dy ? a1 : a0, b1 = dy ? a3 : a2;
-        slot = dz ? b1 : b0;
-    } else {
-        slot = sp_slot(S, cx, cy, cz);
-    }
-    if (slot == RKF_EMPTY_SLOT) return vs * 8.0f;
-    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * 8.0f) : -(vs * 2.0f);
-    uint64_t idx = (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
-    return h2f(S.dist[idx]);
+// Slot of the brick holding voxel (cx, cy, cz) (clamped to the grid): from
+// the cache by selects when the brick is cached (in_cache), else not set.
+RKF_DEV uint32_t bc_slot(const BrickCache &bc, int32_t cx, int32_t cy, int32_t cz, bool &in_cache) {
+    const uint32_t dx = (uint32_t)((cx >> 3) - bc.b0[0]), dy = (uint32_t)((cy >> 3) - bc.b0[1]),
+                   dz = (uint32_t)((cz >> 3) - bc.b0[2]);
+    in_cache = dx < bc.span[0] && dy < bc.span[1] && dz < bc.span[2];
+    const uint32_t *s = bc.slot;
+    const bool ox = dx != 0u, oy = dy != 0u, oz = dz != 0u;
+    const uint32_t a0 = ox ? s[1] : s[0], a1 = ox ? s[3] : s[2], a2 = ox ? s[5] : s[4], a3 = ox ? s[7] : s[6];
+    const uint32_t b0 = oy ? a1 : a0, b1 = oy ? a3 : a2;
+    return oz ? b1 : b0;
 }
 
-// sp_sample with corners from a brick cache (same arithmetic, same results).
+// sp_sample with corners from a brick cache (same arithmetic, same results):
+// the 8 corner slots by selects (the index for a corner outside the cached
+// bricks, only when some lane has one), then the 8 distance loads in one
+// round.
 RKF_DEV float sp_sample_bc(const SparseDev &S, const BrickCache &bc, f3 p, bool grad) {
     float vs = S.vs;
     float be = vs * 8.0f;
@@ -194,20 +205,48 @@ RKF_DEV float sp_sample_bc(const SparseDev &S, const BrickCache &bc, f3 p, bool 
     f3 vc = mk(cl.x * S.inv_vs - 0.5f, cl.y * S.inv_vs - 0.5f, cl.z * S.inv_vs - 0.5f);
     int32_t x0 = (int32_t)floorf(vc.x), y0 = (int32_t)floorf(vc.y), z0 = (int32_t)floorf(vc.z);
     float tx = vc.x - (float)x0, ty = vc.y - (float)y0, tz = vc.z - (float)z0;
-    float c000 = bc_voxel(S, bc, x0, y0, z0, vs, grad), c100 = bc_voxel(S, bc, x0 + 1, y0, z0, vs, grad);
-    float c010 = bc_voxel(S, bc, x0, y0 + 1, z0, vs, grad), c110 = bc_voxel(S, bc, x0 + 1, y0 + 1, z0, vs, grad);
-    float c001 = bc_voxel(S, bc, x0, y0, z0 + 1, vs, grad), c101 = bc_voxel(S, bc, x0 + 1, y0, z0 + 1, vs, grad);
-    float c011 = bc_voxel(S, bc, x0, y0 + 1, z0 + 1, vs, grad);
-    float c111 = bc_voxel(S, bc, x0 + 1, y0 + 1, z0 + 1, vs, grad);
-    float c00 = mix(c000, c100, tx), c10 = mix(c010, c110, tx);
-    float c01 = mix(c001, c101, tx), c11 = mix(c011, c111, tx);
+    const int32_t X[2] = {clampi(x0, 0, S.tot[0] - 1), clampi(x0 + 1, 0, S.tot[0] - 1)};
+    const int32_t Y[2] = {clampi(y0, 0, S.tot[1] - 1), clampi(y0 + 1, 0, S.tot[1] - 1)};
+    const int32_t Z[2] = {clampi(z0, 0, S.tot[2] - 1), clampi(z0 + 1, 0, S.tot[2] - 1)};
+    uint32_t sl[8];
+    bool inc[8];
+    bool miss = false;
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        sl[k] = bc_slot(bc, X[k & 1u], Y[(k >> 1) & 1u], Z[k >> 2], inc[k]);
+        miss = miss || !inc[k];
+    }
+    if (__any(miss)) {
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; k++)
+            if (!inc[k]) sl[k] = sp_slot(S, X[k & 1u], Y[(k >> 1) & 1u], Z[k >> 2]);
+    }
+    uint32_t raw[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const bool live = sl[k] < RKF_INTERIOR_SLOT;
+        const uint32_t loc = (uint32_t)(X[k & 1u] & 7) + (uint32_t)(Y[(k >> 1) & 1u] & 7) * 8u +
+                             (uint32_t)(Z[k >> 2] & 7) * 64u;
+        raw[k] = S.dist[(uint64_t)(live ? sl[k] : 0u) * 512u + (live ? loc : 0u)];
+    }
+    const float v_empty = vs * 8.0f, v_interior = grad ? -(vs * 8.0f) : -(vs * 2.0f);
+    float c[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        loaded(raw[k]);
+        const uint32_t sent = __float_as_uint(sl[k] == RKF_EMPTY_SLOT ? v_empty : v_interior);
+        c[k] = __uint_as_float(sl[k] < RKF_INTERIOR_SLOT ? __float_as_uint(h2f(raw[k])) : sent);
+    }
+    float c00 = mix(c[0], c[1], tx), c10 = mix(c[2], c[3], tx);
+    float c01 = mix(c[4], c[5], tx), c11 = mix(c[6], c[7], tx);
     float c0 = mix(c00, c10, ty), c1 = mix(c01, c11, ty);
     return mix(c0, c1, tz) + outside;
 }
 
 // Nearest voxel floor(grid_pos * inv_vs) as ray_march.wgsl:540-596. Returns the
 // pool index or ~0.
-RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o) {
+RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o,
+                            const BrickCache *bc = nullptr) {
     float vs = S.vs;
     float be = vs * 8.0f;
     f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
@@ -219,7 +258,9 @@ RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t
     int32_t cx = clampi((int32_t)floorf(gp.x * S.inv_vs), 0, S.tot[0] - 1);
     int32_t cy = clampi((int32_t)floorf(gp.y * S.inv_vs), 0, S.tot[1] - 1);
     int32_t cz = clampi((int32_t)floorf(gp.z * S.inv_vs), 0, S.tot[2] - 1);
-    uint32_t slot = sp_slot(S, cx, cy, cz);
+    bool in_cache = false;
+    uint32_t slot = bc ? bc_slot(*bc, cx, cy, cz, in_cache) : 0u;
+    if (!in_cache) slot = sp_slot(S, cx, cy, cz);
     cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * S.cells[0] + ((uint32_t)cz >> 6) * S.cells[0] * S.cells[1];
     slot_o = slot;
     uint32_t lv = (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
@@ -228,9 +269,9 @@ RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t
     return (uint64_t)slot * 512u + lv;
 }
 
-RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p) {
+RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p, const BrickCache *bc = nullptr) {
     uint32_t a, b, d;
-    uint64_t i = sp_nearest(S, p, a, b, d);
+    uint64_t i = sp_nearest(S, p, a, b, d, bc);
     return i == ~0ull ? 0u : (uint32_t)(S.mat[i] & 0x3Fu);
 }
 
@@ -783,11 +824,11 @@ RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, c
     f3 o, dir;
     camera_ray(P.cam, W, H, px, py, o, dir);
     f3 hp = add(o, muls(dir, res.t));
-    uint32_t mat = 0;
-    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p);
     float e = S.vs * 2.0f;
     BrickCache bc;
     bc_init(S, hp, 3, bc);
+    uint32_t mat = 0;
+    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p, &bc);
     float gx = sp_sample_bc(S, bc, add(hp, mk(e, 0.0f, 0.0f)), true) - sp_sample_bc(S, bc, sub(hp, mk(e, 0.0f, 0.0f)), true);
     float gy = sp_sample_bc(S, bc, add(hp, mk(0.0f, e, 0.0f)), true) - sp_sample_bc(S, bc, sub(hp, mk(0.0f, e, 0.0f)), true);
     float gz = sp_sample_bc(S, bc, add(hp, mk(0.0f, 0.0f, e)), true) - sp_sample_bc(S, bc, sub(hp, mk(0.0f, 0.0f, e)), true);
@@ -795,7 +836,7 @@ RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, c
     float len = length(gv);
     f3 n = len < 1e-10f ? mk(0.0f, 1.0f, 0.0f) : mk(gv.x / len, gv.y / len, gv.z / len);
     uint32_t slot, vox, cell;
-    uint64_t vi = sp_nearest(S, hp, slot, vox, cell);
+    uint64_t vi = sp_nearest(S, hp, slot, vox, cell, &bc);
     uint32_t sec = 0, blend = 0;
     if (S.aux && vi != ~0ull) {
         uint32_t a = S.aux[vi];
