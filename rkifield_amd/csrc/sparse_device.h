@@ -719,12 +719,17 @@ RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &
     }
     if (d < 0.0f && r.prev_dist > hit_eps(eps_mode, su.hit_threshold, r.prev_t)) {
         float lo = r.prev_t, hi = r.t, mt = r.t, md = d;
+        // low latency: the bricks within 3 voxels of the segment's midpoint,
+        // looked up once for the 8 bisection samples
+        BrickCache bc;
+        if constexpr (kLowLat) bc_init(S, add(r.o, muls(r.sd, (lo + hi) * 0.5f)), 3, bc);
         for (uint32_t b = 0; b < kBisect; b++) {
             float mid = (lo + hi) * 0.5f;
             const f3 mp = add(r.o, muls(r.sd, mid));
             float mdd;
             if constexpr (kLowLat) {
-                mdd = sp_sample_nb(c, r.nb, mp);
+                c.evals++;
+                mdd = sp_sample_bc(S, bc, mp, false);
             } else {
                 mdd = sp_sample(c, mp, false, true);
             }
