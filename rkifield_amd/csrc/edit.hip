@@ -147,9 +147,11 @@ __global__ __launch_bounds__(256) void edit_collect_kernel(SparseDev S, const Ed
 
 // One 512-lane block per listed brick (grid-stride), one lane per voxel; the
 // ops are applied in order with the voxel kept in registers.
-__global__ __launch_bounds__(512) void edit_apply_kernel(SparseDev S, const EditDev *ops, uint32_t n_ops,
-                                                         const EditBrick *list, const uint32_t *count, uint16_t *dist,
-                                                         uint8_t *mat, uint16_t *aux, uint8_t *out_mod) {
+__global__ __launch_bounds__(512) void edit_apply_kernel(SparseDev S, const EditDev *__restrict__ ops, uint32_t n_ops,
+                                                         const EditBrick *__restrict__ list,
+                                                         const uint32_t *__restrict__ count, uint16_t *__restrict__ dist,
+                                                         uint8_t *__restrict__ mat, uint16_t *__restrict__ aux,
+                                                         uint8_t *__restrict__ out_mod) {
     __shared__ uint32_t changed;
     const uint32_t n = *count;
     const uint32_t t = threadIdx.x;
