@@ -904,18 +904,21 @@ RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
 // Point query (sdf_collision.rs:38-56): distance, nearest material, inside
 // flag, central-difference gradient at eps 0.01 with normalize_or_zero
 // (v * (1/len) when that reciprocal is finite and > 0).
+// The 7 samples and the material share one brick-cache lookup around p (a
+// corner outside it falls back to the index), each sample one round of loads.
 RKF_DEV void point_query(const SparseDev &S, f3 p, rkfm_point_result &out) {
-    SpCtx c{&S, 0u};
-    float d = sp_sample(c, p, false, false);
+    BrickCache bc;
+    bc_init(S, p, 3, bc);
+    float d = sp_sample_bc(S, bc, p, false);
     const float e = 0.01f;
-    float dx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), false, false) - sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), false, false);
-    float dy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), false, false) - sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), false, false);
-    float dz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), false, false) - sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), false, false);
+    float dx = sp_sample_bc(S, bc, add(p, mk(e, 0.0f, 0.0f)), false) - sp_sample_bc(S, bc, sub(p, mk(e, 0.0f, 0.0f)), false);
+    float dy = sp_sample_bc(S, bc, add(p, mk(0.0f, e, 0.0f)), false) - sp_sample_bc(S, bc, sub(p, mk(0.0f, e, 0.0f)), false);
+    float dz = sp_sample_bc(S, bc, add(p, mk(0.0f, 0.0f, e)), false) - sp_sample_bc(S, bc, sub(p, mk(0.0f, 0.0f, e)), false);
     float len = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
     float rcp = 1.0f / len;
     rkfm_point_result r;
     r.distance = d;
-    r.material = sp_material(S, p);
+    r.material = sp_material(S, p, &bc);
     r.inside = d < 0.0f ? 1u : 0u;
     if (isfinite(rcp) && rcp > 0.0f) {
         r.gradient[0] = dx * rcp;
@@ -988,11 +991,12 @@ RKF_DEV f3 quat_mul_vec(const float q[4], f3 v) {
 
 // SdfQueryable::gradient via gradient_central_diff (sdf_collision.rs:44-56).
 RKF_DEV f3 field_gradient(const SparseDev &S, f3 p) {
-    SpCtx c{&S, 0u};
+    BrickCache bc;
+    bc_init(S, p, 3, bc);
     const float e = 0.01f;
-    float dx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), false, false) - sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), false, false);
-    float dy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), false, false) - sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), false, false);
-    float dz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), false, false) - sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), false, false);
+    float dx = sp_sample_bc(S, bc, add(p, mk(e, 0.0f, 0.0f)), false) - sp_sample_bc(S, bc, sub(p, mk(e, 0.0f, 0.0f)), false);
+    float dy = sp_sample_bc(S, bc, add(p, mk(0.0f, e, 0.0f)), false) - sp_sample_bc(S, bc, sub(p, mk(0.0f, e, 0.0f)), false);
+    float dz = sp_sample_bc(S, bc, add(p, mk(0.0f, 0.0f, e)), false) - sp_sample_bc(S, bc, sub(p, mk(0.0f, 0.0f, e)), false);
     float rcp = 1.0f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
     if (isfinite(rcp) && rcp > 0.0f) return mk(dx * rcp, dy * rcp, dz * rcp);
     return mk(0.0f, 0.0f, 0.0f);
