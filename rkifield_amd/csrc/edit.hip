@@ -15,8 +15,8 @@
 // Launch shape: a collect kernel walks every op's brick range (one thread
 // per brick) and lists each allocated brick once (a per-slot bit dedups
 // bricks shared by several ops); an apply kernel gives each listed brick a
-// 512-lane block, one lane per voxel, that walks the ops in order with the
-// voxel in registers, so a batch of edits reads and writes each touched brick
+// wavefront, eight voxels per lane, that walks the ops in order with the
+// voxels in registers, so a batch of edits reads and writes each touched brick
 // once; a clear kernel resets the dedup bits. No host work scales with the
 // edit sizes and nothing waits for the device unless the modified list is
 // asked for. Edits never allocate or free bricks, so the index and
@@ -145,106 +145,126 @@ __global__ __launch_bounds__(256) void edit_collect_kernel(SparseDev S, const Ed
     list[k] = b;
 }
 
-// One 512-lane block per listed brick (grid-stride), one lane per voxel; the
-// ops are applied in order with the voxel kept in registers.
-__global__ __launch_bounds__(512) void edit_apply_kernel(SparseDev S, const EditDev *__restrict__ ops, uint32_t n_ops,
+// One op of apply_edit_cpu (cpu_apply.rs:84-170) on one voxel at (px, py, pz)
+// (its stored binary16 distance bits, 6-bit material and aux word); `any`
+// records a distance change above thr or a paint.
+RKF_DEV void edit_voxel(const EditDev &E, float px, float py, float pz, float thr, uint32_t &dbits, uint32_t &m,
+                        uint32_t &ax, bool &any) {
+    const rkfm_edit &op = E.op;
+    float rx = px - op.position[0], ry = py - op.position[1], rz = pz - op.position[2];
+    float iq[4] = {-op.rotation[0], -op.rotation[1], -op.rotation[2], op.rotation[3]};
+    float lx, ly, lz;
+    quat_rotate(iq, rx, ry, rz, lx, ly, lz);
+    float shape_d = edit_shape(op.shape_type, op.dimensions, lx, ly, lz);
+    float w = edit_falloff(op.falloff, e_len3(rx, ry, rz), E.falloff_radius);
+    if (w <= 0.0f) return;
+    float existing = h2f(dbits);
+    float strength = op.strength * w;
+    float nd = existing;
+    uint32_t nm = m;
+    switch (op.edit_type) {
+    case RKFM_EDIT_CSG_UNION:
+    case RKFM_EDIT_SMOOTH_UNION: {
+        float eff = shape_d + (1.0f - strength) * E.max_extent;
+        if (op.edit_type == RKFM_EDIT_CSG_UNION) {
+            nd = fminf(existing, eff);
+        } else {
+            float kk = fmaxf(op.blend_k, 0.001f);
+            nd = edit_smin(existing, eff, kk);
+        }
+        nm = eff < existing ? (op.material_id & 0x3Fu) : ((nd < 0.0f && m == 0u) ? (op.material_id & 0x3Fu) : m);
+        break;
+    }
+    case RKFM_EDIT_CSG_SUBTRACT: {
+        float eff = shape_d + (1.0f - strength) * E.max_extent;
+        nd = fmaxf(existing, -eff);
+        break;
+    }
+    case RKFM_EDIT_CSG_INTERSECT: {
+        float eff = shape_d - (1.0f - strength) * E.max_extent;
+        nd = fmaxf(existing, eff);
+        break;
+    }
+    case RKFM_EDIT_SMOOTH_SUBTRACT: {
+        float kk = fmaxf(op.blend_k, 0.001f);
+        float eff = shape_d + (1.0f - strength) * E.max_extent;
+        nd = -edit_smin(-existing, eff, kk);
+        break;
+    }
+    case RKFM_EDIT_SMOOTH: nd = existing * (1.0f - strength * 0.3f); break;
+    case RKFM_EDIT_FLATTEN: nd = existing + (ly - existing) * (strength * 0.5f); break;
+    case RKFM_EDIT_PAINT:
+        if (fabsf(existing) < op.dimensions[0] * 2.0f) {
+            m = op.material_id & 0x3Fu;
+            ax = 0u;
+            any = true;
+            dbits = f2h(existing);  // stored again as binary16, like every written voxel
+        }
+        return;
+    default: return;  // color paint: no geometry or material change
+    }
+    any |= fabsf(nd - existing) > thr;
+    dbits = f2h(nd);
+    m = nm;
+}
+
+// One wavefront per listed brick (grid-stride over the list, four bricks per
+// 256-lane block). Lane l holds the x-row y = l & 7, z = l >> 3 (voxels
+// 8l..8l+7 of the brick), read and written as one 16-byte distance, 8-byte
+// material and 16-byte aux vector, so every lane has its three loads in flight
+// at once; the ops are applied in order to each voxel in registers.
+__global__ __launch_bounds__(256) void edit_apply_kernel(SparseDev S, const EditDev *__restrict__ ops, uint32_t n_ops,
                                                          const EditBrick *__restrict__ list,
                                                          const uint32_t *__restrict__ count, uint16_t *__restrict__ dist,
                                                          uint8_t *__restrict__ mat, uint16_t *__restrict__ aux,
                                                          uint8_t *__restrict__ out_mod) {
-    __shared__ uint32_t changed;
     const uint32_t n = *count;
-    const uint32_t t = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nw = gridDim.x * 4u;
     const float vs = S.vs;
     const float bws = vs * 8.0f;
     const float thr = vs * 0.1f;
-    const uint32_t vx = t & 7u, vy = (t >> 3) & 7u, vz = t >> 6;
-    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const uint32_t vy = lane & 7u, vz = lane >> 3;
+    for (uint32_t e = blockIdx.x * 4u + (threadIdx.x >> 6); e < n; e += nw) {
         const EditBrick B = list[e];
-        if (t == 0) changed = 0u;
-        __syncthreads();
         // brick_local_min = origin + b * brick_size; voxel centre = min + v * vs + vs / 2
-        const float px = (S.origin[0] + (float)B.x * bws) + (float)vx * vs + vs * 0.5f;
+        const float bx = S.origin[0] + (float)B.x * bws;
         const float py = (S.origin[1] + (float)B.y * bws) + (float)vy * vs + vs * 0.5f;
         const float pz = (S.origin[2] + (float)B.z * bws) + (float)vz * vs + vs * 0.5f;
-        const uint64_t vi = (uint64_t)B.slot * 512u + t;
-        uint16_t dbits = dist[vi];
-        uint32_t m = mat[vi];
-        uint32_t ax = aux ? aux[vi] : 0u;
+        const uint64_t v0 = (uint64_t)B.slot * 512u + lane * 8u;
+        const uint4 dw = *reinterpret_cast<const uint4 *>(dist + v0);
+        const uint2 mw = *reinterpret_cast<const uint2 *>(mat + v0);
+        const uint4 aw = aux ? *reinterpret_cast<const uint4 *>(aux + v0) : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t dwv[4] = {dw.x, dw.y, dw.z, dw.w}, awv[4] = {aw.x, aw.y, aw.z, aw.w};
+        uint32_t db[8], m[8], ax[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            db[j] = (dwv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            ax[j] = (awv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            m[j] = ((j < 4 ? mw.x : mw.y) >> (8 * (j & 3))) & 0xFFu;
+        }
         bool any = false;
         for (uint32_t k = 0; k < n_ops; k++) {
             const EditDev &E = ops[k];
             if ((int32_t)B.x < E.bmin[0] || (int32_t)B.x > E.bmax[0] || (int32_t)B.y < E.bmin[1] ||
                 (int32_t)B.y > E.bmax[1] || (int32_t)B.z < E.bmin[2] || (int32_t)B.z > E.bmax[2])
-                continue;  // uniform per block
-            const rkfm_edit &op = E.op;
-            float rx = px - op.position[0], ry = py - op.position[1], rz = pz - op.position[2];
-            float iq[4] = {-op.rotation[0], -op.rotation[1], -op.rotation[2], op.rotation[3]};
-            float lx, ly, lz;
-            quat_rotate(iq, rx, ry, rz, lx, ly, lz);
-            float shape_d = edit_shape(op.shape_type, op.dimensions, lx, ly, lz);
-            float w = edit_falloff(op.falloff, e_len3(rx, ry, rz), E.falloff_radius);
-            if (w <= 0.0f) continue;
-            float existing = h2f(dbits);
-            float strength = op.strength * w;
-            float nd = existing;
-            uint32_t nm = m;
-            bool write = true;
-            switch (op.edit_type) {
-            case RKFM_EDIT_CSG_UNION:
-            case RKFM_EDIT_SMOOTH_UNION: {
-                float eff = shape_d + (1.0f - strength) * E.max_extent;
-                if (op.edit_type == RKFM_EDIT_CSG_UNION) {
-                    nd = fminf(existing, eff);
-                } else {
-                    float kk = fmaxf(op.blend_k, 0.001f);
-                    nd = edit_smin(existing, eff, kk);
-                }
-                nm = eff < existing ? (op.material_id & 0x3Fu)
-                                    : ((nd < 0.0f && m == 0u) ? (op.material_id & 0x3Fu) : m);
-                break;
-            }
-            case RKFM_EDIT_CSG_SUBTRACT: {
-                float eff = shape_d + (1.0f - strength) * E.max_extent;
-                nd = fmaxf(existing, -eff);
-                break;
-            }
-            case RKFM_EDIT_CSG_INTERSECT: {
-                float eff = shape_d - (1.0f - strength) * E.max_extent;
-                nd = fmaxf(existing, eff);
-                break;
-            }
-            case RKFM_EDIT_SMOOTH_SUBTRACT: {
-                float kk = fmaxf(op.blend_k, 0.001f);
-                float eff = shape_d + (1.0f - strength) * E.max_extent;
-                nd = -edit_smin(-existing, eff, kk);
-                break;
-            }
-            case RKFM_EDIT_SMOOTH: nd = existing * (1.0f - strength * 0.3f); break;
-            case RKFM_EDIT_FLATTEN: nd = existing + (ly - existing) * (strength * 0.5f); break;
-            case RKFM_EDIT_PAINT:
-                if (fabsf(existing) < op.dimensions[0] * 2.0f) {
-                    nm = op.material_id & 0x3Fu;
-                    ax = 0u;
-                    any = true;
-                } else {
-                    write = false;
-                }
-                break;
-            default: write = false; break;  // color paint: no geometry or material change
-            }
-            if (!write) continue;
-            if (op.edit_type != RKFM_EDIT_PAINT) any |= fabsf(nd - existing) > thr;
-            dbits = (uint16_t)f2h(nd);
-            m = nm;
+                continue;  // uniform per wavefront
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                edit_voxel(E, bx + (float)j * vs + vs * 0.5f, py, pz, thr, db[j], m[j], ax[j], any);
         }
-        dist[vi] = dbits;
-        mat[vi] = (uint8_t)m;
-        if (aux) aux[vi] = (uint16_t)ax;
-        if (any) atomicOr(&changed, 1u);
-        __syncthreads();
-        if (t == 0) out_mod[e] = changed ? 1 : 0;
-        __syncthreads();
+        uint32_t o[4], oa[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o[i] = (db[2 * i] & 0xFFFFu) | (db[2 * i + 1] << 16);
+            oa[i] = (ax[2 * i] & 0xFFFFu) | (ax[2 * i + 1] << 16);
+        }
+        *reinterpret_cast<uint4 *>(dist + v0) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint2 *>(mat + v0) = make_uint2(m[0] | (m[1] << 8) | (m[2] << 16) | (m[3] << 24),
+                                                          m[4] | (m[5] << 8) | (m[6] << 16) | (m[7] << 24));
+        if (aux) *reinterpret_cast<uint4 *>(aux + v0) = make_uint4(oa[0], oa[1], oa[2], oa[3]);
+        const bool changed = __ballot(any) != 0ull;
+        if (lane == 0u) out_mod[e] = changed ? 1 : 0;
     }
 }
 
@@ -348,8 +368,9 @@ extern "C" int rkfm_sparse_edit(rkfm_ctx *c, rkfm_sparse_scene *s, const rkfm_ed
     hipLaunchKernelGGL(rkf::edit_reset_kernel, dim3(1), dim3(1), 0, st, count);
     hipLaunchKernelGGL(rkf::edit_collect_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, D, dops,
                        n_ops, (uint32_t)total, touched, list, count);
-    const uint32_t apply_blocks = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total, I.n_bricks), 8192u);
-    hipLaunchKernelGGL(rkf::edit_apply_kernel, dim3(apply_blocks), dim3(512), 0, st, D, dops, n_ops,
+    const uint64_t max_list = std::min<uint64_t>(total, I.n_bricks);
+    const uint32_t apply_blocks = (uint32_t)std::min<uint64_t>((max_list + 3u) / 4u, 16384u);
+    hipLaunchKernelGGL(rkf::edit_apply_kernel, dim3(apply_blocks), dim3(256), 0, st, D, dops, n_ops,
                        (const rkf::EditBrick *)list, (const uint32_t *)count, (uint16_t *)s->dist.ptr,
                        (uint8_t *)s->mat.ptr, (uint16_t *)s->aux.ptr, (uint8_t *)c->edit_mod.ptr);
     hipLaunchKernelGGL(rkf::edit_clear_kernel, dim3(256), dim3(256), 0, st, (const rkf::EditBrick *)list,
