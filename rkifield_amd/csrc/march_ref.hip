@@ -397,8 +397,10 @@ RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t oi) {
     return normalize(mk(nx, ny, nz));
 }
 
-// ray_march.wgsl:1036-1111. Block = 64 lanes = 8x8 pixels.
-__global__ __launch_bounds__(64) void ray_march_ref_kernel(RefParams P) {
+// ray_march.wgsl:1036-1111. Block = 64 lanes = 8x8 pixels. Held to 128 VGPRs
+// (4 waves/SIMD instead of 3; the few spills stay out of the per-object
+// loop): a 640x360 view is 3600 waves, which then all fit the 256 CUs at once.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
     __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
     __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
     const uint32_t lane = threadIdx.x;
