@@ -120,13 +120,15 @@ def run_config2(args, ctx, M, L, scenes):
     """64 uneven batches over the config1 scene in one graph-captured launch
     (autotuned shape) vs the same batches launched one by one."""
     sc, _, su, ext = build_config1(ctx, M, L, scenes, 1920, 1080)
+    ext_v = ext.copy()  # views: per-tile ray intervals, as in the 1080p frame
+    ext_v["march_flags"] = 0 if args.no_cull else L.MARCH_TILE_CULL
     views, sh_orig, pts = scenes.mixed_batches(seed=2)
     b = M.Batch(ctx)
     singles = []
     for (w, h, pos, tgt) in views:
         cam = M.camera_look_at(pos, tgt, 60.0, w, h)
         gb = M.GBuffer(ctx, w, h, shadow=True)
-        b.add_view(sc, cam, su, ext, gb)
+        b.add_view(sc, cam, su, ext_v, gb)
         singles.append(("view", cam, gb))
     for o in sh_orig:
         od, vd = ctx.to_device(o), ctx.array((o.shape[0],), np.float32)
@@ -156,7 +158,7 @@ def run_config2(args, ctx, M, L, scenes):
     def one_by_one():
         for kind, a, bb in singles:
             if kind == "view":
-                sc.march(a, su, ext, bb, variant=0)
+                sc.march(a, su, ext_v, bb, variant=0)
             elif kind == "shadow":
                 a.run(tuning=(L.SCHEDULE_GRID, 8, 8, 0))
             else:
@@ -177,7 +179,8 @@ def run_config2(args, ctx, M, L, scenes):
         "config": {"workload": "config2: 16 views W~U{256..1920} H~U{256..1080} (+ shadow ray per hit), "
                                "16 shadow batches and 32 point batches of U{1k..64k}, config1 scene",
                    "tuning": {k: int(best[k]) for k in ("schedule", "tile_w", "tile_h", "waves_per_cu")},
-                   "work_items": work, "one_launch_per_batch_ms": round(ms_single, 4)},
+                   "tile_cull": not args.no_cull, "work_items": work,
+                   "one_launch_per_batch_ms": round(ms_single, 4)},
     }
     print(json.dumps(line), flush=True)
 

@@ -571,7 +571,11 @@ typedef struct rkfm_batch_item {
     /* view */
     rkf_camera_uniforms camera;
     rkf_scene_uniforms su;
-    rkfm_scene_ext ext;               /* ext.light_dir etc. also used by shadow batches */
+    rkfm_scene_ext ext;               /* ext.light_dir etc. also used by shadow batches;
+                                         march_flags: views may set RKFM_MARCH_TILE_CULL
+                                         (the tile intervals are computed in the same
+                                         launch sequence), other kinds and the request
+                                         server take 0 */
     rkfm_gbuffer out;
     /* shadow: origins (3 floats each, device) -> visibility (device) */
     const float *origins;

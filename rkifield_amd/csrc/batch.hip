@@ -109,7 +109,7 @@ __global__ void batch_reset_kernel(uint32_t *counter) { *counter = 0u; }
 
 struct rkfm_batch_graph {
     rkfm_ctx *ctx = nullptr;
-    DevBuf items, begins, counter;
+    DevBuf items, begins, counter, tiles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -133,7 +133,9 @@ bool rkf_batch_valid_shape(const rkfm_batch_tuning &t) {
 // skips the check that every scene is resident in the context (the caller
 // has checked it).
 int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning &t,
-            std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total) {
+            std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total,
+            std::vector<uint32_t> *cull_views) {
+    if (cull_views) cull_views->clear();
     if (!items || n == 0) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch: no items");
     if (!rkf_batch_valid_shape(t)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch: invalid launch shape");
     dev.assign(n, rkf::BatchDev{});
@@ -147,8 +149,9 @@ int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, con
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: scene not resident in this context", i);
         if (it.ext.hit_eps_mode > RKFM_EPS_RELATIVE)
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: hit_eps_mode", i);
-        if (it.ext.march_flags)
-            return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: march_flags 0x%x not supported in batches", i,
+        const bool cull = it.ext.march_flags == RKFM_MARCH_TILE_CULL && it.kind == RKFM_BATCH_VIEW && cull_views;
+        if (it.ext.march_flags && !cull)
+            return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: march_flags 0x%x not supported here", i,
                                  it.ext.march_flags);
         d.P.S = it.scene->dev();
         d.P.cam = it.camera;
@@ -165,6 +168,11 @@ int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, con
             if (rc) return rkf_set_error(rc, "batch item %u: %s", i, rkfm_last_error_string());
             d.tiles_x = (it.out.width + t.tile_w - 1) / t.tile_w;
             units = (uint64_t)d.tiles_x * ((it.out.height + t.tile_h - 1) / t.tile_h);
+            if (cull) {
+                rc = rkf_tile_camera_check(&it.camera);
+                if (rc) return rkf_set_error(rc, "batch item %u: %s", i, rkfm_last_error_string());
+                cull_views->push_back(i);
+            }
         } else if (it.kind == RKFM_BATCH_SHADOW) {
             if (it.n && (!it.origins || !it.visibility))
                 return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: null shadow buffers", i);
@@ -233,6 +241,58 @@ rkfm_batch_tuning rkf_batch_default_tuning(const rkfm_ctx *c) {
     return t;
 }
 
+namespace {
+
+// Device layout of the tile intervals of a batch's culled views: each view's
+// (t_lo, t_hi) per 8x8 tile, then one large-rectangle list and its counter,
+// which the views share (they are binned one after another on one stream).
+struct CullLayout {
+    std::vector<uint64_t> off;
+    uint64_t big_off = 0, count_off = 0, bytes = 0;
+};
+
+CullLayout cull_layout(const rkfm_batch_item *items, const std::vector<uint32_t> &views) {
+    CullLayout L;
+    uint64_t acc = 0, nb = 0;
+    for (uint32_t i : views) {
+        const rkfm_batch_item &it = items[i];
+        L.off.push_back(acc);
+        acc += (uint64_t)((it.out.width + 7u) / 8u) * ((it.out.height + 7u) / 8u) * sizeof(uint2);
+        nb = std::max<uint64_t>(nb, it.scene->info.n_blocks);
+    }
+    L.big_off = (acc + 255u) & ~255ull;
+    L.count_off = L.big_off + ((nb * sizeof(rkf::TileBig) + 64u + 255u) & ~255ull);
+    L.bytes = L.count_off + 256u;
+    return L;
+}
+
+// Points the culled views' descriptors at their tile arrays under `base`.
+void cull_patch(std::vector<rkf::BatchDev> &dev, const rkfm_batch_item *items, const std::vector<uint32_t> &views,
+                const CullLayout &L, unsigned char *base) {
+    for (size_t k = 0; k < views.size(); k++) {
+        rkf::BatchDev &d = dev[views[k]];
+        d.P.tiles = (const uint2 *)(base + L.off[k]);
+        d.P.tiles_x = (items[views[k]].out.width + 7u) / 8u;
+    }
+}
+
+// The tile-interval kernels of every culled view (the plain path's own
+// launches, so a culled view marches exactly as it does alone).
+int cull_enqueue(const rkfm_batch_item *items, const std::vector<uint32_t> &views, const CullLayout &L,
+                 unsigned char *base, hipStream_t st) {
+    for (size_t k = 0; k < views.size(); k++) {
+        const rkfm_batch_item &it = items[views[k]];
+        const rkf::SparseDev D = it.scene->dev();
+        hipError_t e = rkf_launch_sparse_tiles(&D, &it.camera, it.su.max_distance, it.out.width, it.out.height,
+                                               (uint2 *)(base + L.off[k]), (rkf::TileBig *)(base + L.big_off),
+                                               (uint32_t *)(base + L.count_off), st);
+        if (e != hipSuccess) return rkf_hip_check(e, "batch tile intervals");
+    }
+    return RKFM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 uint64_t rkfm_batch_work_items(const rkfm_batch_item *items, uint32_t n) {
@@ -247,11 +307,18 @@ int rkfm_batch_run(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const 
     if (!c) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch_run: null ctx");
     rkfm_batch_tuning t = tuning ? *tuning : rkf_batch_default_tuning(c);
     std::vector<rkf::BatchDev> dev;
-    std::vector<uint32_t> begins;
+    std::vector<uint32_t> begins, cull;
     uint32_t total = 0;
-    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total);
+    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total, &cull);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    CullLayout cl;
+    if (!cull.empty()) {
+        cl = cull_layout(items, cull);
+        rc = c->batch_tiles.reserve(cl.bytes, false);
+        if (rc) return rc;
+        cull_patch(dev, items, cull, cl, (unsigned char *)c->batch_tiles.ptr);
+    }
     // The descriptors are rewritten per call: order against earlier launches
     // that read them (they ran on this stream or were synchronised).
     uint32_t waves = 0;
@@ -274,6 +341,7 @@ int rkfm_batch_run(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const 
         rc = rkf_hip_check(hipMemcpyAsync(c->batch_begins.ptr, c->batch_host_begins.data(),
                                           c->batch_host_begins.size() * 4, hipMemcpyHostToDevice, st),
                            "batch begins");
+    if (!rc && !cull.empty()) rc = cull_enqueue(items, cull, cl, (unsigned char *)c->batch_tiles.ptr, st);
     if (rc) return rc;
     return rkf_batch_enqueue(t, (const rkf::BatchDev *)c->batch_items.ptr, (const uint32_t *)c->batch_begins.ptr, n, total,
                    (uint32_t *)c->batch_counter.ptr, waves, st);
@@ -320,9 +388,9 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
     *out = nullptr;
     rkfm_batch_tuning t = tuning ? *tuning : rkf_batch_default_tuning(c);
     std::vector<rkf::BatchDev> dev;
-    std::vector<uint32_t> begins;
+    std::vector<uint32_t> begins, cull;
     uint32_t total = 0;
-    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total);
+    int rc = rkf_batch_prepare(c, items, n, t, dev, begins, total, &cull);
     if (rc) return rc;
     uint32_t waves = 0;
     if (t.schedule == RKFM_SCHEDULE_PERSISTENT) {
@@ -331,7 +399,13 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
     }
     rkfm_batch_graph *g = new rkfm_batch_graph();
     g->ctx = c;
-    rc = g->items.reserve(dev.size() * sizeof(rkf::BatchDev), false);
+    CullLayout cl;
+    if (!cull.empty()) {
+        cl = cull_layout(items, cull);
+        rc = g->tiles.reserve(cl.bytes, false);
+        if (!rc) cull_patch(dev, items, cull, cl, (unsigned char *)g->tiles.ptr);
+    }
+    if (!rc) rc = g->items.reserve(dev.size() * sizeof(rkf::BatchDev), false);
     if (!rc) rc = g->begins.reserve(begins.size() * 4, false);
     if (!rc) rc = g->counter.reserve(256, false);
     if (!rc)
@@ -344,8 +418,10 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
     if (!rc) rc = rkf_hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
     if (!rc) rc = rkf_hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
     if (!rc) {
-        int rc2 = rkf_batch_enqueue(t, (const rkf::BatchDev *)g->items.ptr, (const uint32_t *)g->begins.ptr, n, total,
-                          (uint32_t *)g->counter.ptr, waves, cap);
+        int rc2 = cull.empty() ? RKFM_OK : cull_enqueue(items, cull, cl, (unsigned char *)g->tiles.ptr, cap);
+        if (!rc2)
+            rc2 = rkf_batch_enqueue(t, (const rkf::BatchDev *)g->items.ptr, (const uint32_t *)g->begins.ptr, n, total,
+                                    (uint32_t *)g->counter.ptr, waves, cap);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(cap, &graph);
         rc = rc2 ? rc2 : rkf_hip_check(e, "end capture");
@@ -375,6 +451,7 @@ int rkfm_batch_graph_destroy(rkfm_batch_graph *g) {
     g->items.release();
     g->begins.release();
     g->counter.release();
+    g->tiles.release();
     delete g;
     return RKFM_OK;
 }

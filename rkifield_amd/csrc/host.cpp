@@ -110,6 +110,7 @@ int rkfm_destroy(rkfm_ctx *c) {
     c->batch_items.release();
     c->batch_begins.release();
     c->batch_counter.release();
+    c->batch_tiles.release();
     c->march_hits.release();
     c->march_count.release();
     c->tile_iv.release();

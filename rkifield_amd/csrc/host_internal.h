@@ -66,6 +66,7 @@ struct rkfm_ctx {
     // Batched launch: device descriptors (rewritten per call, stream-ordered),
     // their host staging copies, the persistent work counter, the tuning.
     DevBuf batch_items, batch_begins, batch_counter;
+    DevBuf batch_tiles;  // tile intervals of the culled views (layout: batch.hip)
     std::vector<unsigned char> batch_host_items;
     std::vector<uint32_t> batch_host_begins;
     rkfm_batch_tuning batch_tuning{};
@@ -78,8 +79,13 @@ void rkf_sparse_release_all(rkfm_ctx *c);
 bool rkf_batch_valid_shape(const rkfm_batch_tuning &t);
 rkfm_batch_tuning rkf_batch_default_tuning(const rkfm_ctx *c);
 // Device descriptors and unit begins of a batch set; *total = work units.
+// cull_views receives the views that ask for RKFM_MARCH_TILE_CULL; without
+// it (the request server) any march flag is an error.
 int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const rkfm_batch_tuning &t,
-                      std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total);
+                      std::vector<rkf::BatchDev> &dev, std::vector<uint32_t> &begins, uint32_t &total,
+                      std::vector<uint32_t> *cull_views = nullptr);
+// The camera checks of the tile-interval kernels (jitter, invertible basis).
+extern "C" int rkf_tile_camera_check(const rkf_camera_uniforms *cam);
 // Persistent-grid size: per_cu wavefronts on every CU of the current device.
 int rkf_device_waves(uint32_t per_cu, uint32_t &waves);
 // Enqueues the batch kernel (and the counter reset of the persistent

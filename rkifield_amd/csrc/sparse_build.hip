@@ -1035,8 +1035,7 @@ int rkfm_sparse_download_free(rkfm_ctx *c, const rkfm_sparse_scene *s, uint32_t 
 
 // Tile intervals into `tiles` (the context's large-rectangle list as
 // scratch; stream-ordered, so successive calls on one stream may share it).
-static int tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
-                          const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint2 *tiles, hipStream_t st) {
+int rkf_tile_camera_check(const rkf_camera_uniforms *cam) {
     if (!(fabsf(cam->jitter[0]) <= 0.5f && fabsf(cam->jitter[1]) <= 0.5f))
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: |jitter| above 0.5");
     {
@@ -1052,7 +1051,13 @@ static int tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_cam
         if (!std::isfinite(det) || fabs(det) < 1e-12)
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: degenerate camera basis");
     }
-    int rc = c->tile_big.reserve((uint64_t)s->info.n_blocks * sizeof(rkf::TileBig) + 64, false);
+    return RKFM_OK;
+}
+
+static int tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                          const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint2 *tiles, hipStream_t st) {
+    int rc = rkf_tile_camera_check(cam);
+    if (!rc) rc = c->tile_big.reserve((uint64_t)s->info.n_blocks * sizeof(rkf::TileBig) + 64, false);
     if (!rc) rc = c->tile_big_count.reserve(256, false);
     if (rc) return rc;
     rkf::SparseDev D = s->dev();

@@ -879,27 +879,23 @@ RKF_DEV void shadow_store(const SparseMarchParams &P, uint64_t p, const ShadowRa
 template <bool kLowLat>
 RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
     uint2 iv = make_uint2(0u, 0u);
-    if constexpr (kLowLat) {
-        if (P.tiles) {
-            iv = P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)];
-            if (iv.x == 0x7F800000u) {
-                // a tile no occupied cell projects to: prim_cull sets t = +inf
-                // and the first prim_step misses, so write that miss directly
-                const uint64_t p = (uint64_t)py * P.g.width + px;
-                store_miss(P.g, p);
-                store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, 1u, 0u);
-                if (P.g.shadow) P.g.shadow[p] = 0.0f;
-                return;
-            }
+    if (P.tiles) {
+        iv = P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)];
+        if (iv.x == 0x7F800000u) {
+            // a tile no occupied cell projects to: prim_cull sets t = +inf
+            // and the first prim_step misses, so write that miss directly
+            const uint64_t p = (uint64_t)py * P.g.width + px;
+            store_miss(P.g, p);
+            store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, 1u, 0u);
+            if (P.g.shadow) P.g.shadow[p] = 0.0f;
+            return;
         }
     }
     f3 o, dir;
     camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
     PrimRay r;
     prim_init(r, o, dir, P.S.vs, P.su.max_distance);
-    if constexpr (kLowLat) {
-        if (P.tiles) prim_cull(r, iv, P.S.vs);
-    }
+    if (P.tiles) prim_cull(r, iv, P.S.vs);
     SpRes res;
     int st;
     do {

@@ -18,6 +18,7 @@ here invalid inputs are reported instead of rendered.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -151,9 +152,12 @@ class Context:
         h = _VP()
         _check(lib().rkfm_create(_p(cfg), C.byref(h)))
         self.h = h.value
+        self.graphs = weakref.WeakSet()  # batch graphs, destroyed before the context
 
     def close(self):
         if getattr(self, "h", None):
+            for g in list(getattr(self, "graphs", ())):
+                g.destroy()
             lib().rkfm_destroy(self.h)
             self.h = None
 
@@ -617,6 +621,7 @@ class BatchGraph:
     def __init__(self, batch, handle):
         self.batch = batch
         self.h = handle
+        batch.ctx.graphs.add(self)  # a graph refers to its context: destroyed first
 
     def launch(self, stream=None):
         _check(lib().rkfm_batch_graph_launch(self.h, stream))

@@ -33,22 +33,24 @@ def two_scenes(ctx):
         s.destroy()
 
 
-def _ext(shadows):
+def _ext(shadows, cull=False):
     light = np.array([0.5, 1.0, 0.3])
     return L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1 if shadows else 0,
-                       light_dir=light / np.linalg.norm(light))
+                       light_dir=light / np.linalg.norm(light), march_flags=L.MARCH_TILE_CULL if cull else 0)
 
 
 def _make(ctx, two_scenes):
     rng = np.random.default_rng(5)
     b = M.Batch(ctx)
     views = []
-    for k, (w, h, sc, sh) in enumerate([(72, 40, 0, True), (33, 57, 1, False), (64, 64, 1, True)]):
+    # views 0 and 3 with per-tile ray intervals: each must march exactly as it does alone
+    for k, (w, h, sc, sh, cull) in enumerate([(72, 40, 0, True, True), (33, 57, 1, False, False),
+                                              (64, 64, 1, True, False), (50, 30, 1, True, True)]):
         cam = M.camera_look_at(rng.uniform(4, 7, 3) * rng.choice([-1, 1], 3), (0, 0, 0), 60.0, w, h)
         su = L.scene_uniforms(num_objects=1)
         gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
-        b.add_view(two_scenes[sc], cam, su, _ext(sh), gb)
-        views.append((two_scenes[sc], cam, su, _ext(sh), w, h, gb))
+        b.add_view(two_scenes[sc], cam, su, _ext(sh, cull), gb)
+        views.append((two_scenes[sc], cam, su, _ext(sh, cull), w, h, gb))
     shadows = []
     for sc, n in ((0, 1000), (1, 77)):
         o = rng.uniform(-4, 4, (n, 3)).astype(np.float32)
@@ -154,3 +156,9 @@ def test_batch_errors(ctx, two_scenes):
         b.run(tuning=(L.SCHEDULE_GRID, 8, 4, 0))  # 32 lanes: invalid packet
     with pytest.raises(M.RkfMarchError):
         b.run(tuning=(3, 8, 8, 0))  # no such schedule
+    # tile culling is a view option only
+    b2 = M.Batch(ctx)
+    vd = ctx.array((4,), np.float32)
+    b2.add_shadow(two_scenes[0], ctx.to_device(np.zeros((4, 3), np.float32)), vd, 4, _ext(True, cull=True))
+    with pytest.raises(M.RkfMarchError):
+        b2.run()
