@@ -611,9 +611,10 @@ int rkfm_batch_autotune(rkfm_ctx *ctx, const rkfm_batch_item *items,
                         uint32_t n, uint32_t reps, rkfm_batch_tuning *best,
                         float *best_ms);
 
-/* A batch set captured once into a HIP graph (work-counter reset + launch);
- * replays re-run the same batches (their input buffers may change between
- * replays; outputs are rewritten). */
+/* A batch set captured once into a HIP graph (tile intervals of culled views,
+ * work-counter reset, launch); replays re-run the same batches (their input
+ * buffers may change between replays; outputs are rewritten). A graph uses
+ * its context's stream and scenes: destroy it before the context. */
 typedef struct rkfm_batch_graph rkfm_batch_graph;
 int rkfm_batch_graph_create(rkfm_ctx *ctx, const rkfm_batch_item *items,
                             uint32_t n, const rkfm_batch_tuning *tuning,
