@@ -291,6 +291,32 @@ int rkfm_voxelize_sphere(float radius, float margin, float voxel_size,
                          uint32_t dims[3], uint32_t *map_entries,
                          rkf_voxel_sample *bricks, uint64_t *n_bricks);
 
+/* Brick classification and voxel contents for rkfm_voxelize_prim.
+ *   RKFM_VOX_GEOMETRY_FIRST: voxelize_to_geometry's classification
+ *     (voxelize_object.rs:217-385): a brick whose centre distance is below
+ *     -1.8 brick extents is INTERIOR_SLOT; a narrow-band brick whose 512
+ *     voxel-centre distances are all <= 0 is INTERIOR_SLOT, all > 0
+ *     EMPTY_SLOT; the other narrow-band bricks are stored (VoxelSample words
+ *     as voxelize_sdf writes them) and take consecutive slots in z-y-x order
+ *     (the reference's pool frees the slots of the bricks it drops, leaving
+ *     holes; here the numbering is compact).
+ *   RKFM_VOX_BLEND (with GEOMETRY_FIRST): each stored voxel also carries a
+ *     secondary material and blend weight (VoxelSample::new_blended,
+ *     voxel.rs:43-55) from a second hash of its key: secondary =
+ *     (h >> 8) & 63, blend = (h >> 16) & 255. */
+#define RKFM_VOX_GEOMETRY_FIRST 1u
+#define RKFM_VOX_BLEND 2u
+
+/* voxelize_sdf (flags 0) or the classification above of one primitive (type
+ * 0-2; centre and rot place it in the object's local space) over the AABB
+ * centre +- (bounding radius + margin); per-voxel material and sizing call
+ * as rkfm_voxelize_sphere. */
+int rkfm_voxelize_prim(const rkfm_prim *prim, float margin, float voxel_size,
+                       uint32_t flags, uint32_t object_index, uint64_t mat_seed,
+                       uint32_t material, uint32_t first_slot, uint32_t dims[3],
+                       uint32_t *map_entries, rkf_voxel_sample *bricks,
+                       uint64_t *n_bricks);
+
 /* The 5-sphere smooth union of crates/rkf-core/examples/gen_test_rkf.rs:22-127
  * (vs 0.03, k 0.15, material 6), voxelised the same way. */
 int rkfm_voxelize_test_cross(uint32_t dims[3], uint32_t *map_entries,
@@ -614,7 +640,9 @@ int rkfm_batch_autotune(rkfm_ctx *ctx, const rkfm_batch_item *items,
 /* A batch set captured once into a HIP graph (tile intervals of culled views,
  * work-counter reset, launch); replays re-run the same batches (their input
  * buffers may change between replays; outputs are rewritten). A graph uses
- * its context's stream and scenes: destroy it before the context. */
+ * its context's scenes: destroy it before the context. Replays may run on any
+ * stream; destroy waits for the last replay (an event recorded by every
+ * launch) before it frees the graph's buffers. */
 typedef struct rkfm_batch_graph rkfm_batch_graph;
 int rkfm_batch_graph_create(rkfm_ctx *ctx, const rkfm_batch_item *items,
                             uint32_t n, const rkfm_batch_tuning *tuning,

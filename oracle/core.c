@@ -294,6 +294,87 @@ int orc_voxelize_sdf(orc_sdf_fn sdf, void *user, const float amin[3], const floa
     return 0;
 }
 
+/* ---- voxelize_to_geometry's brick classification (voxelize_object.rs:217-385) ----
+ * Brick states from the brick-centre distance: deeper than the narrow band
+ * -> INTERIOR; outside the band -> EMPTY; inside the band -> sampled at its
+ * 512 voxel centres: no solid voxel (d <= 0) -> EMPTY, all solid -> INTERIOR,
+ * else stored. The voxel words are voxelize_sdf's (orc_voxel_new of the
+ * distance and material the callback returns); stored bricks are numbered
+ * first_slot, first_slot + 1, ... in z-y-x order. */
+int orc_voxelize_geometry(orc_sdf_fn sdf, void *user, const float amin[3], const float amax[3], float vs,
+                          uint32_t first_slot, uint32_t dims[3], uint32_t *map, rkf_voxel_sample *bricks,
+                          uint64_t *n_bricks) {
+    float bws = vs * 8.0f;
+    for (int a = 0; a < 3; a++) {
+        float c = ceilf((amax[a] - amin[a]) / bws);
+        uint32_t d = c > 0.0f ? (uint32_t)c : 0u;
+        dims[a] = d < 1 ? 1 : d;
+    }
+    float go[3];
+    for (int a = 0; a < 3; a++) go[a] = -((float)dims[a] * bws * 0.5f);
+    float band = bws * 1.8f, half = vs * 0.5f;
+    uint64_t nmap = (uint64_t)dims[0] * dims[1] * dims[2], stored = 0;
+    uint32_t *state = (uint32_t *)malloc(nmap * sizeof(uint32_t));
+    if (!state) return -1;
+    uint16_t mat;
+    uint64_t mi = 0;
+    for (uint32_t bz = 0; bz < dims[2]; bz++)
+        for (uint32_t by = 0; by < dims[1]; by++)
+            for (uint32_t bx = 0; bx < dims[0]; bx++, mi++) {
+                float bmin[3] = {go[0] + (float)bx * bws, go[1] + (float)by * bws, go[2] + (float)bz * bws};
+                float hb = bws * 0.5f;
+                float c[3] = {bmin[0] + hb, bmin[1] + hb, bmin[2] + hb};
+                float d = sdf(c, user, &mat);
+                if (d < -band) { state[mi] = RKF_INTERIOR_SLOT; continue; }
+                if (!(fabsf(d) < band)) { state[mi] = RKF_EMPTY_SLOT; continue; }
+                int any_solid = 0, all_solid = 1;
+                for (uint32_t vz = 0; vz < 8; vz++)
+                    for (uint32_t vy = 0; vy < 8; vy++)
+                        for (uint32_t vx = 0; vx < 8; vx++) {
+                            float p[3] = {bmin[0] + ((float)vx * vs + half), bmin[1] + ((float)vy * vs + half),
+                                          bmin[2] + ((float)vz * vs + half)};
+                            int solid = sdf(p, user, &mat) <= 0.0f;
+                            any_solid |= solid;
+                            all_solid &= solid;
+                        }
+                state[mi] = !any_solid ? RKF_EMPTY_SLOT : all_solid ? RKF_INTERIOR_SLOT : 0u;
+                stored += state[mi] == 0u;
+            }
+    if (n_bricks) *n_bricks = stored;
+    if (map) {
+        uint32_t slot = first_slot;
+        uint64_t local = 0;
+        mi = 0;
+        for (uint32_t bz = 0; bz < dims[2]; bz++)
+            for (uint32_t by = 0; by < dims[1]; by++)
+                for (uint32_t bx = 0; bx < dims[0]; bx++, mi++) {
+                    if (state[mi] != 0u) { map[mi] = state[mi]; continue; }
+                    map[mi] = slot++;
+                    rkf_voxel_sample *b = bricks ? bricks + local * 512 : NULL;
+                    local++;
+                    if (!b) continue;
+                    float bmin[3] = {go[0] + (float)bx * bws, go[1] + (float)by * bws, go[2] + (float)bz * bws};
+                    for (uint32_t vz = 0; vz < 8; vz++)
+                        for (uint32_t vy = 0; vy < 8; vy++)
+                            for (uint32_t vx = 0; vx < 8; vx++) {
+                                float p[3] = {bmin[0] + ((float)vx * vs + half), bmin[1] + ((float)vy * vs + half),
+                                              bmin[2] + ((float)vz * vs + half)};
+                                float d = sdf(p, user, &mat);
+                                b[vx + vy * 8 + vz * 64] = orc_voxel_new(d, mat, 0);
+                            }
+                }
+    }
+    free(state);
+    return 0;
+}
+
+/* orc_sdf_fn over one rkfm_prim (user), material = prim->material. */
+float orc_prim_sdf(const float p[3], void *user, uint16_t *mat) {
+    const rkfm_prim *pr = (const rkfm_prim *)user;
+    *mat = (uint16_t)pr->material;
+    return orc_prim_eval(pr, p);
+}
+
 /* gen_test_rkf.rs:28-56 */
 static const float CROSS_C[5][3] = {{0.0f, 0.0f, 0.0f}, {0.55f, 0.0f, 0.0f}, {-0.55f, 0.0f, 0.0f},
                                     {0.0f, 0.0f, 0.55f}, {0.0f, 0.0f, -0.55f}};

@@ -96,6 +96,12 @@ int orc_voxelize_sdf(orc_sdf_fn sdf, void *user, const float aabb_min[3],
                      const float aabb_max[3], float voxel_size,
                      uint32_t first_slot, uint32_t dims[3], uint32_t *map,
                      rkf_voxel_sample *bricks, uint64_t *n_bricks);
+/* voxelize_to_geometry's classification (INTERIOR / EMPTY / stored bricks,
+ * voxelize_object.rs:217-385) over voxelize_sdf's grid and voxel words. */
+int orc_voxelize_geometry(orc_sdf_fn sdf, void *user, const float aabb_min[3], const float aabb_max[3],
+                          float voxel_size, uint32_t first_slot, uint32_t dims[3], uint32_t *map,
+                          rkf_voxel_sample *bricks, uint64_t *n_bricks);
+float orc_prim_sdf(const float p[3], void *user, uint16_t *mat);
 float orc_test_cross_sdf(const float p[3], void *user, uint16_t *mat);
 void orc_test_cross_aabb(float mn[3], float mx[3]);
 
@@ -162,12 +168,31 @@ typedef struct orc_sparse_scene {
     const uint32_t *bext;      /* per L2 cell x 512 bricks: 8 brick octant
                                   extents (4 bits each); used in cells that
                                   are not entirely free */
+    const uint16_t *aux;       /* per slot x 512: secondary | blend << 8, or
+                                  NULL (both 0) */
 } orc_sparse_scene;
 
 /* Unpacked per-ray result. For the FP32 twin all values are exactly the
  * float results widened to double. */
+/* Branches a ray's evaluations took (orc_ray_result.flags): a stepping
+ * sample read an INTERIOR corner (-2 vs), a gradient tap read one (-8 vs),
+ * a geometry-AABB early-out returned, the overstep bisection ran, the hit
+ * object is analytic. */
+#define ORC_FLAG_INTERIOR_STEP 1u
+#define ORC_FLAG_INTERIOR_GRAD 2u
+#define ORC_FLAG_GEOMETRY_EARLY_OUT 4u
+#define ORC_FLAG_BISECT 8u
+#define ORC_FLAG_ANALYTIC_HIT 16u
+
+/* u32(f32(x)) with WGSL's saturating conversion. */
+static inline uint32_t orc_u32_via_f32(uint32_t x) {
+    float f = (float)x;
+    return f >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)f;
+}
+
 typedef struct orc_ray_result {
     int32_t hit;
+    uint32_t flags;            /* ORC_FLAG_* */
     double t;
     double pos[3];
     double normal[3];

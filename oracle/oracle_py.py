@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 RAY_RESULT = np.dtype([
-    ("hit", "<i4"), ("_pad0", "<i4"),
+    ("hit", "<i4"), ("flags", "<u4"),
     ("t", "<f8"),
     ("pos", "<f8", (3,)),
     ("normal", "<f8", (3,)),
@@ -60,6 +60,7 @@ class SparseScene(C.Structure):
         ("cells", C.c_uint32 * 3),
         ("l2", C.c_void_p), ("l1", C.c_void_p), ("dist", C.c_void_p), ("mat", C.c_void_p),
         ("free_l2", C.c_void_p), ("oct", C.c_void_p), ("bext", C.c_void_p),
+        ("aux", C.c_void_p),
     ]
 
 
@@ -86,6 +87,7 @@ def lib():
         _LIB.orc_smin.argtypes = [C.c_float, C.c_float, C.c_float]
         _LIB.orc_voxelize_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _LIB.orc_voxelize_geometry.argtypes = _LIB.orc_voxelize_sdf.argtypes
     return _LIB
 
 
@@ -264,7 +266,8 @@ def ref_trace_parallel(inp: RefInputs, threads=0):
 class SparseInputs:
     """A sparse scene as host arrays (l2, l1, dist f16 bits, mat u8)."""
 
-    def __init__(self, origin, voxel_size, cells, l2, l1, dist, mat):
+    def __init__(self, origin, voxel_size, cells, l2, l1, dist, mat, aux=None):
+        self.aux = None if aux is None else np.ascontiguousarray(aux, np.uint16)
         self.l2 = np.ascontiguousarray(l2, np.uint32)
         self.l1 = np.ascontiguousarray(l1, np.uint32)
         self.dist = np.ascontiguousarray(dist, np.uint16)
@@ -285,6 +288,7 @@ class SparseInputs:
         s.free_l2 = _p(self.free_l2).value
         s.oct = _p(self.oct).value
         s.bext = _p(self.bext).value
+        s.aux = _p(self.aux).value
         self.s = s
         lib().orc_sparse_free_l2(C.byref(s), _p(self.free_l2))
         lib().orc_sparse_oct(C.byref(s), _p(self.free_l2), _p(self.oct))

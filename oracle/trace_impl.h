@@ -70,6 +70,7 @@ static inline R F(hit_eps)(const rkfm_scene_ext *ext, const rkf_scene_uniforms *
 typedef struct {
     const orc_ref_scene *s;
     uint32_t evals;
+    uint32_t flags; /* ORC_FLAG_*: which branches this ray's evaluations took */
 } F(rctx);
 
 static inline uint32_t F(map_entry)(const orc_ref_scene *s, uint64_t i) {
@@ -84,7 +85,7 @@ static inline uint32_t F(pool_word1)(const orc_ref_scene *s, uint64_t i) {
 
 /* :270-288 (grad = 0) and :300-315 (grad = 1) */
 static R F(voxel_at)(const orc_ref_scene *s, uint32_t off, int32_t cx, int32_t cy, int32_t cz,
-                     const uint32_t *dims, const int32_t *tot, R vs, int grad) {
+                     const uint32_t *dims, const int32_t *tot, R vs, int grad, uint32_t *flags) {
     cx = F(clampi)(cx, 0, tot[0] - 1);
     cy = F(clampi)(cy, 0, tot[1] - 1);
     cz = F(clampi)(cz, 0, tot[2] - 1);
@@ -93,7 +94,10 @@ static R F(voxel_at)(const orc_ref_scene *s, uint32_t off, int32_t cx, int32_t c
     uint32_t flat = bx + by * dims[0] + bz * dims[0] * dims[1];
     uint32_t slot = F(map_entry)(s, (uint64_t)off + flat);
     if (slot == RKF_EMPTY_SLOT) return vs * (R)8.0f;
-    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * (R)8.0f) : -(vs * (R)2.0f);
+    if (slot == RKF_INTERIOR_SLOT) {
+        *flags |= grad ? ORC_FLAG_INTERIOR_GRAD : ORC_FLAG_INTERIOR_STEP;
+        return grad ? -(vs * (R)8.0f) : -(vs * (R)2.0f);
+    }
     uint64_t idx = (uint64_t)slot * 512u + lx + ly * 8u + lz * 64u;
     return (R)orc_f16_to_f32((uint16_t)(F(pool_word0)(s, idx) & 0xFFFFu));
 }
@@ -116,7 +120,10 @@ static R F(sample_voxelized)(F(rctx) *c, V3 lp, const rkf_gpu_object *o, int gra
             V3 gc = F(mk)(F(clampr)(lp.x, gmin.x, gmax.x), F(clampr)(lp.y, gmin.y, gmax.y),
                           F(clampr)(lp.z, gmin.z, gmax.z));
             R gd = F(length)(F(sub)(lp, gc));
-            if (gd > be) return gd;
+            if (gd > be) {
+                c->flags |= ORC_FLAG_GEOMETRY_EARLY_OUT;
+                return gd;
+            }
         }
     }
     V3 vc = F(mk)(cl.x / vs - (R)0.5f, cl.y / vs - (R)0.5f, cl.z / vs - (R)0.5f);
@@ -127,14 +134,14 @@ static R F(sample_voxelized)(F(rctx) *c, V3 lp, const rkf_gpu_object *o, int gra
     uint32_t off = o->brick_map_offset;
     const orc_ref_scene *s = c->s;
     if (!grad) c->evals++;
-    R c000 = F(voxel_at)(s, off, x0, y0, z0, dims, tot, vs, grad);
-    R c100 = F(voxel_at)(s, off, x0 + 1, y0, z0, dims, tot, vs, grad);
-    R c010 = F(voxel_at)(s, off, x0, y0 + 1, z0, dims, tot, vs, grad);
-    R c110 = F(voxel_at)(s, off, x0 + 1, y0 + 1, z0, dims, tot, vs, grad);
-    R c001 = F(voxel_at)(s, off, x0, y0, z0 + 1, dims, tot, vs, grad);
-    R c101 = F(voxel_at)(s, off, x0 + 1, y0, z0 + 1, dims, tot, vs, grad);
-    R c011 = F(voxel_at)(s, off, x0, y0 + 1, z0 + 1, dims, tot, vs, grad);
-    R c111 = F(voxel_at)(s, off, x0 + 1, y0 + 1, z0 + 1, dims, tot, vs, grad);
+    R c000 = F(voxel_at)(s, off, x0, y0, z0, dims, tot, vs, grad, &c->flags);
+    R c100 = F(voxel_at)(s, off, x0 + 1, y0, z0, dims, tot, vs, grad, &c->flags);
+    R c010 = F(voxel_at)(s, off, x0, y0 + 1, z0, dims, tot, vs, grad, &c->flags);
+    R c110 = F(voxel_at)(s, off, x0 + 1, y0 + 1, z0, dims, tot, vs, grad, &c->flags);
+    R c001 = F(voxel_at)(s, off, x0, y0, z0 + 1, dims, tot, vs, grad, &c->flags);
+    R c101 = F(voxel_at)(s, off, x0 + 1, y0, z0 + 1, dims, tot, vs, grad, &c->flags);
+    R c011 = F(voxel_at)(s, off, x0, y0 + 1, z0 + 1, dims, tot, vs, grad, &c->flags);
+    R c111 = F(voxel_at)(s, off, x0 + 1, y0 + 1, z0 + 1, dims, tot, vs, grad, &c->flags);
     R c00 = F(mix)(c000, c100, tx), c10 = F(mix)(c010, c110, tx);
     R c01 = F(mix)(c001, c101, tx), c11 = F(mix)(c011, c111, tx);
     R c0 = F(mix)(c00, c10, ty), c1 = F(mix)(c01, c11, ty);
@@ -216,7 +223,9 @@ static R F(eval_object)(F(rctx) *c, V3 wp, uint32_t oi, uint32_t *mat) {
     R d;
     if (o->sdf_type == RKF_SDF_TYPE_ANALYTICAL) {
         d = F(analytical)(lp, o);
-        *mat = o->material_id;
+        /* evaluate_object returns the material as an f32 (:633) that the
+           march converts back with u32() */
+        *mat = orc_u32_via_f32(o->material_id);
     } else {
         d = F(sample_voxelized)(c, lp, o, 0);
         *mat = d < (R)o->voxel_size * (R)16.0f ? F(voxelized_material)(s, lp, o) : 0u;
@@ -263,6 +272,7 @@ typedef struct {
 static void F(bisect)(F(rctx) *c, V3 o, V3 sd, R lo, R hi, uint32_t oi, uint32_t best_mat,
                       uint32_t best_id, F(mres) *r) {
     uint32_t bm = best_mat;
+    c->flags |= ORC_FLAG_BISECT;
     for (uint32_t b = 0; b < K_BISECT; b++) {
         R mid = (lo + hi) * (R)0.5f;
         V3 mp = F(add)(o, F(muls)(sd, mid));
@@ -449,7 +459,7 @@ static void F(motion)(const rkf_camera_uniforms *cam, uint32_t w, uint32_t h, ui
 }
 
 static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out) {
-    F(rctx) c = {s, 0};
+    F(rctx) c = {s, 0, 0};
     uint32_t px = (uint32_t)(p % s->width), py = (uint32_t)(p / s->width);
     V3 o, dir;
     F(camera_ray)(&s->cam, s->width, s->height, px, py, &o, &dir);
@@ -468,6 +478,7 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
     out->cell_index = 0xFFFFFFFFu;
     if (!r.hit) {
         out->evals = c.evals;
+        out->flags = c.flags;
         out->t = (double)K_MAX_FLOAT;
         return;
     }
@@ -491,6 +502,7 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
         sdf_at = F(sample_voxelized)(&c, lh, ho, 0);
     }
     out->evals = evals_march;
+    out->flags = c.flags | (ho->sdf_type == RKF_SDF_TYPE_ANALYTICAL ? ORC_FLAG_ANALYTIC_HIT : 0u);
     out->hit = 1;
     out->t = (double)r.t;
     out->pos[0] = (double)hp.x;
@@ -542,6 +554,7 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
 typedef struct {
     const orc_sparse_scene *s;
     uint32_t evals;
+    uint32_t flags; /* ORC_FLAG_* */
     uint32_t nb[3];
     int32_t tot[3];
 } F(sctx);
@@ -556,13 +569,16 @@ static inline uint32_t F(sp_slot)(const F(sctx) *c, int32_t cx, int32_t cy, int3
     return s->l1[(uint64_t)blk * 512u + (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u];
 }
 
-static R F(sp_voxel)(const F(sctx) *c, int32_t cx, int32_t cy, int32_t cz, R vs, int grad) {
+static R F(sp_voxel)(F(sctx) *c, int32_t cx, int32_t cy, int32_t cz, R vs, int grad) {
     cx = F(clampi)(cx, 0, c->tot[0] - 1);
     cy = F(clampi)(cy, 0, c->tot[1] - 1);
     cz = F(clampi)(cz, 0, c->tot[2] - 1);
     uint32_t slot = F(sp_slot)(c, cx, cy, cz);
     if (slot == RKF_EMPTY_SLOT) return vs * (R)8.0f;
-    if (slot == RKF_INTERIOR_SLOT) return grad ? -(vs * (R)8.0f) : -(vs * (R)2.0f);
+    if (slot == RKF_INTERIOR_SLOT) {
+        c->flags |= grad ? ORC_FLAG_INTERIOR_GRAD : ORC_FLAG_INTERIOR_STEP;
+        return grad ? -(vs * (R)8.0f) : -(vs * (R)2.0f);
+    }
     uint64_t idx = (uint64_t)slot * 512u + (uint32_t)(cx & 7) + (uint32_t)(cy & 7) * 8u + (uint32_t)(cz & 7) * 64u;
     return (R)orc_f16_to_f32(c->s->dist[idx]);
 }
@@ -833,6 +849,7 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
         R d = F(sp_sample)(c, pos, 0, 1);
         if (d < (R)0 && prev_dist > F(hit_eps)(ext, su, prev_t)) {
             R lo = prev_t, hi = t, mt = t, md = d;
+            c->flags |= ORC_FLAG_BISECT;
             for (uint32_t b = 0; b < K_BISECT; b++) {
                 R mid = (lo + hi) * (R)0.5f;
                 R mdd = F(sp_sample)(c, F(add)(o, F(muls)(sd, mid)), 0, 1);
@@ -888,6 +905,7 @@ static R F(sp_shadow)(F(sctx) *c, V3 o, V3 L, R maxd, R k, uint32_t *steps) {
 static void F(sp_init)(F(sctx) *c, const orc_sparse_scene *s) {
     c->s = s;
     c->evals = 0;
+    c->flags = 0;
     for (int a = 0; a < 3; a++) {
         c->nb[a] = s->cells[a] * 8u;
         c->tot[a] = (int32_t)(s->cells[a] * 64u);
@@ -911,6 +929,7 @@ static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
     out->voxel_index = 0xFFFFFFFFu;
     out->cell_index = 0xFFFFFFFFu;
     out->evals = c.evals;
+    out->flags = c.flags;
     if (!r.hit) {
         out->t = (double)K_MAX_FLOAT;
         out->shadow = 0;
@@ -928,11 +947,18 @@ static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
     V3 n = len < (R)1e-10f ? F(mk)(0, 1, 0) : F(mk)(g.x / len, g.y / len, g.z / len);
     uint32_t sl, vx, cell;
     int64_t vi = F(sp_nearest)(&c, hp, &sl, &vx, &cell);
-    (void)vi;
+    /* secondary material and blend weight of the nearest voxel (:570-596),
+       from the aux plane (secondary | blend << 8) when the scene has one */
+    uint32_t sec = 0, blend = 0;
+    if (s->aux && vi >= 0) {
+        sec = s->aux[vi] & 0x3Fu;
+        blend = (uint32_t)(s->aux[vi] >> 8) & 0xFFu;
+    }
     out->brick_slot = sl;
     out->voxel_index = vx;
     out->cell_index = cell;
     R sdf_at = F(sp_sample)(&c, hp, 0, 0);
+    out->flags = c.flags;
     out->hit = 1;
     out->t = (double)r.t;
     out->pos[0] = (double)hp.x;
@@ -942,7 +968,7 @@ static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
     out->normal[1] = (double)n.y;
     out->normal[2] = (double)n.z;
     out->material_id = mat;
-    out->packed_material = mat & 0x3Fu;
+    out->packed_material = (mat & 0x3Fu) | (sec << 6) | (blend << 12);
     R mx, my;
     F(motion)(cam, w, h, px, py, hp, &mx, &my);
     out->motion[0] = (double)mx;

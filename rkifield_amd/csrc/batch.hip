@@ -112,6 +112,9 @@ struct rkfm_batch_graph {
     DevBuf items, begins, counter, tiles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Recorded on the stream of every replay: destroy waits for it, so the
+    // buffers above outlive the last replay whatever stream it ran on.
+    hipEvent_t done = nullptr;
 };
 
 namespace {
@@ -313,7 +316,7 @@ int rkfm_batch_run(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, const 
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     CullLayout cl;
-    if (!cull.empty()) {
+    if (!rc && !cull.empty()) {
         cl = cull_layout(items, cull);
         rc = c->batch_tiles.reserve(cl.bytes, false);
         if (rc) return rc;
@@ -399,8 +402,9 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
     }
     rkfm_batch_graph *g = new rkfm_batch_graph();
     g->ctx = c;
+    rc = rkf_hip_check(hipEventCreateWithFlags(&g->done, hipEventDisableTiming), "graph event");
     CullLayout cl;
-    if (!cull.empty()) {
+    if (!rc && !cull.empty()) {
         cl = cull_layout(items, cull);
         rc = g->tiles.reserve(cl.bytes, false);
         if (!rc) cull_patch(dev, items, cull, cl, (unsigned char *)g->tiles.ptr);
@@ -440,12 +444,18 @@ int rkfm_batch_graph_create(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t 
 int rkfm_batch_graph_launch(rkfm_batch_graph *g, void *stream) {
     if (!g || !g->exec) return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch_graph_launch: null graph");
     hipStream_t st = stream ? (hipStream_t)stream : g->ctx->stream;
-    return rkf_hip_check(hipGraphLaunch(g->exec, st), "graph launch");
+    int rc = rkf_hip_check(hipGraphLaunch(g->exec, st), "graph launch");
+    if (!rc) rc = rkf_hip_check(hipEventRecord(g->done, st), "graph launch event");
+    return rc;
 }
 
 int rkfm_batch_graph_destroy(rkfm_batch_graph *g) {
     if (!g) return RKFM_OK;
     if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->done) {
+        (void)hipEventSynchronize(g->done);  // the last replay, on whichever stream it ran
+        (void)hipEventDestroy(g->done);
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->items.release();

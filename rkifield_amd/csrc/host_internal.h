@@ -106,3 +106,10 @@ static inline uint32_t rkf_voxel_material(uint64_t seed, uint32_t object, uint32
     uint64_t key = (uint64_t)gx | ((uint64_t)gy << 21) | ((uint64_t)gz << 42);
     return (uint32_t)(rkf_mix64(seed ^ rkf_mix64(key ^ ((uint64_t)object * 0xD6E8FEB86659FD93ull))) & 15u);
 }
+// Secondary material (6 bits) | blend weight (8 bits) << 8 of a voxel: a
+// second hash of the same key with another salt.
+static inline uint32_t rkf_voxel_blend(uint64_t seed, uint32_t object, uint32_t gx, uint32_t gy, uint32_t gz) {
+    uint64_t key = (uint64_t)gx | ((uint64_t)gy << 21) | ((uint64_t)gz << 42);
+    uint64_t h = rkf_mix64(seed ^ rkf_mix64(key ^ ((uint64_t)object * 0xD6E8FEB86659FD93ull)) ^ 0xA24BAED4963EE407ull);
+    return (uint32_t)((h >> 8) & 0x3Fu) | ((uint32_t)((h >> 16) & 0xFFu) << 8);
+}

@@ -182,7 +182,10 @@ RKF_DEV float eval_obj(RefCtx &c, f3 wp, const rkf_gpu_object &o, uint32_t oi, u
     float d;
     if (st == RKF_SDF_TYPE_ANALYTICAL) {
         d = analytical(lp, o);
-        mat = o.material_id;
+        // evaluate_object hands the material on as an f32 (:633); the march
+        // converts it back with u32() (saturating)
+        const float mf = (float)o.material_id;
+        mat = mf >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)mf;
     } else {
         uint32_t near_mat = ~0u;
         d = sample_voxelized(c, lp, o, false, &near_mat);

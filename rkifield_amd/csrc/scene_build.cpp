@@ -45,10 +45,127 @@ rkf_voxel_sample voxel_new(float d, uint32_t mat) {
     return v;
 }
 
+// VoxelSample::new_blended (voxel.rs:43-55).
+rkf_voxel_sample voxel_new_blended(float d, uint32_t mat, uint32_t sec, uint32_t blend) {
+    rkf_voxel_sample v;
+    v.word0 = (uint32_t)f32_to_f16(d) | ((mat & 0x3Fu) << 16) | ((sec & 0x3Fu) << 22);
+    v.word1 = (blend & 0xFFu) << 24;
+    return v;
+}
+
 float vlen(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+
+// Primitive SDFs of sdf.rs:12-37 for rkfm_prim types 0-2 (centre, then the
+// row-major world->local rotation), the operation order of the GPU scene
+// builder.
+float rkf_prim_eval(const rkfm_prim &pr, float x, float y, float z) {
+    const float d0 = x - pr.centre[0], d1 = y - pr.centre[1], d2 = z - pr.centre[2];
+    const float *R = pr.rot;
+    const float lx = R[0] * d0 + R[1] * d1 + R[2] * d2;
+    const float ly = R[3] * d0 + R[4] * d1 + R[5] * d2;
+    const float lz = R[6] * d0 + R[7] * d1 + R[8] * d2;
+    const float *q = pr.params;
+    switch (pr.type) {
+    case 0: return vlen(lx, ly, lz) - q[0];
+    case 1: {
+        const float qx = std::fabs(lx) - q[0], qy = std::fabs(ly) - q[1], qz = std::fabs(lz) - q[2];
+        const float outside = vlen(std::fmax(qx, 0.0f), std::fmax(qy, 0.0f), std::fmax(qz, 0.0f));
+        return outside + std::fmin(std::fmax(qx, std::fmax(qy, qz)), 0.0f);
+    }
+    case 2: {
+        const float hh = q[1];
+        const float pax = lx - 0.0f, pay = ly - -hh, paz = lz - 0.0f;
+        const float bax = 0.0f - 0.0f, bay = hh - -hh, baz = 0.0f - 0.0f;
+        float h = (pax * bax + pay * bay + paz * baz) / (bax * bax + bay * bay + baz * baz);
+        h = std::fmin(std::fmax(h, 0.0f), 1.0f);
+        return vlen(pax - bax * h, pay - bay * h, paz - baz * h) - q[0];
+    }
+    default: return 3.402823e+38f;
+    }
+}
 
 using SdfFn = std::function<float(float, float, float)>;
 using MatFn = std::function<uint32_t(uint32_t, uint32_t, uint32_t)>;
+// Secondary material | blend weight << 8 of global voxel (gx, gy, gz).
+using AuxFn = std::function<uint32_t(uint32_t, uint32_t, uint32_t)>;
+
+// The voxelize_to_geometry classification (voxelize_object.rs:217-385) over
+// voxelize_sdf's grid and voxel contents: a brick whose centre lies deeper
+// than the narrow band is INTERIOR; a narrow-band brick is sampled at its
+// 512 voxel centres and becomes EMPTY when none is solid (d <= 0), INTERIOR
+// when all are, and is stored otherwise. Stored bricks take consecutive
+// slots in z-y-x order.
+void voxelize_geometry(const SdfFn &sdf, const MatFn &mat, const AuxFn *aux, const float amin[3],
+                       const float amax[3], float vs, uint32_t first_slot, uint32_t dims[3], uint32_t *map,
+                       rkf_voxel_sample *bricks, uint64_t *n_bricks) {
+    const float bws = vs * 8.0f;
+    for (int a = 0; a < 3; a++) {
+        float c = std::ceil((amax[a] - amin[a]) / bws);
+        uint32_t d = c > 0.0f ? (uint32_t)c : 0u;
+        dims[a] = std::max(d, 1u);
+    }
+    float go[3];
+    for (int a = 0; a < 3; a++) go[a] = -((float)dims[a] * bws * 0.5f);
+    const float band = bws * 1.8f, half = vs * 0.5f;
+    const size_t nmap = (size_t)dims[0] * dims[1] * dims[2];
+    enum : uint8_t { kEmpty, kInterior, kStored };
+    std::vector<uint8_t> state(nmap, kEmpty);
+    std::vector<float> vox(512);
+    uint64_t count = 0;
+    for (uint32_t bz = 0; bz < dims[2]; bz++)
+        for (uint32_t by = 0; by < dims[1]; by++)
+            for (uint32_t bx = 0; bx < dims[0]; bx++) {
+                const size_t mi = bx + (size_t)by * dims[0] + (size_t)bz * dims[0] * dims[1];
+                const float bx0 = go[0] + (float)bx * bws, by0 = go[1] + (float)by * bws, bz0 = go[2] + (float)bz * bws;
+                const float hb = bws * 0.5f;
+                const float dc = sdf(bx0 + hb, by0 + hb, bz0 + hb);
+                if (dc < -band) {
+                    state[mi] = kInterior;
+                    continue;
+                }
+                if (!(std::fabs(dc) < band)) continue;
+                uint32_t solid = 0;
+                for (uint32_t v = 0; v < 512; v++) {
+                    const float px = bx0 + ((float)(v & 7u) * vs + half);
+                    const float py = by0 + ((float)((v >> 3) & 7u) * vs + half);
+                    const float pz = bz0 + ((float)(v >> 6) * vs + half);
+                    solid += sdf(px, py, pz) <= 0.0f;
+                }
+                state[mi] = solid == 0 ? kEmpty : solid == 512 ? kInterior : kStored;
+                count += state[mi] == kStored;
+            }
+    *n_bricks = count;
+    if (!map) return;
+    uint32_t slot = first_slot;
+    uint64_t local = 0;
+    for (uint32_t bz = 0; bz < dims[2]; bz++)
+        for (uint32_t by = 0; by < dims[1]; by++)
+            for (uint32_t bx = 0; bx < dims[0]; bx++) {
+                const size_t mi = bx + (size_t)by * dims[0] + (size_t)bz * dims[0] * dims[1];
+                if (state[mi] != kStored) {
+                    map[mi] = state[mi] == kInterior ? RKF_INTERIOR_SLOT : RKF_EMPTY_SLOT;
+                    continue;
+                }
+                map[mi] = slot++;
+                rkf_voxel_sample *b = bricks ? bricks + local * 512 : nullptr;
+                local++;
+                if (!b) continue;
+                const float bx0 = go[0] + (float)bx * bws, by0 = go[1] + (float)by * bws, bz0 = go[2] + (float)bz * bws;
+                for (uint32_t v = 0; v < 512; v++) {
+                    const uint32_t vx = v & 7u, vy = (v >> 3) & 7u, vz = v >> 6;
+                    const float d = sdf(bx0 + ((float)vx * vs + half), by0 + ((float)vy * vs + half),
+                                        bz0 + ((float)vz * vs + half));
+                    const uint32_t gx = bx * 8 + vx, gy = by * 8 + vy, gz = bz * 8 + vz;
+                    const uint32_t m = mat(gx, gy, gz);
+                    if (aux) {
+                        const uint32_t a = (*aux)(gx, gy, gz);
+                        b[v] = voxel_new_blended(d, m, a & 0x3Fu, a >> 8);
+                    } else {
+                        b[v] = voxel_new(d, m);
+                    }
+                }
+            }
+}
 
 // voxelize_object.rs:53-157 + populate_brick :160-182.
 void voxelize(const SdfFn &sdf, const MatFn &mat, const float amin[3], const float amax[3], float vs,
@@ -122,6 +239,39 @@ int rkfm_voxelize_sphere(float radius, float margin, float vs, uint32_t object_i
         return mat_seed ? rkf_voxel_material(mat_seed, object_index, gx, gy, gz) : material;
     };
     voxelize(sdf, mat, amin, amax, vs, first_slot, dims, map_entries, bricks, n_bricks);
+    return RKFM_OK;
+}
+
+int rkfm_voxelize_prim(const rkfm_prim *prim, float margin, float vs, uint32_t flags, uint32_t object_index,
+                       uint64_t mat_seed, uint32_t material, uint32_t first_slot, uint32_t dims[3],
+                       uint32_t *map_entries, rkf_voxel_sample *bricks, uint64_t *n_bricks) {
+    if (!prim || !dims || !n_bricks) return rkf_set_error(RKFM_ERR_INVALID_ARG, "voxelize_prim: null argument");
+    if (prim->type > 2u) return rkf_set_error(RKFM_ERR_INVALID_ARG, "voxelize_prim: type %u", prim->type);
+    if (!(vs > 0.0f) || margin < 0.0f || (flags & ~(RKFM_VOX_GEOMETRY_FIRST | RKFM_VOX_BLEND)))
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "voxelize_prim: voxel size/margin/flags");
+    const float *q = prim->params;
+    const float br = prim->type == 0u ? q[0] : prim->type == 1u ? vlen(q[0], q[1], q[2]) : q[0] + q[1];
+    if (!(br > 0.0f)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "voxelize_prim: empty primitive");
+    float amin[3], amax[3];
+    for (int a = 0; a < 3; a++) {
+        amin[a] = prim->centre[a] - (br + margin);
+        amax[a] = prim->centre[a] + (br + margin);
+    }
+    const rkfm_prim pr = *prim;
+    SdfFn sdf = [pr](float x, float y, float z) { return rkf_prim_eval(pr, x, y, z); };
+    MatFn mat = [=](uint32_t gx, uint32_t gy, uint32_t gz) {
+        return mat_seed ? rkf_voxel_material(mat_seed, object_index, gx, gy, gz) : material;
+    };
+    AuxFn aux = [=](uint32_t gx, uint32_t gy, uint32_t gz) {
+        return rkf_voxel_blend(mat_seed, object_index, gx, gy, gz);
+    };
+    if (flags & RKFM_VOX_GEOMETRY_FIRST)
+        voxelize_geometry(sdf, mat, (flags & RKFM_VOX_BLEND) ? &aux : nullptr, amin, amax, vs, first_slot, dims,
+                          map_entries, bricks, n_bricks);
+    else if (flags & RKFM_VOX_BLEND)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "voxelize_prim: BLEND needs GEOMETRY_FIRST");
+    else
+        voxelize(sdf, mat, amin, amax, vs, first_slot, dims, map_entries, bricks, n_bricks);
     return RKFM_OK;
 }
 

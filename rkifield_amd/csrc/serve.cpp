@@ -170,6 +170,13 @@ int plan_request(const rkfm_server *s, const rkfm_request &r, Req &q) {
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "server: scene %u of %zu", r.scene, s->scenes.size());
     if (r.ext.hit_eps_mode > RKFM_EPS_RELATIVE)
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "server: hit_eps_mode %u", r.ext.hit_eps_mode);
+    // A round's batched launch has no per-request tile-interval buffers and a
+    // round of one request takes the plain kernel without them, so a flagged
+    // request is refused here, before admission, instead of failing (or being
+    // silently run unculled) together with the requests of its round.
+    if (r.ext.march_flags != 0)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "server: ext.march_flags %u (the server takes 0)",
+                             r.ext.march_flags);
     uint64_t off = 0;
     if (r.kind == RKFM_BATCH_VIEW) {
         if (r.width == 0 || r.height == 0 || (uint64_t)r.width * r.height > (1ull << 30))

@@ -18,6 +18,7 @@ BRICK_DIM = 8
 TILE_SIZE = 16
 TILE_MAX_OBJECTS = 32
 SDF_TYPE_NONE, SDF_TYPE_ANALYTICAL, SDF_TYPE_VOXELIZED = 0, 1, 2
+VOX_GEOMETRY_FIRST, VOX_BLEND = 1, 2
 PRIM_SPHERE, PRIM_BOX, PRIM_CAPSULE, PRIM_TORUS, PRIM_CYLINDER, PRIM_PLANE = range(6)
 EPS_ABSOLUTE, EPS_RELATIVE = 0, 1
 STRATEGY_AUTO, STRATEGY_TILED, STRATEGY_BVH, STRATEGY_BRUTE = 0, 1, 2, 3

@@ -64,6 +64,7 @@ _SIGS = {
     "rkfm_dispatch": (C.c_int, [_VP, _VP, _VP]),
     "rkfm_voxelize_sphere": (C.c_int, [_F, _F, _F, _U32, _U64, _U32, _U32, _VP, _VP, _VP, _VP]),
     "rkfm_voxelize_test_cross": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "rkfm_voxelize_prim": (C.c_int, [_VP, _F, _F, _U32, _U32, _U64, _U32, _U32, _VP, _VP, _VP, _VP]),
     "rkfm_camera_look_at": (C.c_int, [_VP, _VP, _F, _U32, _U32, _VP]),
     "rkfm_build_bvh": (C.c_int, [_VP, _U32, _VP, _VP]),
     "rkfm_sparse_build": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
@@ -396,6 +397,20 @@ def voxelize_sphere(radius, margin, voxel_size, object_index=0, mat_seed=0, mate
     b = np.empty(int(nb[0]) * 512, L.VOXEL_SAMPLE)
     _check(lib().rkfm_voxelize_sphere(radius, margin, voxel_size, object_index, mat_seed, material, first_slot,
                                       _p(dims), _p(m), _p(b), _p(nb)))
+    return dims, m, b
+
+
+def voxelize_prim(prim, margin, voxel_size, flags=0, object_index=0, mat_seed=0, material=0, first_slot=0):
+    """Host voxelisation of one primitive record (rkfm_voxelize_prim; flags
+    VOX_GEOMETRY_FIRST / VOX_BLEND). Returns (dims, map, bricks)."""
+    pr = np.ascontiguousarray(prim, L.PRIM)
+    dims = np.zeros(3, np.uint32)
+    nb = np.zeros(1, np.uint64)
+    args = (margin, voxel_size, flags, object_index, mat_seed, material, first_slot)
+    _check(lib().rkfm_voxelize_prim(_p(pr), *args, _p(dims), None, None, _p(nb)))
+    m = np.empty(int(np.prod(dims)), np.uint32)
+    b = np.empty(int(nb[0]) * 512, L.VOXEL_SAMPLE)
+    _check(lib().rkfm_voxelize_prim(_p(pr), *args, _p(dims), _p(m), _p(b), _p(nb)))
     return dims, m, b
 
 

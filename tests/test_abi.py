@@ -84,6 +84,125 @@ def test_product_sphere_voxelizer_matches_oracle_distances():
         assert np.array_equal(b["word0"], b2["word0"])
 
 
+def _mix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _oracle_voxelize_prim(prim, margin, vs, geometry, first_slot):
+    import ctypes as C
+    lib = O.lib()
+    pr = np.ascontiguousarray(prim, L.PRIM).reshape(())
+    q = pr["params"]
+    br = {0: q[0], 1: np.sqrt(np.float32(q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]), 2: q[0] + q[1]}[int(pr["type"])]
+    mn = (pr["centre"] - np.float32(br + np.float32(margin))).astype(np.float32)
+    mx = (pr["centre"] + np.float32(br + np.float32(margin))).astype(np.float32)
+    fn = lib.orc_voxelize_geometry if geometry else lib.orc_voxelize_sdf
+    fp = C.cast(lib.orc_prim_sdf, C.c_void_p).value
+    d2 = np.zeros(3, np.uint32)
+    nb = C.c_uint64(0)
+    args = (C.c_float(vs), C.c_uint32(first_slot), d2.ctypes.data)
+    fn(C.c_void_p(fp), C.c_void_p(pr.ctypes.data), mn.ctypes.data, mx.ctypes.data, *args, None, None, C.byref(nb))
+    m2 = np.empty(int(np.prod(d2)), np.uint32)
+    b2 = np.empty(nb.value * 512, L.VOXEL_SAMPLE)
+    fn(C.c_void_p(fp), C.c_void_p(pr.ctypes.data), mn.ctypes.data, mx.ctypes.data, *args, m2.ctypes.data,
+       b2.ctypes.data, C.byref(nb))
+    return d2, m2, b2
+
+
+def _prims():
+    rng = scenes.Pcg32(17)
+    out = []
+    for t, par in ((0, (0.9, 0, 0)), (1, (0.9, 0.5, 0.7)), (2, (0.45, 0.6, 0))):
+        p = np.zeros((), L.PRIM)
+        p["type"] = t
+        p["params"][:3] = par
+        p["rot"] = scenes.quat_to_world_to_local(rng.quaternion()).reshape(-1)
+        p["material"] = 5
+        out.append(p)
+    return out
+
+
+def test_product_prim_voxelizer_matches_oracle():
+    """rkfm_voxelize_prim (voxelize_sdf and the geometry-first classification)
+    equals the oracle's restatements word for word, and the classification
+    meets its definition, checked independently in numpy on a sphere."""
+    for p in _prims():
+        for geometry in (False, True):
+            flags = L.VOX_GEOMETRY_FIRST if geometry else 0
+            dims, m, b = M.voxelize_prim(p, 0.2, 0.05, flags, 0, 0, 5, 3)
+            d2, m2, b2 = _oracle_voxelize_prim(p, 0.2, 0.05, geometry, 3)
+            assert np.array_equal(dims, d2) and np.array_equal(m, m2)
+            assert np.array_equal(b.view(np.uint64), b2.view(np.uint64))
+            if geometry:
+                assert (m == L.INTERIOR_SLOT).any() and (m < L.INTERIOR_SLOT).any()
+    # classification of a sphere of radius 0.9 at vs 0.05 from float32 numpy
+    vs, r = np.float32(0.05), np.float32(0.9)
+    sph = _prims()[0]
+    dims, m, _ = M.voxelize_prim(sph, 0.2, 0.05, L.VOX_GEOMETRY_FIRST)
+    bws = vs * np.float32(8)
+    go = -(dims.astype(np.float32) * bws * np.float32(0.5))
+    band = bws * np.float32(1.8)
+    half = vs * np.float32(0.5)
+    v = np.arange(512)
+    k = 0
+    for bz in range(dims[2]):
+        for by in range(dims[1]):
+            for bx in range(dims[0]):
+                bmin = go + np.float32([bx, by, bz]) * bws
+                c = bmin + bws * np.float32(0.5)
+                dc = np.sqrt(np.float32(c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]) - r
+                p = [bmin[a] + (np.float32(vv) * vs + half) for a, vv in enumerate((v & 7, (v >> 3) & 7, v >> 6))]
+                dv = np.sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]) - r
+                if dc < -band or (abs(dc) < band and (dv <= 0).all()):
+                    want = "interior"
+                elif not abs(dc) < band or (dv > 0).all():
+                    want = "empty"
+                else:
+                    want = "stored"
+                e = m[bx + by * dims[0] + bz * dims[0] * dims[1]]
+                got = "interior" if e == L.INTERIOR_SLOT else "empty" if e == L.EMPTY_SLOT else "stored"
+                assert got == want, (bx, by, bz)
+                if got == "stored":
+                    assert e == k
+                    k += 1
+    assert k > 0
+
+
+def test_product_prim_voxelizer_blend_words():
+    """VOX_BLEND: the distance and material bits are the plain ones, the
+    secondary (bits 22-27) and blend (word1 byte 3) fields are the second
+    hash of the voxel key (restated in numpy), and both cover their range."""
+    p = _prims()[1]
+    seed = scenes.material_seed(9)
+    dims, m, b = M.voxelize_prim(p, 0.2, 0.05, L.VOX_GEOMETRY_FIRST | L.VOX_BLEND, 4, seed, 0, 0)
+    _, m0, b0 = M.voxelize_prim(p, 0.2, 0.05, L.VOX_GEOMETRY_FIRST, 4, seed, 0, 0)
+    assert np.array_equal(m, m0)
+    assert np.array_equal(b["word0"] & 0x3FFFFF, b0["word0"])
+    assert np.all(b0["word1"] == 0) and np.all(b0["word0"] >> 22 == 0)
+    slots = np.flatnonzero(m < L.INTERIOR_SLOT)
+    bz, by, bx = slots // (dims[0] * dims[1]), (slots // dims[0]) % dims[1], slots % dims[0]
+    v = np.arange(512)
+    gx = (bx[:, None] * 8 + (v & 7)).astype(np.uint64).ravel()
+    gy = (by[:, None] * 8 + ((v >> 3) & 7)).astype(np.uint64).ravel()
+    gz = (bz[:, None] * 8 + (v >> 6)).astype(np.uint64).ravel()
+    key = gx | (gy << np.uint64(21)) | (gz << np.uint64(42))
+    with np.errstate(over="ignore"):
+        inner = _mix64(key ^ (np.uint64(4) * np.uint64(0xD6E8FEB86659FD93)))
+        mat = _mix64(np.uint64(seed) ^ inner) & np.uint64(15)
+        h = _mix64(np.uint64(seed) ^ inner ^ np.uint64(0xA24BAED4963EE407))
+    order = np.argsort(m[slots])
+    w0 = b["word0"].reshape(-1, 512)[m[slots][order]].ravel()
+    w1 = b["word1"].reshape(-1, 512)[m[slots][order]].ravel()
+    mat, h = mat.reshape(-1, 512)[order].ravel(), h.reshape(-1, 512)[order].ravel()
+    assert np.array_equal((w0 >> 16) & 0x3F, mat.astype(np.uint32))
+    assert np.array_equal((w0 >> 22) & 0x3F, ((h >> np.uint64(8)) & np.uint64(63)).astype(np.uint32))
+    assert np.array_equal(w1 >> 24, ((h >> np.uint64(16)) & np.uint64(255)).astype(np.uint32))
+    assert np.unique((w0 >> 22) & 0x3F).size == 64 and np.unique(w1 >> 24).size == 256
+
+
 def test_per_voxel_material_is_uniform_0_15():
     dims, m, b = M.voxelize_sphere(1.0, 0.2, 0.05, 3, scenes.material_seed(0), 0, 0)
     mat = (b["word0"] >> 16) & 0x3F

@@ -102,6 +102,36 @@ def _clear(views, shadows, points):
         p[3].fill(0xAB)
 
 
+def _mixed_packets(t_lo, w, h, tw, th):
+    """Number of tw x th pixel packets that hold pixels of both an empty
+    8x8 interval tile (t_lo = +inf) and a non-empty one."""
+    tx = (w + 7) // 8
+    empty = np.isinf(t_lo.reshape(-1, tx))
+    px, py = np.meshgrid(np.arange(w), np.arange(h))
+    e = empty[py // 8, px // 8]
+    n = 0
+    for y0 in range(0, h, th):
+        for x0 in range(0, w, tw):
+            blk = e[y0:y0 + th, x0:x0 + tw]
+            n += int(blk.any() and not blk.all())
+    return n
+
+
+def test_culled_views_mix_empty_and_busy_tiles(ctx, two_scenes):
+    """The culled views' frames hold empty 8x8 tiles (t_lo = +inf) next to
+    non-empty ones inside one packet for the 32x2, 4x16 and 16x4 shapes, so
+    the batched parity below covers lanes that leave early beside lanes that
+    march."""
+    b, views, shadows, points = _make(ctx, two_scenes)
+    for sc, cam, su, ext, w, h, _ in views:
+        if not ext["march_flags"]:
+            continue
+        iv = sc.tile_intervals(cam, su, w, h).reshape(-1, 2)
+        assert np.isinf(iv[:, 0]).any() and np.isfinite(iv[:, 0]).any()
+        for tw, th in ((32, 2), (4, 16), (16, 4)):
+            assert _mixed_packets(iv[:, 0], w, h, tw, th) > 0, (w, h, tw, th)
+
+
 def test_batched_equals_alone(ctx, two_scenes):
     b, views, shadows, points = _make(ctx, two_scenes)
     ref_views, ref_points = _alone(ctx, views, points)

@@ -272,5 +272,20 @@ def test_replay_and_errors(ctx, three_scenes):
             srv.submit(bad)
         with pytest.raises(M.RkfMarchError):
             srv.release(123456789)
+        # a view asking for per-tile ray intervals is refused before admission
+        # and does not fail the requests submitted around it
+        mix.clear()
+        ids = [srv.submit(mix.request(0))]
+        bad = mix.request(2)
+        bad["ext"]["march_flags"] = L.MARCH_TILE_CULL
+        with pytest.raises(M.RkfMarchError) as e:
+            srv.submit(bad)
+        assert e.value.code == M.ERR["INVALID_ARG"]
+        ids += [srv.submit(mix.request(2)), srv.submit(mix.request(5))]
+        for k, rid in zip((0, 2, 5), ids):
+            st = srv.wait(rid, timeout=60.0)
+            assert st["state"] == L.REQ_DONE, (k, st)
+            mix.check(srv, k, rid)
+            srv.release(rid)
     finally:
         srv.close()
