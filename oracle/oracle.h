@@ -270,6 +270,11 @@ void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t
 /* Sparse field value at a world point (stepping semantics). */
 double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec);
 
+/* evaluate_object (ray_march.wgsl:600-634) of object oi at world point p:
+ * distance (FP32 twin widened, or FP64) and material. */
+void orc_ref_eval_object(const orc_ref_scene *s, int prec, uint32_t oi, const double p[3], double *dist,
+                         uint32_t *mat);
+
 /* Pack unpacked results into the reference G-buffer formats. */
 void orc_pack_gbuffer(const orc_ray_result *r, uint64_t n, float *position,
                       uint16_t *normal, uint32_t *material, float *motion);

@@ -257,6 +257,22 @@ def ref_trace(inp: RefInputs, prec=32, p0=0, p1=None):
     return out
 
 
+def ref_eval_object(inp: RefInputs, oi, pts, prec=64):
+    """evaluate_object of object oi at world points pts (n x 3):
+    (distances float64, materials uint32)."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    d = np.zeros(pts.shape[0])
+    m = np.zeros(pts.shape[0], np.uint32)
+    f = lib().orc_ref_eval_object
+    f.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    for i in range(pts.shape[0]):
+        dd = C.c_double(0)
+        mm = C.c_uint32(0)
+        f(C.byref(inp.s), prec, oi, pts[i].ctypes.data, C.byref(dd), C.byref(mm))
+        d[i], m[i] = dd.value, mm.value
+    return d, m
+
+
 def ref_trace_parallel(inp: RefInputs, threads=0):
     out = np.zeros(inp.width * inp.height, RAY_RESULT)
     used = lib().orc_ref_trace_parallel(C.byref(inp.s), _p(out), C.c_int(threads))

@@ -47,6 +47,17 @@ void orc_ref_trace(const orc_ref_scene *s, int prec, uint64_t p0, uint64_t p1, o
     }
 }
 
+void orc_ref_eval_object(const orc_ref_scene *s, int prec, uint32_t oi, const double p[3], double *dist,
+                         uint32_t *mat) {
+    if (prec == 64) {
+        rctx_f64 c = {s, 0, 0};
+        *dist = eval_object_f64(&c, mk_f64(p[0], p[1], p[2]), oi, mat);
+    } else {
+        rctx_f32 c = {s, 0, 0};
+        *dist = (double)eval_object_f32(&c, mk_f32((float)p[0], (float)p[1], (float)p[2]), oi, mat);
+    }
+}
+
 int orc_ref_trace_parallel(const orc_ref_scene *s, orc_ray_result *out, int threads) {
     uint64_t n = (uint64_t)s->width * s->height;
     int used = 1;

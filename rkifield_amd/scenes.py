@@ -143,6 +143,146 @@ def spheres_scene(seed=0, n=64, voxel_size=0.05, width=640, height=360, r_range=
     return RefScene(pool, mp, objects, cam, su, ext, width, height, float(vs))
 
 
+def _object_transform(rng, half, scale_range):
+    """Translation U[-half, half]^3, uniform rotation, per-axis scale
+    U[scale_range] (draws: 3 centre, quaternion (3), 3 scales). Returns
+    (centre, R local->world 3x3, scale, inverse_world column-major f32):
+    world = T * R * S (flatten_object's order), inverse_world = S^-1 R^T T^-1."""
+    c = np.array([rng.uniform(-half, half) for _ in range(3)])
+    q = rng.quaternion()
+    sc = np.array([rng.uniform(*scale_range) for _ in range(3)])
+    rot = quat_to_world_to_local(q).astype(np.float64).T  # local -> world
+    w = np.eye(4)
+    w[:3, :3] = rot @ np.diag(sc)
+    w[:3, 3] = c
+    inv = np.linalg.inv(w)
+    return c, rot, sc, inv.T.astype(np.float32).reshape(-1)  # column-major: m[c*4 + r]
+
+
+def _world_aabb(rot, sc, c, lo, hi):
+    corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+    wc = (rot @ (np.diag(sc) @ corners.T)).T + c
+    return wc.min(axis=0).astype(np.float32), wc.max(axis=0).astype(np.float32)
+
+
+def objects_scene(seed=5, n_analytic=12, n_voxel=8, voxel_size=0.05, width=160, height=96, half=2.5,
+                  cam_dist=9.0, jitter=(0.31, -0.27), rel_eps=True, scale_range=(0.6, 1.4)):
+    """Reference-layout scene that exercises every object path of the march:
+    analytic objects cycling through the six primitives (sphere, box,
+    capsule, torus, cylinder, plane; ray_march.wgsl:182-264), geometry-first
+    voxelised primitives (INTERIOR_SLOT bricks, secondary material and blend
+    weight per voxel), each object rotated and scaled non-uniformly
+    (inverse_world and accumulated_scale), half of the voxelised ones with a
+    geometry AABB; a camera with sub-pixel jitter whose prev_vp is the
+    view-projection of a displaced previous pose. One object's material id is
+    2^24 + 1, which the f32 round trip of evaluate_object turns into 2^24.
+    Draw order: per analytic object its transform then its parameters (a
+    plane draws a tilt quaternion first), per voxelised object its transform
+    and size, then the camera direction (its y made positive) and the
+    previous-pose offset."""
+    rng = Pcg32(seed)
+    n = n_analytic + n_voxel
+    objects = np.zeros(n, L.GPU_OBJECT)
+    pools, maps = [], []
+    slot = off = 0
+    vs = np.float32(voxel_size)
+    be = np.float32(vs * np.float32(8.0))
+    mseed = material_seed(seed)
+    for i in range(n_analytic):
+        o = objects[i]
+        c, rot, sc, inv = _object_transform(rng, half, scale_range)
+        t = i % 6
+        if t == L.PRIM_SPHERE:
+            prm = (rng.uniform(0.3, 0.7), 0.0, 0.0)
+            ext = (prm[0],) * 3
+        elif t == L.PRIM_BOX:
+            prm = tuple(rng.uniform(0.15, 0.6) for _ in range(3))
+            ext = prm
+        elif t == L.PRIM_CAPSULE:
+            prm = (rng.uniform(0.15, 0.35), rng.uniform(0.2, 0.6), 0.0)
+            ext = (prm[0], prm[0] + prm[1], prm[0])
+        elif t == L.PRIM_TORUS:
+            prm = (rng.uniform(0.35, 0.6), rng.uniform(0.08, 0.2), 0.0)
+            ext = (prm[0] + prm[1], prm[1], prm[0] + prm[1])
+        elif t == L.PRIM_CYLINDER:
+            prm = (rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.6), 0.0)
+            ext = (prm[0], prm[1], prm[0])
+        else:
+            # a ground plane below the other objects (camera above it): the
+            # object's transform keeps its scale and a small tilt, its
+            # normal (params) leans off local +y; its AABB bounds the slab
+            # the tile cull sees
+            c = np.array([c[0] * 0.3, -half - 0.8, c[2] * 0.3])
+            tilt = rng.quaternion()
+            tilt = np.array([tilt[0] * 0.15, tilt[1] * 0.15, tilt[2] * 0.15, 1.0])
+            tilt /= np.linalg.norm(tilt)
+            rot = quat_to_world_to_local(tilt).astype(np.float64).T
+            w = np.eye(4)
+            w[:3, :3] = rot @ np.diag(sc)
+            w[:3, 3] = c
+            inv = np.linalg.inv(w).T.astype(np.float32).reshape(-1)
+            prm = (rng.uniform(-0.2, 0.2), 1.0, rng.uniform(-0.2, 0.2))
+            ext = (3.0, 0.3, 3.0)
+        o["inverse_world"] = inv
+        lo, hi = _world_aabb(rot, sc, c, -np.array(ext), np.array(ext))
+        o["aabb_min"][:3], o["aabb_max"][:3] = lo, hi
+        o["sdf_type"] = L.SDF_TYPE_ANALYTICAL
+        o["primitive_type"] = t
+        o["sdf_params"][:3] = prm
+        o["accumulated_scale"] = sc
+        o["material_id"] = (1 << 24) + 1 if i == 1 else 1 + (i * 7) % 63
+        o["object_id"] = i + 1
+    for j in range(n_voxel):
+        i = n_analytic + j
+        o = objects[i]
+        c, rot, sc, inv = _object_transform(rng, half, scale_range)
+        pr = np.zeros((), L.PRIM)
+        pr["type"] = j % 3
+        size = rng.uniform(0.55, 0.9)
+        pr["params"][:3] = (size, size * 0.75, size * 0.6) if j % 3 == 1 else (size, size * 0.7, 0.0) \
+            if j % 3 == 2 else (size, 0.0, 0.0)
+        if j % 3 == 2:
+            pr["params"][0] = size * 0.55
+        pr["rot"] = np.eye(3, dtype=np.float32).reshape(-1)
+        dims, m, b = M.voxelize_prim(pr, float(vs * np.float32(4.0)), float(vs),
+                                     L.VOX_GEOMETRY_FIRST | L.VOX_BLEND, i, mseed, 0, slot)
+        nb = b.size // 512
+        o["inverse_world"] = inv
+        hgrid = dims.astype(np.float32) * be * np.float32(0.5)
+        lo, hi = _world_aabb(rot, sc, c, -hgrid, hgrid)
+        o["aabb_min"][:3], o["aabb_max"][:3] = lo, hi
+        o["brick_map_offset"] = off
+        o["brick_map_dims"] = dims
+        o["voxel_size"] = vs
+        o["sdf_type"] = L.SDF_TYPE_VOXELIZED
+        o["accumulated_scale"] = sc
+        o["object_id"] = i + 1
+        occ = np.argwhere((m != L.EMPTY_SLOT).reshape(dims[2], dims[1], dims[0]))
+        if j % 2 == 0 and occ.size:
+            glo = occ.min(axis=0)[::-1].astype(np.float32)
+            ghi = (occ.max(axis=0)[::-1] + 1).astype(np.float32)
+            o["geometry_aabb_min"] = glo * be - hgrid
+            o["geometry_aabb_max"] = ghi * be - hgrid
+        pools.append(b)
+        maps.append(m)
+        slot += nb
+        off += m.size
+    cam_dir = rng.unit_vector()
+    cam_dir[1] = abs(cam_dir[1]) + 0.35  # above the ground planes
+    cam_dir /= np.linalg.norm(cam_dir)
+    prev_off = np.array([rng.uniform(-0.4, 0.4) for _ in range(3)])
+    pool = np.concatenate(pools) if pools else np.zeros(0, L.VOXEL_SAMPLE)
+    mp = np.concatenate(maps) if maps else np.zeros(0, np.uint32)
+    eye = cam_dir * cam_dist
+    cam = M.camera_look_at(eye, (0.0, 0.0, 0.0), 60.0, width, height)
+    prev = M.camera_look_at(eye + prev_off, (0.1, -0.05, 0.0), 60.0, width, height)
+    cam["prev_vp"] = prev["prev_vp"]
+    cam["jitter"][:2] = jitter
+    su = L.scene_uniforms(num_objects=n)
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE if rel_eps else L.EPS_ABSOLUTE)
+    return RefScene(pool, mp, objects, cam, su, ext, width, height, float(vs))
+
+
 def primitive_field(seed=1, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8):
     """4096 primitives: type U{sphere, box, capsule}, size U[0.1,4] (full
     extent), uniform random rotation, centre U[-128,128]^3; draws per

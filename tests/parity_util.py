@@ -1,0 +1,76 @@
+"""Shared helpers of the GPU parity tests (not a test module): upload a
+reference-layout scene, compare the G-buffer with the oracle's FP32 twin bit
+for bit, and measure depth / normal errors against the FP64 oracle."""
+import numpy as np
+
+from oracle import oracle_py as O
+from rkifield_amd import layouts as L
+from rkifield_amd import march as M
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def upload_ref(ctx, sc, strategy=L.STRATEGY_AUTO, tiles=True, coarse=True):
+    gs = M.GpuScene(ctx)
+    gs.upload_brick_pool(sc.pool)
+    gs.upload_brick_maps(sc.maps)
+    gs.upload_objects(sc.objects)
+    gs.upload_bvh(M.build_bvh(sc.objects))
+    gs.update_camera(sc.cam)
+    ext = sc.ext.copy()
+    ext["strategy"] = strategy
+    gs.update_scene(sc.su, ext)
+    tc = M.TileObjectCullPass(ctx, sc.width, sc.height)
+    cf = M.CoarseField.from_scene_aabbs(ctx, sc.aabbs())
+    if tiles:
+        tc.dispatch()
+    else:
+        tc.clear()
+    if coarse:
+        cf.populate_and_upload()
+    else:
+        cf.clear()
+    return gs, tc, cf, ext
+
+
+def ref_inputs(sc, ext, tc=None, cf=None, tiles=True, coarse=True):
+    kw = dict(bvh=M.build_bvh(sc.objects))
+    if tiles and tc is not None:
+        idx, counts = tc.read()
+        kw.update(tile_idx=idx, tile_counts=counts)
+    if coarse and cf is not None:
+        cells, u = cf.read()
+        kw.update(coarse=cells, coarse_u=u)
+    return O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height, **kw)
+
+
+def compare_ref(gpu, res):
+    """Bit-exact comparison of the drop-in kernel's G-buffer and debug plane
+    with the FP32 twin."""
+    pos, nrm, mat, mot = O.pack_gbuffer(res)
+    dbg = gpu["debug"]
+    hit = res["hit"] == 1
+    ghit = gpu["position"][:, 3] != L.MAX_FLOAT
+    assert np.array_equal(ghit, hit), f"hit mask differs on {np.sum(ghit != hit)} rays"
+    assert np.array_equal(gpu["material"], mat), f"material differs on {np.sum(gpu['material'] != mat)} rays"
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals"):
+        assert np.array_equal(dbg[f], res[f]), f
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    dn = np.asarray(res["normal"], np.float32)
+    assert np.array_equal(dbg["normal"][hit].view(np.uint32), dn[hit].view(np.uint32))
+
+
+def depth_normal_errors(t_gpu, n_gpu, res64, res32):
+    """Over rays whose discrete path (hit, step count) is the same in FP32
+    and FP64: (same-path share, relative depth errors, normal angles)."""
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    hit = same & (res32["hit"] == 1) & (res64["t"] > 0)
+    t = np.asarray(t_gpu, np.float64)
+    rel = np.abs(t[hit] - res64["t"][hit]) / res64["t"][hit]
+    n = np.asarray(n_gpu, np.float64)
+    cosang = np.clip(np.sum(n[hit] * res64["normal"][hit], axis=1), -1.0, 1.0)
+    return same.mean(), rel, np.arccos(cosang)
