@@ -56,12 +56,7 @@ def build_config1(ctx, M, L, scenes, width, height):
     ctx.sync()
     log(f"config1 scene: {int(sc.info['n_bricks'])} bricks, {int(sc.info['n_blocks'])} L1 blocks, "
         f"cells {list(sc.info['cells'])}, {sc.info['device_bytes'] / 2**30:.2f} GiB, built in {time.time() - t0:.1f} s")
-    rng = scenes.Pcg32(1001)
-    cam_pos = rng.unit_vector() * 64.0
-    cam = M.camera_look_at(cam_pos, (0.0, 0.0, 0.0), 60.0, width, height)
-    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
-    light = np.array([0.5, 1.0, 0.3])
-    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    cam, su, ext = scenes.config1_view(width, height, cull=False)
     return sc, cam, su, ext
 
 
@@ -298,25 +293,11 @@ def run_config4(args, ctx, M, L, scenes):
         f"{[int(c) for c in sc.info['cells']]}, {sc.info['device_bytes'] / 2**30:.2f} GiB, built in {time.time() - t0:.1f} s")
     W, H = 3840, 2160
     t = prims[0]
-    eye = (0.0, float(scenes.terrain_height(t, 0.0, -60.0)) + 14.0, -60.0)
-    tgt = (0.0, float(scenes.terrain_height(t, 0.0, 25.0)), 25.0)
-    cam = M.camera_look_at(eye, tgt, 60.0, W, H)
-    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
-    light = np.array([0.5, 1.0, 0.3])
-    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light),
-                      march_flags=0 if args.no_cull else L.MARCH_TILE_CULL)
+    cam, su, ext = scenes.terrain_view(t, W, H, cull=not args.no_cull)
     rng = scenes.Pcg32(44)
 
     def frame_edits():
-        ops = []
-        for k in range(32):
-            x, z = rng.uniform(-40.0, 40.0), rng.uniform(-20.0, 70.0)
-            r = rng.uniform(0.5, 3.0)
-            y = float(scenes.terrain_height(t, x, z)) + rng.uniform(-0.5, 0.5)
-            kind = L.EDIT_SMOOTH_UNION if k % 2 == 0 else L.EDIT_SMOOTH_SUBTRACT
-            ops.append(L.edit_op((x, y, z), kind, L.SHAPE_SPHERE, (r, 0.0, 0.0), 1.0, 0.3 * r, L.FALLOFF_SMOOTH,
-                                 1 + k % 15))
-        return np.array(ops, L.EDIT)
+        return scenes.terrain_edits(t, rng)
 
     gbd = M.GBuffer(ctx, W, H, debug=True, shadow=True)
     sc.march(cam, su, ext, gbd)

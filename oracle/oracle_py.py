@@ -394,12 +394,12 @@ def sparse_oct(cells, free_l2):
     return out
 
 
-def sparse_edit(sp, ops):
-    """orc_sparse_edit on copies of the scene's planes: returns (dist, mat,
-    sorted modified slots)."""
+def sparse_edit(sp, ops, in_place=False):
+    """orc_sparse_edit on copies of the scene's planes (or on the scene's own
+    planes with in_place): returns (dist, mat, sorted modified slots)."""
     ops = np.ascontiguousarray(ops, L.EDIT)
-    dist = sp.dist.copy()
-    mat = sp.mat.copy()
+    dist = sp.dist if in_place else sp.dist.copy()
+    mat = sp.mat if in_place else sp.mat.copy()
     n_bricks = dist.size // 512
     mod = np.zeros(max(n_bricks, 1), np.uint8)
     lib().orc_sparse_edit(C.byref(sp.s), _p(dist), _p(mat), _p(ops), C.c_uint32(ops.size), _p(mod))

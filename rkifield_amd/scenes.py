@@ -322,6 +322,46 @@ def prim_bounds(prims, pad):
     return lo.astype(np.float32), hi.astype(np.float32)
 
 
+def config1_view(width=1920, height=1080, shadows=True, cull=True):
+    """The 1080p view of the 4096-primitive scene: camera 64 m from the centre
+    on the direction Pcg32(1001).unit_vector(), looking at the origin, fov 60;
+    128 near steps, hit eps 1e-3 t, far 100 m; one shadow ray per hit toward
+    normalize(0.5, 1, 0.3); per-tile ray intervals. Returns (cam, su, ext)."""
+    rng = Pcg32(1001)
+    cam = M.camera_look_at(rng.unit_vector() * 64.0, (0.0, 0.0, 0.0), 60.0, width, height)
+    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
+    light = np.array([0.5, 1.0, 0.3])
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1 if shadows else 0,
+                      light_dir=light / np.linalg.norm(light), march_flags=L.MARCH_TILE_CULL if cull else 0)
+    return cam, su, ext
+
+
+def terrain_view(prim, width=3840, height=2160, cull=True):
+    """The terrain frame's camera: 14 m above the terrain at (0, -60) looking
+    at the terrain point (0, 25); limits and shadows as config1_view."""
+    eye = (0.0, float(terrain_height(prim, 0.0, -60.0)) + 14.0, -60.0)
+    tgt = (0.0, float(terrain_height(prim, 0.0, 25.0)), 25.0)
+    cam = M.camera_look_at(eye, tgt, 60.0, width, height)
+    _, su, ext = config1_view(cull=cull)
+    return cam, su, ext
+
+
+def terrain_edits(prim, rng, n=32):
+    """One frame of CSG edits on the terrain in front of the camera: per edit
+    x U[-40, 40], z U[-20, 70], radius U[0.5, 3], centre height terrain +
+    U[-0.5, 0.5] (draws in that order); even edits smooth-union, odd ones
+    smooth-subtract, blend 0.3 r, smooth falloff, material 1 + k % 15."""
+    ops = []
+    for k in range(n):
+        x, z = rng.uniform(-40.0, 40.0), rng.uniform(-20.0, 70.0)
+        r = rng.uniform(0.5, 3.0)
+        y = float(terrain_height(prim, x, z)) + rng.uniform(-0.5, 0.5)
+        kind = L.EDIT_SMOOTH_UNION if k % 2 == 0 else L.EDIT_SMOOTH_SUBTRACT
+        ops.append(L.edit_op((x, y, z), kind, L.SHAPE_SPHERE, (r, 0.0, 0.0), 1.0, 0.3 * r, L.FALLOFF_SMOOTH,
+                             1 + k % 15))
+    return np.array(ops, L.EDIT)
+
+
 def mixed_batches(seed=2, n_views=16, n_shadow=16, n_points=32, half_cube=128.0):
     """Config-2 batch mix over a 256 m scene: views with W ~ U{256..1920},
     H ~ U{256..1080} (camera 64 m from the centre on a random direction,

@@ -74,3 +74,48 @@ def depth_normal_errors(t_gpu, n_gpu, res64, res32):
     n = np.asarray(n_gpu, np.float64)
     cosang = np.clip(np.sum(n[hit] * res64["normal"][hit], axis=1), -1.0, 1.0)
     return same.mean(), rel, np.arccos(cosang)
+
+
+SPARSE_DEBUG = ("brick_slot", "voxel_index", "cell_index", "steps", "evals", "shadow_steps", "shadow_evals")
+
+
+def compare_sparse(gpu, res, shadow=True):
+    """Bit-exact comparison of a sparse-scene G-buffer (+ debug, shadow
+    planes) with the FP32 twin; returns the number of hits."""
+    pos, nrm, mat, mot = O.pack_gbuffer(res)
+    hit = res["hit"] == 1
+    ghit = gpu["position"][:, 3] != L.MAX_FLOAT
+    assert np.array_equal(ghit, hit), f"hit mask differs on {np.sum(ghit != hit)} rays"
+    assert np.array_equal(gpu["material"], mat), f"material differs on {np.sum(gpu['material'] != mat)} rays"
+    if "debug" in gpu:
+        for f in SPARSE_DEBUG:
+            assert np.array_equal(gpu["debug"][f], res[f]), f
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    if shadow:
+        assert np.array_equal(gpu["shadow"].view(np.uint32), res["shadow"].astype(np.float32).view(np.uint32))
+    return int(hit.sum())
+
+
+def sparse_inputs(sc, aux=None):
+    l2, l1, dist, mat = sc.download()
+    i = sc.info
+    return O.SparseInputs(i["origin"], float(i["voxel_size"]), i["cells"], l2, l1, dist, mat, aux=aux)
+
+
+def fp64_rows(si, cam, su, ext, w, h, gpu, n_rows=24):
+    """FP64 oracle over the n_rows rows with the most GPU hits: (same-path
+    share, relative depth errors, normal angles) of those rows."""
+    hits = (gpu["position"][:, 3] != L.MAX_FLOAT).reshape(h, w).sum(axis=1)
+    rows = np.sort(np.argsort(-hits, kind="stable")[:n_rows])
+    sames, rels, angs = [], [], []
+    for r in rows:
+        p0, p1 = int(r) * w, (int(r) + 1) * w
+        r64 = O.sparse_trace(si, cam, su, ext, w, h, prec=64, p0=p0, p1=p1)
+        r32 = O.sparse_trace(si, cam, su, ext, w, h, prec=32, p0=p0, p1=p1)
+        s, rel, ang = depth_normal_errors(gpu["position"][p0:p1, 3], gpu["debug"]["normal"][p0:p1], r64, r32)
+        sames.append(s)
+        rels.append(rel)
+        angs.append(ang)
+    return float(np.mean(sames)), np.concatenate(rels), np.concatenate(angs)

@@ -1,0 +1,165 @@
+"""Parity at the workloads' own sizes, against the oracle:
+
+* the 1080p frame over the 4096-primitive scene (8.9 M bricks at 2 cm,
+  primary rays with per-tile intervals + one shadow ray per hit): every pixel
+  bit-exact against the FP32 twin (hit mask, brick slot, voxel and cell
+  index, step and evaluation counts, packed material, position, f16 normal,
+  motion, shadow visibility); FP64 depth and normal errors over the 24 rows
+  with most hits;
+* the 64 uneven batches of the batched workload (16 views up to 1080p, 16
+  shadow batches, 32 point batches of 1k-64k) in one graph-captured launch
+  with the autotuned shape: every batch bit-exact against the oracle;
+* a 300-request slice of the served trace over the 8 resident scenes (seeds
+  1-8): every result equal, bit for bit, to the same request alone, and the
+  requests on the seed-1 scene equal to the oracle."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from parity_util import compare_sparse, fp64_rows, say, sparse_inputs
+from rkifield_amd import layouts as L
+from rkifield_amd import march as M
+from rkifield_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DEPTH_REL_TOL = 1e-4
+NORMAL_RAD_TOL = 1e-3
+
+
+def _build(ctx, seed):
+    prims = scenes.primitive_field(seed=seed, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8)
+    lo, hi = scenes.prim_bounds(prims, 1.0)
+    return M.SparseScene.build(ctx, prims, 0.02, lo, hi, smin_k=0.1, mat_seed=scenes.material_seed(seed))
+
+
+@pytest.fixture(scope="module")
+def cfg1(ctx):
+    t0 = time.time()
+    sc = _build(ctx, 1)
+    si = sparse_inputs(sc)
+    say(f"config1 scene: {int(sc.info['n_bricks'])} bricks, oracle copy in {time.time() - t0:.1f} s")
+    yield sc, si
+    sc.destroy()
+
+
+def test_config1_full_frame(ctx, cfg1):
+    sc, si = cfg1
+    W, H = 1920, 1080
+    cam, su, ext = scenes.config1_view(W, H)
+    say("case: config1 1920x1080 + shadows, tile intervals")
+    gb = M.GBuffer(ctx, W, H, debug=True, shadow=True)
+    sc.march(cam, su, ext, gb)
+    gpu = gb.read()
+    res, _ = O.sparse_trace_parallel(si, cam, su, ext, W, H)
+    hits = compare_sparse(gpu, res)
+    assert hits > 100000 and (res["shadow_evals"] > 0).sum() > 50000
+    same, rel, ang = fp64_rows(si, cam, su, ext, W, H, gpu)
+    say(f"config1: {hits} hits bit-exact; FP64 over 24 rows: same-path {same:.5f}, {rel.size} hits, "
+        f"max rel depth {rel.max():.3e}, max normal angle {ang.max():.3e} rad")
+    assert same > 0.99 and rel.size > 1000
+    assert rel.max() <= DEPTH_REL_TOL and ang.max() <= NORMAL_RAD_TOL
+
+
+def test_config2_batches_match_oracle(ctx, cfg1):
+    sc, si = cfg1
+    _, su, ext = scenes.config1_view(cull=False)
+    ext_v = ext.copy()
+    ext_v["march_flags"] = L.MARCH_TILE_CULL
+    views, sh_orig, pts = scenes.mixed_batches(seed=2)
+    b = M.Batch(ctx)
+    vouts, souts, pouts = [], [], []
+    for (w, h, pos, tgt) in views:
+        cam = M.camera_look_at(pos, tgt, 60.0, w, h)
+        gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
+        b.add_view(sc, cam, su, ext_v, gb)
+        vouts.append((cam, w, h, gb))
+    for o in sh_orig:
+        od, vd = ctx.to_device(o), ctx.array((o.shape[0],), np.float32)
+        b.add_shadow(sc, od, vd, o.shape[0], ext)
+        souts.append((o, vd))
+    for x in pts:
+        xd, rd = ctx.to_device(x), ctx.array((x.shape[0],), L.POINT_RESULT)
+        b.add_points(sc, xd, rd, x.shape[0])
+        pouts.append((x, rd))
+    best, ms = b.autotune(reps=1)
+    say(f"case: config2 {b.work_items()} rays + queries, graph with schedule {int(best['schedule'])} "
+        f"tile {int(best['tile_w'])}x{int(best['tile_h'])}")
+    g = b.graph(tuning=best)
+    g.launch()
+    ctx.sync()
+    nh = 0
+    for cam, w, h, gb in vouts:
+        res, _ = O.sparse_trace_parallel(si, cam, su, ext_v, w, h)
+        nh += compare_sparse(gb.read(), res)
+    for o, vd in souts:
+        want = O.sparse_shadow(si, o, ext["light_dir"])
+        assert np.array_equal(vd.download().view(np.uint32), want.view(np.uint32))
+    for x, rd in pouts:
+        assert np.array_equal(rd.download().view(np.uint8), O.sparse_query(si, x).view(np.uint8))
+    g.destroy()
+    say(f"config2: 64 batches bit-exact ({nh} view hits)")
+    assert nh > 10000
+
+
+def test_config3_slice_served_equals_alone_and_oracle(ctx, cfg1):
+    sc1, si = cfg1
+    resident = [sc1] + [_build(ctx, seed) for seed in range(2, 9)]
+    try:
+        trace = scenes.served_trace(seed=3, n=300, rate=2000.0, n_scenes=8)
+        cam_, su, ext = scenes.config1_view(cull=False)
+        rng = np.random.default_rng(3)
+        srv = M.Server(ctx, resident, memory_budget=4 << 30, max_pending=len(trace) + 16)
+        items = []
+        for i, (t, kind, s, p) in enumerate(trace):
+            if kind == "points":
+                x = M.PinnedArray((p, 3), np.float32)
+                x.a[:] = rng.uniform(-128.0, 128.0, (p, 3))
+                out = M.PinnedArray((p,), L.POINT_RESULT)
+                out.a.view(np.uint8)[...] = 0xAB
+                req = M.Server.points_request(s, x, out, tag=i)
+                items.append((kind, s, (x, out)))
+            else:
+                w, h, pos, tgt = p
+                cam = M.camera_look_at(pos, tgt, 60.0, w, h)
+                req = M.Server.view_request(s, cam, su, ext, w, h, keep_on_device=True, tag=i)
+                items.append((kind, s, (cam, w, h)))
+            items[-1] = items[-1] + (srv.submit(req),)
+        say(f"case: config3 slice, {len(trace)} requests submitted at once over 8 scenes")
+        rounds, checked_oracle = set(), 0
+        for i, (kind, s, data, rid) in enumerate(items):
+            st = srv.wait(rid, timeout=120.0)
+            assert st["state"] == L.REQ_DONE, (i, st)
+            rounds.add(int(st["round"]))
+            if kind == "points":
+                x, out = data
+                xd, rd = ctx.to_device(x.a), ctx.array((x.shape[0],), L.POINT_RESULT)
+                resident[s].query(xd, rd, x.shape[0])
+                alone = rd.download()
+                assert np.array_equal(out.a.view(np.uint8), alone.view(np.uint8)), i
+                if s == 0:
+                    assert np.array_equal(alone.view(np.uint8), O.sparse_query(si, x.a).view(np.uint8)), i
+                    checked_oracle += 1
+            else:
+                cam, w, h = data
+                got = srv.read_view(rid)
+                gb = M.GBuffer(ctx, w, h, shadow=True)
+                resident[s].march(cam, su, ext, gb)
+                alone = gb.read()
+                for k in ("position", "normal", "material", "motion", "shadow"):
+                    assert np.array_equal(got[k].view(np.uint8), alone[k].view(np.uint8)), (i, k)
+                if s == 0:
+                    res, _ = O.sparse_trace_parallel(si, cam, su, ext, w, h)
+                    compare_sparse(alone, res)
+                    checked_oracle += 1
+            srv.release(rid)
+        st = srv.stats()
+        srv.close()
+        say(f"config3 slice: {len(items)} requests equal alone, {checked_oracle} also vs the oracle, "
+            f"{len(rounds)} rounds (max {int(st['max_round_requests_seen'])} requests in one)")
+        assert checked_oracle > 10 and len(rounds) < len(items)
+    finally:
+        for s in resident[1:]:
+            s.destroy()
