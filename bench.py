@@ -68,7 +68,46 @@ def algorithmic_bytes(dbg, hit):
     return int(b)
 
 
-def cpu_baseline_sparse(sc, cam, su, ext, width, height, seconds):
+def accuracy_sparse(O, si, cam, su, ext, width, height, gpu, res, r0, rows, n64=8):
+    """GPU frame against the oracle: every field of rows r0..r0+rows-1 against
+    the FP32 twin (count of differing pixels), and the FP64 oracle over the
+    n64 rows with most hits (rays whose discrete path is the same in FP32 and
+    FP64): max relative depth error, max normal angle."""
+    import numpy as np
+    from rkifield_amd import layouts as L
+    p0, p1 = r0 * width, (r0 + rows) * width
+    pos, nrm, mat, mot = O.pack_gbuffer(res)
+    g = {k: v[p0:p1] for k, v in gpu.items()}
+    bad = ~((g["position"].view(np.uint32) == pos.view(np.uint32)).all(axis=1)
+            & (g["normal"] == nrm).all(axis=1) & (g["material"] == mat)
+            & (g["motion"].view(np.uint32) == mot.view(np.uint32)).all(axis=1)
+            & (g["shadow"].view(np.uint32) == res["shadow"].astype(np.float32).view(np.uint32)))
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals", "shadow_steps", "shadow_evals"):
+        bad |= g["debug"][f] != res[f]
+    hits = (gpu["position"][:, 3] != L.MAX_FLOAT).reshape(height, width).sum(axis=1)
+    rows64 = np.sort(np.argsort(-hits, kind="stable")[:n64])
+    rels, angs, same_n, tot = [], [], 0, 0
+    for r in rows64:
+        q0, q1 = int(r) * width, (int(r) + 1) * width
+        r64 = O.sparse_trace(si, cam, su, ext, width, height, prec=64, p0=q0, p1=q1)
+        r32 = O.sparse_trace(si, cam, su, ext, width, height, prec=32, p0=q0, p1=q1)
+        same = (r64["hit"] == r32["hit"]) & (r64["steps"] == r32["steps"]) & (r64["path"] == r32["path"])
+        same_n += int(same.sum())
+        tot += same.size
+        h = same & (r32["hit"] == 1)
+        t = gpu["position"][q0:q1, 3].astype(np.float64)
+        rels.append(np.abs(t[h] - r64["t"][h]) / r64["t"][h])
+        n = gpu["debug"]["normal"][q0:q1].astype(np.float64)
+        angs.append(np.arccos(np.clip(np.sum(n[h] * r64["normal"][h], axis=1), -1.0, 1.0)))
+    rel, ang = np.concatenate(rels), np.concatenate(angs)
+    return dict(twin_rows=f"{r0}..{r0 + rows - 1}", twin_pixels=int(p1 - p0), twin_mismatches=int(bad.sum()),
+                fp64_rows=[int(r) for r in rows64], fp64_hits=int(rel.size), same_path=round(same_n / tot, 6),
+                max_rel_depth=float(rel.max(initial=0.0)), share_depth_1e4=round(float((rel <= 1e-4).mean()), 6),
+                max_normal_rad=float(ang.max(initial=0.0)),
+                share_normal_1e3=round(float((ang <= 1e-3).mean()), 6))
+
+
+def cpu_baseline_sparse(sc, cam, su, ext, width, height, seconds, gpu=None):
     from oracle import oracle_py as O
     l2, l1, dist, mat = sc.download()
     si = O.SparseInputs(sc.info["origin"], float(sc.info["voxel_size"]), sc.info["cells"], l2, l1, dist, mat)
@@ -88,10 +127,12 @@ def cpu_baseline_sparse(sc, cam, su, ext, width, height, seconds):
         dt = time.time() - t0
         if dt >= seconds or reps >= 200:
             break
+    acc = accuracy_sparse(O, si, cam, su, ext, width, height, gpu, res, r0, rows) if gpu is not None else None
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=int(used), kind="port",
-                sample=f"FP32 oracle twin (C, OpenMP dynamic/64), {reps} pass(es) over rows {r0}..{r0 + rows - 1} "
-                       f"of the 1920x1080 frame: {res.size} primary + {int(res['hit'].sum())} shadow rays per "
-                       f"pass, {dt:.1f} s; host {cpu_model()}")
+                sample=f"the oracle's FP32 twin of this path (C, OpenMP dynamic/64): the reference's brick "
+                       f"sampling and step rules with the same free-space DDA and tile intervals as the kernel, "
+                       f"{reps} pass(es) over rows {r0}..{r0 + rows - 1} of the 1920x1080 frame: {res.size} "
+                       f"primary + {int(res['hit'].sum())} shadow rays per pass, {dt:.1f} s; host {cpu_model()}"), acc
 
 
 def cpu_baseline_ref(sc, ext, idx, counts, cells, u, seconds):
@@ -107,8 +148,35 @@ def cpu_baseline_ref(sc, ext, idx, counts, cells, u, seconds):
         if dt >= seconds or reps >= 500:
             break
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=int(used), kind="port",
-                sample=f"FP32 oracle twin (C, OpenMP), {reps} passes over the full 640x360 frame in {dt:.1f} s; "
-                       f"host {cpu_model()}")
+                sample=f"the oracle's FP32 twin of ray_march_tiled (C, OpenMP): the reference's control flow "
+                       f"and sampling over the same tile lists and coarse field, {reps} passes over the full "
+                       f"640x360 frame in {dt:.1f} s; host {cpu_model()}"), res
+
+
+def accuracy_ref(sc, ext, idx, counts, cells, u, gpu, twin):
+    """Drop-in frame against the oracle: pixels whose G-buffer or debug
+    fields differ from the FP32 twin, and the FP64 oracle's depth and normal
+    errors over the rays whose discrete path is the same in FP32 and FP64."""
+    from oracle import oracle_py as O
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=u)
+    pos, nrm, mat, mot = O.pack_gbuffer(twin)
+    bad = ~((gpu["position"].view(np.uint32) == pos.view(np.uint32)).all(axis=1)
+            & (gpu["normal"] == nrm).all(axis=1) & (gpu["material"] == mat)
+            & (gpu["motion"].view(np.uint32) == mot.view(np.uint32)).all(axis=1))
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals"):
+        bad |= gpu["debug"][f] != twin[f]
+    r64 = O.ref_trace(inp, prec=64)
+    same = (r64["hit"] == twin["hit"]) & (r64["steps"] == twin["steps"]) & (r64["path"] == twin["path"])
+    h = same & (twin["hit"] == 1)
+    t = gpu["position"][:, 3].astype(np.float64)
+    rel = np.abs(t[h] - r64["t"][h]) / r64["t"][h]
+    n = gpu["debug"]["normal"].astype(np.float64)
+    ang = np.arccos(np.clip(np.sum(n[h] * r64["normal"][h], axis=1), -1.0, 1.0))
+    return dict(twin_pixels=int(twin.size), twin_mismatches=int(bad.sum()), fp64_hits=int(h.sum()),
+                same_path=round(float(same.mean()), 6), max_rel_depth=float(rel.max(initial=0.0)),
+                share_depth_1e4=round(float((rel <= 1e-4).mean()), 6), max_normal_rad=float(ang.max(initial=0.0)),
+                share_normal_1e3=round(float((ang <= 1e-3).mean()), 6))
 
 
 def run_config2(args, ctx, M, L, scenes):
@@ -180,37 +248,25 @@ def run_config2(args, ctx, M, L, scenes):
     print(json.dumps(line), flush=True)
 
 
-def run_config3(args, ctx, M, L, scenes):
-    """Served mixed stream over 8 resident config1-type scenes (seeds 1-8):
-    Poisson arrivals at 2000 requests/s (70% point batches U{1k..64k}, 25%
-    512x512 views, 5% 1920x1080 views, each view with a shadow ray per hit),
-    admitted by a 4 GiB memory budget. The same trace is replayed by a native
-    client against the batched server and against the per-request policy
-    (one plain launch per request), first at its arrival times (latency), then
-    all at once (saturation throughput). Both use the server's default shape
-    (4 rounds in flight, rounds capped at 2^20 rays + queries, a round of one
-    request on its plain kernel). Point results are copied to pinned host
-    memory; view G-buffers stay in HBM for their GPU consumer and are
-    released on completion."""
-    n_scenes = 8
-    resident = []
-    for seed in range(1, n_scenes + 1):
-        prims = scenes.primitive_field(seed=seed, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8)
-        lo, hi = scenes.prim_bounds(prims, 1.0)
-        resident.append(M.SparseScene.build(ctx, prims, 0.02, lo, hi, smin_k=0.1,
-                                            mat_seed=scenes.material_seed(seed)))
-    ctx.sync()
-    gib = sum(float(s.info["device_bytes"]) for s in resident) / 2**30
-    log(f"config3: {n_scenes} resident scenes, {sum(int(s.info['n_bricks']) for s in resident)} bricks, {gib:.1f} GiB")
-    trace = scenes.served_trace(seed=3, n=args.requests, rate=2000.0, n_scenes=n_scenes)
+def build_scene(ctx, M, scenes, seed):
+    prims = scenes.primitive_field(seed=seed, n=4096, half_cube=128.0, size_range=(0.1, 4.0), group=8)
+    lo, hi = scenes.prim_bounds(prims, 1.0)
+    return M.SparseScene.build(ctx, prims, 0.02, lo, hi, smin_k=0.1, mat_seed=scenes.material_seed(seed))
+
+
+def served_vs_plain(ctx, M, L, scenes, resident, n_requests):
+    """Replays the served trace (n_requests, Poisson 2000 req/s over the
+    resident scenes) against the batched server and against the per-request
+    policy, at its arrival times (latency) and all at once (saturation).
+    Returns (per-policy results, work items, offered Mrays/s, trace)."""
+    n_scenes = len(resident)
+    trace = scenes.served_trace(seed=3, n=n_requests, rate=2000.0, n_scenes=n_scenes)
     pool_n, ring_n, cap = 64, 256, 64 * 1024
     rng = np.random.default_rng(3)
     pool = M.PinnedArray((pool_n, cap, 3), np.float32)
     pool.a[:] = rng.uniform(-128.0, 128.0, (pool_n, cap, 3))
     ring = M.PinnedArray((ring_n, cap), L.POINT_RESULT)
-    su = L.scene_uniforms(num_objects=1, max_steps=128, max_distance=100.0, hit_threshold=1e-3)
-    light = np.array([0.5, 1.0, 0.3])
-    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=light / np.linalg.norm(light))
+    _, su, ext = scenes.config1_view(cull=False)
     reqs = np.zeros(len(trace), L.REQUEST)
     arrivals = np.zeros(len(trace))
     work = 0
@@ -259,6 +315,27 @@ def run_config3(args, ctx, M, L, scenes):
             f"{r['mrays_s_at_2k']:.1f} Mrays/s at 2k req/s in {r['rounds']} rounds; saturation "
             f"{r['saturation_mrays_s']:.1f} Mrays/s ({r['saturation_s']:.3f} s, {r['saturation_rounds']} rounds); "
             f"arena peak {r['arena_peak_mib']:.0f} MiB")
+    return results, work, offered, trace
+
+
+def run_config3(args, ctx, M, L, scenes):
+    """Served mixed stream over 8 resident config1-type scenes (seeds 1-8):
+    Poisson arrivals at 2000 requests/s (70% point batches U{1k..64k}, 25%
+    512x512 views, 5% 1920x1080 views, each view with a shadow ray per hit),
+    admitted by a 4 GiB memory budget. The same trace is replayed by a native
+    client against the batched server and against the per-request policy
+    (one plain launch per request), first at its arrival times (latency), then
+    all at once (saturation throughput). Both use the server's default shape
+    (4 rounds in flight, rounds capped at 2^20 rays + queries, a round of one
+    request on its plain kernel). Point results are copied to pinned host
+    memory; view G-buffers stay in HBM for their GPU consumer and are
+    released on completion."""
+    n_scenes = 8
+    resident = [build_scene(ctx, M, scenes, seed) for seed in range(1, n_scenes + 1)]
+    ctx.sync()
+    gib = sum(float(s.info["device_bytes"]) for s in resident) / 2**30
+    log(f"config3: {n_scenes} resident scenes, {sum(int(s.info['n_bricks']) for s in resident)} bricks, {gib:.1f} GiB")
+    results, work, offered, trace = served_vs_plain(ctx, M, L, scenes, resident, args.requests)
     ratio = results["served"]["saturation_mrays_s"] / results["plain"]["saturation_mrays_s"]
     line = {
         "metric": "served Mrays/s (rays + queries) at saturation; p50/p99 latency at 2000 req/s, served vs plain",
@@ -366,6 +443,8 @@ def main():
                     help="config1/config4: march without per-tile ray intervals")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=1000,
+                    help="config1: requests of the served-vs-plain replay reported beside the frame (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also launch the plain copy kernel on this many MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -457,16 +536,35 @@ def main():
         except (OSError, ValueError):
             traffic = None
 
-    cpu = None
+    cpu = acc = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         if args.workload == "config1":
-            cpu = cpu_baseline_sparse(sc, cam, su, ext, width, height, args.cpu_seconds)
+            cpu, acc = cpu_baseline_sparse(sc, cam, su, ext, width, height, args.cpu_seconds, gpu=out)
         else:
             idx, counts = tc.read()
             cells, u = cf.read()
-            cpu = cpu_baseline_ref(sc, sc.ext, idx, counts, cells, u, args.cpu_seconds)
-        log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads")
+            cpu, twin = cpu_baseline_ref(sc, sc.ext, idx, counts, cells, u, args.cpu_seconds)
+            acc = accuracy_ref(sc, sc.ext, idx, counts, cells, u, out, twin)
+        log(f"cpu baseline {cpu['value']:.3f} Mrays/s on {cpu['cores']} threads; accuracy {acc}")
+
+    served = None
+    if args.workload == "config1" and args.served_requests > 0:
+        others = [build_scene(ctx, M, scenes, seed) for seed in range(2, 9)]
+        ctx.sync()
+        res_s, work_s, offered, _ = served_vs_plain(ctx, M, L, scenes, [sc] + others, args.served_requests)
+        keys = ("p50_ms", "p99_ms", "p99_ms_points", "p99_ms_view512", "p99_ms_view1080", "mrays_s_at_2k",
+                "saturation_mrays_s", "failed")
+        served = {"workload": f"{args.served_requests} requests of the served trace (Poisson 2000 req/s; 70% point "
+                              f"batches U{{1k..64k}}, 25% 512x512, 5% 1920x1080 views + shadow rays) over 8 "
+                              f"resident scenes of this kind (seeds 1-8), 4 GiB admission budget",
+                  "offered_mrays_s": round(offered, 2),
+                  "served": {k: round(res_s["served"][k], 4) for k in keys if k in res_s["served"]},
+                  "plain": {k: round(res_s["plain"][k], 4) for k in keys if k in res_s["plain"]},
+                  "served_vs_plain_saturation": round(res_s["served"]["saturation_mrays_s"]
+                                                      / res_s["plain"]["saturation_mrays_s"], 3)}
+        for o in others:
+            o.destroy()
 
     line = {
         "metric": "Mrays/s + % HBM roofline at 1080p; p50/p99 latency served vs plain",
@@ -487,6 +585,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
+        "accuracy": acc,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

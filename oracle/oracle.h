@@ -184,6 +184,12 @@ typedef struct orc_sparse_scene {
 #define ORC_FLAG_BISECT 8u
 #define ORC_FLAG_ANALYTIC_HIT 16u
 
+/* Hash of a march's discrete decisions, one code per loop iteration (free
+ * skip, sample, near sample + candidate object) and per bisection halving:
+ * FP32 and FP64 rays with equal hashes took the same path, so their depth
+ * and normal differ by rounding alone. */
+#define ORC_PATH(h, code) ((h) = (h) * 31u + (uint32_t)(code))
+
 /* u32(f32(x)) with WGSL's saturating conversion. */
 static inline uint32_t orc_u32_via_f32(uint32_t x) {
     float f = (float)x;
@@ -209,6 +215,7 @@ typedef struct orc_ray_result {
     uint32_t evals;
     uint32_t shadow_steps;
     uint32_t shadow_evals;
+    uint32_t path;             /* ORC_PATH hash of the primary march */
     double shadow;
 } orc_ray_result;
 

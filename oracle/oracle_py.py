@@ -31,7 +31,7 @@ RAY_RESULT = np.dtype([
     ("steps", "<u4"),
     ("evals", "<u4"),
     ("shadow_steps", "<u4"),
-    ("shadow_evals", "<u4"), ("_pad1", "<u4"),
+    ("shadow_evals", "<u4"), ("path", "<u4"),
     ("shadow", "<f8"),
 ])
 assert RAY_RESULT.itemsize == 144

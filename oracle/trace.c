@@ -50,10 +50,10 @@ void orc_ref_trace(const orc_ref_scene *s, int prec, uint64_t p0, uint64_t p1, o
 void orc_ref_eval_object(const orc_ref_scene *s, int prec, uint32_t oi, const double p[3], double *dist,
                          uint32_t *mat) {
     if (prec == 64) {
-        rctx_f64 c = {s, 0, 0};
+        rctx_f64 c = {s, 0, 0, 0};
         *dist = eval_object_f64(&c, mk_f64(p[0], p[1], p[2]), oi, mat);
     } else {
-        rctx_f32 c = {s, 0, 0};
+        rctx_f32 c = {s, 0, 0, 0};
         *dist = (double)eval_object_f32(&c, mk_f32((float)p[0], (float)p[1], (float)p[2]), oi, mat);
     }
 }

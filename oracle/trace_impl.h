@@ -71,6 +71,7 @@ typedef struct {
     const orc_ref_scene *s;
     uint32_t evals;
     uint32_t flags; /* ORC_FLAG_*: which branches this ray's evaluations took */
+    uint32_t path;  /* ORC_PATH hash of the march's discrete decisions */
 } F(rctx);
 
 static inline uint32_t F(map_entry)(const orc_ref_scene *s, uint64_t i) {
@@ -278,6 +279,7 @@ static void F(bisect)(F(rctx) *c, V3 o, V3 sd, R lo, R hi, uint32_t oi, uint32_t
         V3 mp = F(add)(o, F(muls)(sd, mid));
         uint32_t mm;
         R md = F(eval_object)(c, mp, oi, &mm);
+        ORC_PATH(c->path, md < (R)0 ? 5u : 6u);
         if (md < (R)0) hi = mid;
         else { lo = mid; bm = mm; }
     }
@@ -319,6 +321,7 @@ static void F(march)(F(rctx) *c, V3 o, V3 dir, uint32_t px, uint32_t py, int mod
         if (mode == 0) {
             R cd = F(coarse)(s, F(sub)(pos, cam));
             if (cd > K_COARSE_NEAR) {
+                ORC_PATH(c->path, 1u);
                 prev_dist = K_MAX_FLOAT;
                 t += cd;
                 continue;
@@ -393,6 +396,7 @@ static void F(march)(F(rctx) *c, V3 o, V3 dir, uint32_t px, uint32_t py, int mod
             r->obj_idx = best_idx;
             return;
         }
+        ORC_PATH(c->path, (min_dist < K_NEAR ? 3u : 2u) + 8u * best_idx);
         if (min_dist < K_NEAR) near++;
         prev_t = t;
         prev_dist = min_dist;
@@ -459,7 +463,7 @@ static void F(motion)(const rkf_camera_uniforms *cam, uint32_t w, uint32_t h, ui
 }
 
 static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out) {
-    F(rctx) c = {s, 0, 0};
+    F(rctx) c = {s, 0, 0, 0};
     uint32_t px = (uint32_t)(p % s->width), py = (uint32_t)(p / s->width);
     V3 o, dir;
     F(camera_ray)(&s->cam, s->width, s->height, px, py, &o, &dir);
@@ -479,6 +483,7 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
     if (!r.hit) {
         out->evals = c.evals;
         out->flags = c.flags;
+        out->path = c.path;
         out->t = (double)K_MAX_FLOAT;
         return;
     }
@@ -502,6 +507,7 @@ static void F(ref_pixel)(const orc_ref_scene *s, uint64_t p, orc_ray_result *out
         sdf_at = F(sample_voxelized)(&c, lh, ho, 0);
     }
     out->evals = evals_march;
+    out->path = c.path;
     out->flags = c.flags | (ho->sdf_type == RKF_SDF_TYPE_ANALYTICAL ? ORC_FLAG_ANALYTIC_HIT : 0u);
     out->hit = 1;
     out->t = (double)r.t;
@@ -694,6 +700,7 @@ typedef struct {
     V3 mat_p;
     int hit;
     uint32_t steps;
+    uint32_t path; /* ORC_PATH hash of the march's discrete decisions */
 } F(sres);
 
 /* Sparse march: ray_march_tiled's step rules (:905-982) over one field. */
@@ -817,6 +824,7 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
     r->hit = 0;
     r->t = K_MAX_FLOAT;
     r->steps = 0;
+    r->path = 0;
     V3 sd = F(mk)(FABS(dir.x) < (R)1e-10f ? (R)1e-10f : dir.x, FABS(dir.y) < (R)1e-10f ? (R)1e-10f : dir.y,
                   FABS(dir.z) < (R)1e-10f ? (R)1e-10f : dir.z);
     R t = 0, prev_t = 0, prev_dist = K_MAX_FLOAT;
@@ -841,6 +849,7 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
         V3 pos = F(add)(o, F(muls)(sd, t));
         R tx;
         if (F(sp_free)(c, pos, sd, inv, t, &tx)) {
+            ORC_PATH(r->path, 1u);
             prev_t = tx;
             prev_dist = vs * (R)8.0f;
             t = tx + nudge;
@@ -853,6 +862,7 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
             for (uint32_t b = 0; b < K_BISECT; b++) {
                 R mid = (lo + hi) * (R)0.5f;
                 R mdd = F(sp_sample)(c, F(add)(o, F(muls)(sd, mid)), 0, 1);
+                ORC_PATH(r->path, mdd < (R)0 ? 5u : 6u);
                 if (mdd < (R)0) hi = mid;
                 else { lo = mid; mt = mid; md = mdd; }
             }
@@ -869,6 +879,7 @@ static void F(sp_march)(F(sctx) *c, V3 o, V3 dir, const rkf_scene_uniforms *su, 
             r->mat_d = d;
             return;
         }
+        ORC_PATH(r->path, d < K_NEAR ? 3u : 2u);
         if (d < K_NEAR) near++;
         prev_t = t;
         prev_dist = d;
@@ -930,6 +941,7 @@ static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
     out->cell_index = 0xFFFFFFFFu;
     out->evals = c.evals;
     out->flags = c.flags;
+    out->path = r.path;
     if (!r.hit) {
         out->t = (double)K_MAX_FLOAT;
         out->shadow = 0;

@@ -65,9 +65,12 @@ def compare_ref(gpu, res):
 
 
 def depth_normal_errors(t_gpu, n_gpu, res64, res32):
-    """Over rays whose discrete path (hit, step count) is the same in FP32
-    and FP64: (same-path share, relative depth errors, normal angles)."""
-    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    """Over rays whose discrete path (hit, step count and the hash of every
+    branch decision of the march) is the same in FP32 and FP64: (same-path
+    share, relative depth errors, normal angles). A ray whose FP32 and FP64
+    decisions differ (a sample on either side of a threshold or of a free
+    region's face) is counted outside the same-path share."""
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"]) & (res64["path"] == res32["path"])
     hit = same & (res32["hit"] == 1) & (res64["t"] > 0)
     t = np.asarray(t_gpu, np.float64)
     rel = np.abs(t[hit] - res64["t"][hit]) / res64["t"][hit]

@@ -67,7 +67,7 @@ def _compare(gpu, res, width, height):
 
 
 def _tolerance(gpu, res64, res32):
-    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"]) & (res64["path"] == res32["path"])
     hit = same & (res32["hit"] == 1)
     t = gpu["position"][:, 3].astype(np.float64)
     rel = np.abs(t[hit] - res64["t"][hit]) / res64["t"][hit]

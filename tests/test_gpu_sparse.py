@@ -198,7 +198,7 @@ def test_sparse_march_parity(ctx, small_scene, variant, cull):
     assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
     assert np.array_equal(gpu["shadow"].view(np.uint32), res32["shadow"].astype(np.float32).view(np.uint32))
     res64 = O.sparse_trace(sp, cam, su, ext, w, h, prec=64)
-    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"])
+    same = (res64["hit"] == res32["hit"]) & (res64["steps"] == res32["steps"]) & (res64["path"] == res32["path"])
     both = same & hit
     rel = np.abs(gpu["position"][both, 3] - res64["t"][both]) / res64["t"][both]
     n = gpu["debug"]["normal"][both].astype(np.float64)

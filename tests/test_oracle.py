@@ -255,7 +255,7 @@ def test_fp32_twin_agrees_with_fp64_oracle():
     inp, _ = _single_sphere_scene(rel=True)
     a = O.ref_trace(inp, prec=32)
     b = O.ref_trace(inp, prec=64)
-    same = (a["hit"] == b["hit"]) & (a["steps"] == b["steps"])
+    same = (a["hit"] == b["hit"]) & (a["steps"] == b["steps"]) & (a["path"] == b["path"])
     assert same.mean() > 0.98
     hit = same & (a["hit"] == 1)
     rel = np.abs(a["t"][hit] - b["t"][hit]) / b["t"][hit]
