@@ -396,6 +396,27 @@ int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                       const rkfm_gbuffer *out, uint32_t variant,
                       void *hip_stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). rkfm_sparse_march (variants 0 and 1 only) with the scratch of
+ * RKFM_MARCH_TILE_CULL (per-tile ray intervals, large-rectangle list and its
+ * counter) in a fixed device array of RKFM_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each device until the process ends, instead of context
+ * buffers that grow: it calls no allocating routine. Culled frames are
+ * limited to RKFM_NOALLOC_MAX_TILES 8x8 tiles and scenes to
+ * RKFM_NOALLOC_MAX_BLOCKS L1 blocks (RKFM_ERR_OUT_OF_RANGE beyond). Every
+ * call on one device shares that device's array: calls that overlap on one
+ * device (other streams, contexts or threads) race. The results equal
+ * rkfm_sparse_march's bit for bit. */
+#define RKFM_NOALLOC_SCRATCH_BYTES 8389120u /* 256 + 262144 * 8 + 262144 * 24 */
+#define RKFM_NOALLOC_MAX_TILES 262144u      /* e.g. 4096 x 4096 pixels       */
+#define RKFM_NOALLOC_MAX_BLOCKS 262144u
+int rkfm_sparse_march_noalloc(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                              const rkf_camera_uniforms *camera,
+                              const rkf_scene_uniforms *su,
+                              const rkfm_scene_ext *ext, const rkfm_gbuffer *out,
+                              uint32_t variant, void *hip_stream);
+
 /* Per-tile ray intervals used by RKFM_MARCH_TILE_CULL: tiles_dev receives
  * ceil(width/8) * ceil(height/8) pairs (t_lo, t_hi) of floats, row-major.
  * For each L2 cell holding an L1 block, its box widened by 9 vs (by 25 vs on

@@ -86,6 +86,14 @@ int rkfm_create(const rkfm_config *cfg, rkfm_ctx **out) {
     }
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
+    int rc = rkf_noalloc_init(dev);
+    if (rc) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     c->su.num_objects = 0;
     c->su.max_steps = 128;
     c->su.max_distance = 100.0f;

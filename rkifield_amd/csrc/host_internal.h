@@ -88,6 +88,9 @@ int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, con
 extern "C" int rkf_tile_camera_check(const rkf_camera_uniforms *cam);
 // Persistent-grid size: per_cu wavefronts on every CU of the current device.
 int rkf_device_waves(uint32_t per_cu, uint32_t &waves);
+// Looks up this device's copy of the allocation-free march's scratch array
+// (noalloc.hip); called by rkfm_create with the device current.
+int rkf_noalloc_init(int device);
 // Enqueues the batch kernel (and the counter reset of the persistent
 // schedule) over device copies of the descriptors and begins.
 int rkf_batch_enqueue(const rkfm_batch_tuning &t, const rkf::BatchDev *items, const uint32_t *begins, uint32_t n,

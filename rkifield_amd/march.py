@@ -74,6 +74,7 @@ _SIGS = {
     "rkfm_sparse_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_download_free": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
+    "rkfm_sparse_march_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_tile_intervals": (C.c_int, [_VP, _VP, _VP, _VP, _U32, _U32, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
@@ -518,6 +519,15 @@ class SparseScene:
         e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
         _check(lib().rkfm_sparse_march(self.ctx.h, self.h, _p(c), _p(s), _p(e), gbuffer.desc.ctypes.data,
                                        variant, stream))
+
+    def march_noalloc(self, cam, su, ext, gbuffer, variant=0, stream=None):
+        """rkfm_sparse_march_noalloc: the same march with its scratch in the
+        library's fixed per-device array (no device allocation)."""
+        c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+        s = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+        e = np.ascontiguousarray(ext, L.SCENE_EXT) if ext is not None else None
+        _check(lib().rkfm_sparse_march_noalloc(self.ctx.h, self.h, _p(c), _p(s), _p(e), gbuffer.desc.ctypes.data,
+                                               variant, stream))
 
     def tile_intervals(self, cam, su, width, height):
         """(t_lo, t_hi) per 8x8 tile (rkfm_sparse_tile_intervals), shape
