@@ -447,6 +447,9 @@ def main():
                     help="config1: requests of the served-vs-plain replay reported beside the frame (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also launch the plain copy kernel on this many MiB (profiler counter calibration)")
+    ap.add_argument("--gather-mib", type=int, default=0,
+                    help="also launch the scattered 2-byte read kernel over this many MiB, one read per 128-B line "
+                         "(profiler counter calibration for gathers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path shards rays across GPUs with no exchange, "
@@ -527,6 +530,15 @@ def main():
             ctx.copy(b, a, nb)
         ctx.sync()
         log(f"copy kernel: 3 x {args.copy_mib} MiB")
+    if args.gather_mib:
+        nb = args.gather_mib << 20
+        src = ctx.array((nb,), np.uint8)
+        n = nb // 128
+        dst = ctx.array((n,), np.uint16)
+        for _ in range(3):
+            ctx.gather16(dst, src, nb, n)
+        ctx.sync()
+        log(f"gather16 kernel: 3 x {n} lines of {args.gather_mib} MiB")
 
     traffic = None
     tpath =os.path.join(ROOT, "profiles", f"{kernel}_{args.workload}_traffic.json")

@@ -807,6 +807,13 @@ int rkfm_memset(rkfm_ctx *ctx, void *dst_dev, int value, uint64_t bytes);
  * measured HBM traffic calibrates profiler byte counters. */
 int rkfm_device_copy(rkfm_ctx *ctx, void *dst_dev, const void *src_dev,
                      uint64_t bytes);
+/* Scattered-read kernel (n <= src_bytes / 128 lanes, src_bytes / 128 a power
+ * of two): lane i reads the first 2 bytes of 128-byte line (i * 0x9E3779B1)
+ * mod (src_bytes / 128) of src (every line at most once) into dst[i] (2 B per
+ * lane). Its measured traffic calibrates profiler byte counters for
+ * scattered 2-byte loads like the march's distance gathers. */
+int rkfm_device_gather16(rkfm_ctx *ctx, void *dst_dev, const void *src_dev,
+                         uint64_t src_bytes, uint64_t n);
 /* HIP events on the context stream around the work enqueued in between. */
 int rkfm_timer_begin(rkfm_ctx *ctx);
 int rkfm_timer_end(rkfm_ctx *ctx, float *ms);

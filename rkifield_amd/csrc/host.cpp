@@ -433,6 +433,15 @@ int rkfm_device_copy(rkfm_ctx *c, void *dst, const void *src, uint64_t bytes) {
     return rkf_hip_check(rkf_launch_copy(dst, src, bytes, c->stream), "copy launch");
 }
 
+int rkfm_device_gather16(rkfm_ctx *c, void *dst, const void *src, uint64_t src_bytes, uint64_t n) {
+    if (!c || !dst || !src) return rkf_set_error(RKFM_ERR_INVALID_ARG, "device_gather16: null argument");
+    const uint64_t lines = src_bytes / 128u;
+    if (lines == 0 || (lines & (lines - 1u)) || n > lines)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "device_gather16: %llu lines (a power of two >= n needed)",
+                             (unsigned long long)lines);
+    return rkf_hip_check(rkf_launch_gather16(dst, src, lines, n, c->stream), "gather16 launch");
+}
+
 // ---- timing on the context stream ----
 int rkfm_timer_begin(rkfm_ctx *c) {
     if (!c) return rkf_set_error(RKFM_ERR_INVALID_ARG, "timer: null ctx");

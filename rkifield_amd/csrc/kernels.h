@@ -106,6 +106,7 @@ struct BatchDev {
 extern "C" {
 hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream);
 hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream);
+hipError_t rkf_launch_gather16(void *dst, const void *src, uint64_t n_lines, uint64_t n, hipStream_t stream);
 hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream);
 hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream);
 hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,

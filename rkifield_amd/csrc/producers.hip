@@ -116,7 +116,30 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4 *__restrict__ dst, cons
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Scattered 2-byte reads, one per 128-byte line: lane i reads the first
+// binary16 of line (i * 0x9E3779B1) mod n_lines (a permutation of the lines,
+// n_lines a power of two) and writes it to dst[i]. Its measured traffic
+// calibrates the byte counters for the march's scattered distance loads.
+__global__ __launch_bounds__(256) void gather16_kernel(uint16_t *__restrict__ dst, const uint16_t *__restrict__ src,
+                                                       uint64_t n_lines, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
+        const uint64_t line = (i * 0x9E3779B1ull) & (n_lines - 1u);
+        dst[i] = src[line * 64u];
+    }
+}
+
 }  // namespace rkf
+
+extern "C" hipError_t rkf_launch_gather16(void *dst, const void *src, uint64_t n_lines, uint64_t n,
+                                          hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    hipLaunchKernelGGL(rkf::gather16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (uint16_t *)dst,
+                       (const uint16_t *)src, n_lines, n);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream) {
     uint64_t n16 = bytes / 16u;

@@ -89,6 +89,7 @@ _SIGS = {
     "rkfm_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, _U64]),
     "rkfm_memset": (C.c_int, [_VP, _VP, C.c_int, _U64]),
     "rkfm_device_copy": (C.c_int, [_VP, _VP, _VP, _U64]),
+    "rkfm_device_gather16": (C.c_int, [_VP, _VP, _VP, _U64, _U64]),
     "rkfm_batch_run": (C.c_int, [_VP, _VP, _U32, _VP, _VP]),
     "rkfm_batch_autotune": (C.c_int, [_VP, _VP, _U32, _U32, _VP, C.POINTER(_F)]),
     "rkfm_batch_graph_create": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
@@ -182,6 +183,9 @@ class Context:
 
     def copy(self, dst, src, nbytes):
         _check(lib().rkfm_device_copy(self.h, dst.ptr, src.ptr, nbytes))
+
+    def gather16(self, dst, src, src_bytes, n):
+        _check(lib().rkfm_device_gather16(self.h, dst.ptr, src.ptr, src_bytes, n))
 
     def array(self, shape, dtype):
         return DeviceArray(self, shape, dtype)
