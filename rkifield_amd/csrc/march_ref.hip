@@ -269,6 +269,7 @@ RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, const rkf_gpu_ob
 // ray_march.wgsl:641-985. Tiled mode: the tile's `count` object records and
 // indices, staged in LDS by the workgroup (an 8x8 pixel workgroup lies in one
 // 16x16 object tile), so a step's object loop reads no global memory.
+template <uint32_t MODE>
 RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
                    MarchRes &r) {
     const RefParams &P = *c.P;
@@ -276,9 +277,9 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
     r.hit = false;
     r.mat = r.obj_id = r.obj_idx = 0;
     r.steps = 0;
-    if (P.mode == 0) {
+    if (MODE == 0) {
         if (count == 0) return;
-    } else if (P.mode == 1) {
+    } else if (MODE == 1) {
         if (P.su.num_objects == 0) return;
     }
     f3 sd = safe_dir(dir);
@@ -289,7 +290,7 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
         if (t > P.su.max_distance || near >= P.su.max_steps) break;
         r.steps++;
         f3 pos = add(o, muls(sd, t));
-        if (P.mode == 0) {
+        if (MODE == 0) {
             float cd = coarse(P, sub(pos, cam));
             if (cd > kCoarseNear) {
                 prev_dist = kMaxFloat;
@@ -299,7 +300,7 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
         }
         float min_dist = kMaxFloat;
         uint32_t best_mat = 0, best_id = 0, best_idx = 0, best_j = 0;
-        if (P.mode == 0) {
+        if (MODE == 0) {
             for (uint32_t i = 0; i < count; i++) {
                 const uint32_t oi = toi[i];
                 uint32_t m;
@@ -312,7 +313,7 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
                     best_j = i;
                 }
             }
-        } else if (P.mode == 1) {
+        } else if (MODE == 1) {
             uint32_t stack[32];
             uint32_t sp = 1;
             stack[0] = 0;
@@ -355,7 +356,7 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
         }
         if (min_dist < 0.0f && prev_dist > hit_eps(P.eps_mode, P.su.hit_threshold, prev_t)) {
             const rkf_gpu_object &bo =
-                P.mode == 0 ? tobj[best_j] : P.objects[best_idx < P.su.num_objects ? best_idx : 0u];
+                MODE == 0 ? tobj[best_j] : P.objects[best_idx < P.su.num_objects ? best_idx : 0u];
             bisect(c, o, sd, prev_t, t, bo, best_idx, best_mat, best_id, r);
             return;
         }
@@ -400,9 +401,11 @@ RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t oi) {
     return normalize(mk(nx, ny, nz));
 }
 
-// ray_march.wgsl:1036-1111. Block = 64 lanes = 8x8 pixels. Held to 128 VGPRs
-// (4 waves/SIMD instead of 3; the few spills stay out of the per-object
-// loop): a 640x360 view is 3600 waves, which then all fit the 256 CUs at once.
+// ray_march.wgsl:1036-1111, one instance per strategy (MODE 0 tiled, 1 BVH,
+// 2 brute; :1056-1063 picks it, the host launches that instance). Block = 64
+// lanes = 8x8 pixels. Held to 128 VGPRs (4 waves/SIMD): a 640x360 view is
+// 3600 waves, which then all fit the 256 CUs at once.
+template <uint32_t MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
     __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
     __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     uint32_t py = blockIdx.y * 8u + (lane >> 3);
     const uint32_t W = P.g.width, H = P.g.height;
     uint32_t count = 0;
-    if (P.mode == 0) {
+    if (MODE == 0) {
         // Stage the tile's object list and records (64 lanes, one dword of a
         // 256-byte record each; records past num_objects are never read).
         const uint32_t ntx = (W + 15u) / 16u;
@@ -442,7 +445,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     f3 o, dir;
     camera_ray(P.cam, W, H, px, py, o, dir);
     MarchRes r;
-    march(c, o, dir, s_obj, s_oi, count, r);
+    march<MODE>(c, o, dir, s_obj, s_oi, count, r);
     if (!r.hit) {
         store_miss(P.g, p);
         store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
@@ -474,6 +477,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 
 extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream) {
     dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
-    hipLaunchKernelGGL(rkf::ray_march_ref_kernel, grid, dim3(64, 1, 1), 0, stream, *P);
+    if (P->mode == 0)
+        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<0>, grid, dim3(64, 1, 1), 0, stream, *P);
+    else if (P->mode == 1)
+        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<1>, grid, dim3(64, 1, 1), 0, stream, *P);
+    else
+        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<2>, grid, dim3(64, 1, 1), 0, stream, *P);
     return hipGetLastError();
 }

@@ -331,6 +331,53 @@ def test_sparse_field_equals_reference_field_at_origin():
     assert np.array_equal(ref["material_id"][hit], spr["material_id"][hit])
 
 
+def test_sparse_dda_trace_stays_in_the_hit_band_of_the_plain_trace():
+    """The sparse march (free-space DDA) against the reference's plain trace
+    (ray_march_brute semantics, :776-858) of the same field, both FP64, over
+    test_cross.rkf from four directions with absolute and relative hit
+    thresholds: the DDA moves only the steps taken through free space, so
+    hit masks and materials are equal, and the hit points differ along the
+    surface normal by less than 2 hit thresholds (measured up to 1.57); the
+    depth then differs by up to 1.6e-3 relative and up to 2.5 % of the hits
+    land in another voxel of the band (this bound, not bit equality, is what
+    the sparse path promises against the plain trace)."""
+    from rkifield_amd import march as M
+    data = open(os.path.join(GOLDEN, "test_cross.rkf"), "rb").read()
+    h, m, b = O.rkf_load(data)
+    vs = np.float32(h["voxel_size"])
+    half = h["dims"].astype(np.float32) * (vs * np.float32(8.0)) * np.float32(0.5)
+    sp = _sparse_from_dense_host(h["dims"], m, b, vs, -half)
+    obj = np.zeros(1, L.GPU_OBJECT)
+    obj["inverse_world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    obj["brick_map_dims"] = h["dims"]
+    obj["voxel_size"] = vs
+    obj["sdf_type"] = L.SDF_TYPE_VOXELIZED
+    obj["accumulated_scale"] = (1, 1, 1)
+    su = L.scene_uniforms(num_objects=1)
+    w, hh = 160, 120
+    worst, vox_share = 0.0, 0.0
+    for mode in (L.EPS_ABSOLUTE, L.EPS_RELATIVE):
+        for eye in ((0.3, 1.0, 1.6), (1.8, 0.6, -0.4), (-1.2, -0.9, 1.1), (0.1, 2.2, 0.2)):
+            cu = M.camera_look_at(eye, (0, 0, 0), 60.0, w, hh)
+            ref = O.ref_trace(O.RefInputs(b, m, obj, cu, su, L.scene_ext(strategy=L.STRATEGY_BRUTE, hit_eps_mode=mode),
+                                          w, hh), prec=64)
+            spr = O.sparse_trace(sp, cu, su, L.scene_ext(hit_eps_mode=mode), w, hh, prec=64)
+            hit = ref["hit"] == 1
+            assert hit.sum() > 2000
+            assert np.array_equal(ref["hit"], spr["hit"])
+            assert np.array_equal(ref["material_id"][hit], spr["material_id"][hit])
+            d = ref["pos"][hit] - np.asarray(eye)
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            cosi = np.abs(np.sum(d * ref["normal"][hit], axis=1))
+            eps = 1e-3 * ref["t"][hit] if mode == L.EPS_RELATIVE else np.full(int(hit.sum()), 1e-3)
+            along = np.abs(ref["t"][hit] - spr["t"][hit]) * cosi / eps
+            worst = max(worst, float(along.max()))
+            vox = (ref["voxel_index"][hit] != spr["voxel_index"][hit]) | (ref["brick_slot"][hit] != spr["brick_slot"][hit])
+            vox_share = max(vox_share, float(vox.mean()))
+    assert worst < 2.0, worst
+    assert vox_share < 0.05, vox_share
+
+
 def test_octant_extents_match_brute_force():
     """orc_sparse_oct against its definition: for a free cell C and octant o,
     E = min over non-free cells M in o's quadrant of max_a |M_a - C_a|, capped
