@@ -373,6 +373,29 @@ int rkfm_sparse_from_dense(rkfm_ctx *ctx, const uint32_t *map,
                            const rkf_voxel_sample *bricks, uint64_t n_bricks,
                            float voxel_size, const float origin[3],
                            rkfm_sparse_scene **out);
+/* Reference scene -> resident sparse scene (AoS -> SoA; the renderer's
+ * per-frame pool and object upload, render.rs:106-137, becomes a one-time
+ * conversion). Resamples the reference-layout scene uploaded to this context
+ * (brick pool, brick maps, objects and su.num_objects: what rkfm_dispatch
+ * marches) onto a sparse scene of voxel size vs whose grid corner is
+ * bounds_min. The field at a point is the minimum of evaluate_object
+ * (ray_march.wgsl:600-634: inverse_world, analytic primitive or trilinear
+ * voxel field with its sentinels, times min(accumulated_scale)) over the
+ * objects whose AABB widened by 3 bricks meets the point's L2 cell, in
+ * ascending index (the AABB must bound where an object's distance falls
+ * below 3 bricks, as the coarse field and tile cull assume). A brick is
+ * stored iff |field at its centre| < 1.8 brick extents (voxelize_object.rs:
+ * 86,104); each voxel stores binary16(field at its centre), the material
+ * evaluate_object returns for the first object reaching the minimum, and (in
+ * an aux plane, present when some voxel has them) that object's nearest
+ * voxel's secondary material and blend weight. */
+int rkfm_sparse_from_objects(rkfm_ctx *ctx, float voxel_size,
+                             const float bounds_min[3], const float bounds_max[3],
+                             rkfm_sparse_scene **out);
+/* The scene's aux plane (n_bricks * 512 u16: secondary | blend << 8) when it
+ * has one (*present = 1); aux may be NULL to ask. */
+int rkfm_sparse_download_aux(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                             uint16_t *aux, uint32_t *present);
 int rkfm_sparse_info_get(const rkfm_sparse_scene *scene,
                          rkfm_sparse_info *info);
 int rkfm_sparse_destroy(rkfm_ctx *ctx, rkfm_sparse_scene *scene);

@@ -282,6 +282,16 @@ double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec);
 void orc_ref_eval_object(const orc_ref_scene *s, int prec, uint32_t oi, const double p[3], double *dist,
                          uint32_t *mat);
 
+/* rkfm_sparse_from_objects' resampling of a reference scene (objects binned
+ * by AABB + 3 bricks into L2 cells, field = min of evaluate_object, stored
+ * bricks |field(centre)| < 1.8 bricks, slots in cell then z-y-x order), FP32.
+ * Call with l2 == NULL to size (cells, n_blocks, n_bricks); then l2 holds
+ * cells product u32, l1 n_blocks * 512, dist/aux n_bricks * 512 u16, mat
+ * n_bricks * 512 u8 (aux may be NULL). */
+int orc_sparse_from_objects(const orc_ref_scene *s, float vs, const float bmin[3], const float bmax[3],
+                            uint32_t cells[3], uint32_t *n_blocks, uint64_t *n_bricks, uint32_t *l2, uint32_t *l1,
+                            uint16_t *dist, uint8_t *mat, uint16_t *aux);
+
 /* Pack unpacked results into the reference G-buffer formats. */
 void orc_pack_gbuffer(const orc_ray_result *r, uint64_t n, float *position,
                       uint16_t *normal, uint32_t *material, float *motion);

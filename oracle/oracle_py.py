@@ -273,6 +273,25 @@ def ref_eval_object(inp: RefInputs, oi, pts, prec=64):
     return d, m
 
 
+def sparse_from_objects(inp: RefInputs, vs, bmin, bmax):
+    """orc_sparse_from_objects: (cells, l2, l1, dist, mat, aux) of the
+    resampled scene."""
+    f = lib().orc_sparse_from_objects
+    f.argtypes = [C.c_void_p, C.c_float] + [C.c_void_p] * 10
+    lo, hi = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+    cells = np.zeros(3, np.uint32)
+    nb = np.zeros(1, np.uint32)
+    nr = np.zeros(1, np.uint64)
+    f(C.byref(inp.s), vs, _p(lo), _p(hi), _p(cells), _p(nb), _p(nr), None, None, None, None, None)
+    l2 = np.zeros(int(np.prod(cells.astype(np.int64))), np.uint32)
+    l1 = np.zeros(int(nb[0]) * 512, np.uint32)
+    dist = np.zeros(int(nr[0]) * 512, np.uint16)
+    mat = np.zeros(int(nr[0]) * 512, np.uint8)
+    aux = np.zeros(int(nr[0]) * 512, np.uint16)
+    f(C.byref(inp.s), vs, _p(lo), _p(hi), _p(cells), _p(nb), _p(nr), _p(l2), _p(l1), _p(dist), _p(mat), _p(aux))
+    return cells, l2, l1, dist, mat, aux
+
+
 def ref_trace_parallel(inp: RefInputs, threads=0):
     out = np.zeros(inp.width * inp.height, RAY_RESULT)
     used = lib().orc_ref_trace_parallel(C.byref(inp.s), _p(out), C.c_int(threads))

@@ -58,6 +58,123 @@ void orc_ref_eval_object(const orc_ref_scene *s, int prec, uint32_t oi, const do
     }
 }
 
+/* ---- reference scene -> sparse scene (rkfm_sparse_from_objects) ---- */
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+/* min over the cell's candidate objects (ascending index) of evaluate_object;
+ * *win = first object reaching it (UINT32_MAX if none), *mat its material. */
+static float objects_field_f32(const orc_ref_scene *s, const uint32_t *objs, uint32_t n, v3_f32 p, uint32_t *win,
+                               uint32_t *mat) {
+    rctx_f32 c = {s, 0, 0, 0};
+    float best = INFINITY;
+    *win = UINT32_MAX;
+    *mat = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t m;
+        float d = eval_object_f32(&c, p, objs[k], &m);
+        if (d < best) {
+            best = d;
+            *win = objs[k];
+            *mat = m;
+        }
+    }
+    return best;
+}
+
+int orc_sparse_from_objects(const orc_ref_scene *s, float vs, const float bmin[3], const float bmax[3],
+                            uint32_t cells[3], uint32_t *n_blocks, uint64_t *n_bricks, uint32_t *l2, uint32_t *l1,
+                            uint16_t *dist, uint8_t *mat, uint16_t *aux) {
+    float bws = vs * 8.0f, cws = bws * 8.0f;
+    for (int a = 0; a < 3; a++) {
+        double cc = ceil((double)(bmax[a] - bmin[a]) / (double)cws);
+        cells[a] = cc < 1.0 ? 1u : (uint32_t)cc;
+    }
+    uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
+    uint32_t nobj = s->su.num_objects;
+    double M = 3.0 * (double)bws;
+    /* (cell << 32 | object) pairs, sorted: the candidate lists */
+    uint64_t cap = 1024, np = 0;
+    uint64_t *pairs = (uint64_t *)malloc(cap * 8);
+    for (uint32_t j = 0; j < nobj; j++) {
+        const rkf_gpu_object *o = &s->objects[j];
+        if (o->sdf_type == RKF_SDF_TYPE_NONE) continue;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)floor(((double)o->aabb_min[a] - M - (double)bmin[a]) / (double)cws);
+            hi[a] = (int64_t)floor(((double)o->aabb_max[a] + M - (double)bmin[a]) / (double)cws);
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > (int64_t)cells[a] - 1) hi[a] = (int64_t)cells[a] - 1;
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    if (np == cap) { cap *= 2; pairs = (uint64_t *)realloc(pairs, cap * 8); }
+                    pairs[np++] = (((uint64_t)x + (uint64_t)y * cells[0] + (uint64_t)z * cells[0] * cells[1]) << 32) | j;
+                }
+    }
+    qsort(pairs, np, 8, cmp_u64);
+    uint32_t *objs = (uint32_t *)malloc((np + 1) * 4);
+    uint32_t blocks = 0;
+    uint64_t bricks = 0;
+    if (l2) for (uint64_t i = 0; i < ncell; i++) l2[i] = RKF_EMPTY_SLOT;
+    float hb = bws * 0.5f, half = vs * 0.5f;
+    for (uint64_t i = 0; i < np;) {
+        uint32_t cell = (uint32_t)(pairs[i] >> 32), n = 0;
+        while (i < np && (uint32_t)(pairs[i] >> 32) == cell) objs[n++] = (uint32_t)pairs[i++];
+        uint32_t qx = cell % cells[0], qy = (cell / cells[0]) % cells[1], qz = cell / (cells[0] * cells[1]);
+        uint32_t stored = 0;
+        for (uint32_t lb = 0; lb < 512u; lb++) {
+            uint32_t gbx = qx * 8u + (lb & 7u), gby = qy * 8u + ((lb >> 3) & 7u), gbz = qz * 8u + (lb >> 6);
+            v3_f32 c = mk_f32((bmin[0] + (float)gbx * bws) + hb, (bmin[1] + (float)gby * bws) + hb,
+                              (bmin[2] + (float)gbz * bws) + hb);
+            uint32_t win, m;
+            float d = objects_field_f32(s, objs, n, c, &win, &m);
+            int need = fabsf(d) < bws * 1.8f;
+            if (l2 && need) {
+                if (!stored) {
+                    l2[cell] = blocks;
+                    for (uint32_t k = 0; k < 512u; k++) l1[(uint64_t)blocks * 512u + k] = RKF_EMPTY_SLOT;
+                }
+                uint64_t slot = bricks + stored;
+                l1[(uint64_t)blocks * 512u + lb] = (uint32_t)slot;
+                for (uint32_t v = 0; v < 512u; v++) {
+                    uint32_t vx = v & 7u, vy = (v >> 3) & 7u, vz = v >> 6;
+                    v3_f32 p = mk_f32((bmin[0] + (float)gbx * bws) + ((float)vx * vs + half),
+                                      (bmin[1] + (float)gby * bws) + ((float)vy * vs + half),
+                                      (bmin[2] + (float)gbz * bws) + ((float)vz * vs + half));
+                    uint32_t w, mm;
+                    float dv = objects_field_f32(s, objs, n, p, &w, &mm);
+                    uint64_t idx = slot * 512u + v;
+                    dist[idx] = orc_f32_to_f16(dv);
+                    mat[idx] = (uint8_t)(mm & 0x3Fu);
+                    uint32_t a = 0;
+                    if (w != UINT32_MAX && s->objects[w].sdf_type == RKF_SDF_TYPE_VOXELIZED) {
+                        const rkf_gpu_object *o = &s->objects[w];
+                        uint32_t sl, vo;
+                        uint64_t vi = nearest_voxel_f32(s, xform_point_f32(o->inverse_world, p), o, &sl, &vo);
+                        if (vi != UINT64_MAX)
+                            a = ((pool_word0_f32(s, vi) >> 22) & 0x3Fu) | (((pool_word1_f32(s, vi) >> 24) & 0xFFu) << 8);
+                    }
+                    if (aux) aux[idx] = (uint16_t)a;
+                }
+            }
+            stored += need;
+        }
+        if (stored) {
+            blocks++;
+            bricks += stored;
+        }
+    }
+    free(pairs);
+    free(objs);
+    *n_blocks = blocks;
+    *n_bricks = bricks;
+    return 0;
+}
+
 int orc_ref_trace_parallel(const orc_ref_scene *s, orc_ray_result *out, int threads) {
     uint64_t n = (uint64_t)s->width * s->height;
     int used = 1;

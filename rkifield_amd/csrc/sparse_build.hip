@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "host_internal.h"
+#include "ref_eval.h"
 
 namespace rkf {
 
@@ -491,6 +492,102 @@ __global__ __launch_bounds__(256) void cellword_kernel(const uint32_t *free_idx,
     uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     cellw[i] = ((uint64_t)free_idx[i] << 32) | (uint64_t)oct[i];
+}
+
+// ---- reference scene -> sparse scene (rkfm_sparse_from_objects) ----
+// The field is the minimum of evaluate_object (ray_march.wgsl:600-634) over
+// the candidate objects of the point's L2 cell (the objects whose AABB,
+// widened by 3 bricks, meets the cell), in ascending object index; the first
+// object reaching the minimum gives the material (evaluate_object's) and the
+// aux word (secondary material | blend weight << 8 of its nearest voxel,
+// :570-596).
+struct ObjBuildDev {
+    RefParams R;                // pool, maps, objects, su.num_objects
+    const uint32_t *cand_cell;  // per candidate cell: flat L2 index
+    const uint32_t *cand_off;   // CSR into cand_objs (n_cand + 1)
+    const uint32_t *cand_objs;
+    float origin[3];
+    float vs;
+    uint32_t cells[3];
+};
+
+RKF_DEV float objects_field(const ObjBuildDev &B, uint32_t cand, f3 p, uint32_t &win, uint32_t &mat) {
+    RefCtx c{&B.R, 0u};
+    float best = INFINITY;
+    win = ~0u;
+    mat = 0u;
+    for (uint32_t k = B.cand_off[cand]; k < B.cand_off[cand + 1]; k++) {
+        const uint32_t oi = B.cand_objs[k];
+        uint32_t m;
+        const float d = eval_object(c, p, oi, m);
+        if (d < best) {
+            best = d;
+            win = oi;
+            mat = m;
+        }
+    }
+    return best;
+}
+
+// One block per candidate cell, one lane per brick: stored iff |field at the
+// brick centre| < 1.8 brick extents (voxelize_object.rs:86,104).
+__global__ __launch_bounds__(512) void obj_alloc_kernel(ObjBuildDev B, uint32_t ncand, uint8_t *flags,
+                                                        uint32_t *counts) {
+    __shared__ uint32_t wsum[8];
+    uint64_t item = item_index();
+    if (item >= ncand) return;
+    uint32_t cand = (uint32_t)item, t = threadIdx.x;
+    uint32_t cell = B.cand_cell[cand];
+    uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
+    uint32_t gbx = qx * 8u + (t & 7u), gby = qy * 8u + ((t >> 3) & 7u), gbz = qz * 8u + (t >> 6);
+    float bws = B.vs * 8.0f;
+    float hb = bws * 0.5f;
+    f3 c = mk((B.origin[0] + (float)gbx * bws) + hb, (B.origin[1] + (float)gby * bws) + hb,
+              (B.origin[2] + (float)gbz * bws) + hb);
+    uint32_t win, m;
+    float d = objects_field(B, cand, c, win, m);
+    bool need = fabsf(d) < bws * 1.8f;
+    flags[(uint64_t)cand * 512u + t] = need ? 1 : 0;
+    uint64_t ball = __ballot(need);
+    if ((t & 63u) == 0) wsum[t >> 6] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t n = 0;
+        for (int w = 0; w < 8; w++) n += wsum[w];
+        counts[cand] = n;
+    }
+}
+
+// One block per stored brick, one lane per voxel (centre brick_min + (v + 0.5) vs).
+__global__ __launch_bounds__(512) void obj_fill_kernel(ObjBuildDev B, uint64_t nbricks, const uint32_t *slot_cand,
+                                                       const uint32_t *slot_brick, uint16_t *dist, uint8_t *mat,
+                                                       uint16_t *aux, uint32_t *any_aux) {
+    uint64_t item = item_index();
+    if (item >= nbricks) return;
+    uint32_t slot = (uint32_t)item, v = threadIdx.x;
+    uint32_t cand = slot_cand[slot], lb = slot_brick[slot];
+    uint32_t cell = B.cand_cell[cand];
+    uint32_t qx = cell % B.cells[0], qy = (cell / B.cells[0]) % B.cells[1], qz = cell / (B.cells[0] * B.cells[1]);
+    uint32_t gbx = qx * 8u + (lb & 7u), gby = qy * 8u + ((lb >> 3) & 7u), gbz = qz * 8u + (lb >> 6);
+    uint32_t vx = v & 7u, vy = (v >> 3) & 7u, vz = v >> 6;
+    float vs = B.vs, bws = vs * 8.0f, half = vs * 0.5f;
+    f3 p = mk((B.origin[0] + (float)gbx * bws) + ((float)vx * vs + half),
+              (B.origin[1] + (float)gby * bws) + ((float)vy * vs + half),
+              (B.origin[2] + (float)gbz * bws) + ((float)vz * vs + half));
+    uint32_t win, m;
+    float d = objects_field(B, cand, p, win, m);
+    uint64_t idx = (uint64_t)slot * 512u + v;
+    dist[idx] = (uint16_t)f2h(d);
+    mat[idx] = (uint8_t)(m & 0x3Fu);
+    uint32_t a = 0;
+    if (win != ~0u && B.R.objects[win].sdf_type == RKF_SDF_TYPE_VOXELIZED) {
+        const rkf_gpu_object &o = B.R.objects[win];
+        uint32_t sl, vox;
+        uint64_t i = nearest_voxel(B.R, xform_point(o.inverse_world, p), o, sl, vox);
+        if (i != ~0ull) a = ((pool_w0(B.R, i) >> 22) & 0x3Fu) | (((pool_w1(B.R, i) >> 24) & 0xFFu) << 8);
+    }
+    aux[idx] = (uint16_t)a;
+    if (a) atomicOr(any_aux, 1u);
 }
 
 }  // namespace rkf
@@ -977,6 +1074,189 @@ int rkfm_sparse_from_dense(rkfm_ctx *c, const uint32_t *map, const uint32_t dims
     c->scenes.push_back(s);
     *out = s;
     return RKFM_OK;
+}
+
+int rkfm_sparse_from_objects(rkfm_ctx *c, float vs, const float bmin[3], const float bmax[3],
+                             rkfm_sparse_scene **out) {
+    if (!c || !bmin || !bmax || !out) return rkf_set_error(RKFM_ERR_INVALID_ARG, "from_objects: null argument");
+    *out = nullptr;
+    if (!(vs > 0.0f)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "from_objects: voxel size");
+    const uint32_t nobj = c->su.num_objects;
+    if (nobj > c->n_objects)
+        return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "from_objects: num_objects %u > %u uploaded", nobj, c->n_objects);
+    if (nobj == 0) return rkf_set_error(RKFM_ERR_NOT_READY, "from_objects: no objects");
+    const float bws = vs * 8.0f, cws = bws * 8.0f;
+    uint32_t cells[3];
+    for (int a = 0; a < 3; a++) {
+        float ext = bmax[a] - bmin[a];
+        if (!(ext > 0.0f)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "from_objects: empty bounds");
+        double cc = std::ceil((double)ext / (double)cws);
+        if (cc > 32768.0) return rkf_set_error(RKFM_ERR_INVALID_ARG, "from_objects: grid too large");
+        cells[a] = std::max(1u, (uint32_t)cc);
+    }
+    const uint64_t ncell = (uint64_t)cells[0] * cells[1] * cells[2];
+    if (ncell > 0xFFFFFFF0ull) return rkf_set_error(RKFM_ERR_INVALID_ARG, "from_objects: too many cells");
+    std::vector<rkf_gpu_object> objs(nobj);
+    int rc = rkf_hip_check(hipMemcpy(objs.data(), c->objects.ptr, (uint64_t)nobj * sizeof(rkf_gpu_object),
+                                     hipMemcpyDeviceToHost),
+                           "from_objects: object records");
+    if (rc) return rc;
+    // Bin every object into the cells its AABB, widened by 3 bricks, meets.
+    const double M = 3.0 * (double)bws;
+    std::vector<uint64_t> pairs;
+    for (uint32_t j = 0; j < nobj; j++) {
+        if (objs[j].sdf_type == RKF_SDF_TYPE_NONE) continue;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)std::floor(((double)objs[j].aabb_min[a] - M - (double)bmin[a]) / (double)cws);
+            hi[a] = (int64_t)std::floor(((double)objs[j].aabb_max[a] + M - (double)bmin[a]) / (double)cws);
+            lo[a] = std::max<int64_t>(lo[a], 0);
+            hi[a] = std::min<int64_t>(hi[a], (int64_t)cells[a] - 1);
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++)
+                    pairs.push_back((((uint64_t)x + (uint64_t)y * cells[0] + (uint64_t)z * cells[0] * cells[1]) << 32) |
+                                    j);
+    }
+    std::sort(pairs.begin(), pairs.end());
+    std::vector<uint32_t> cand_cell, cand_off, cand_objs;
+    for (uint64_t pr : pairs) {
+        uint32_t cell = (uint32_t)(pr >> 32);
+        if (cand_cell.empty() || cand_cell.back() != cell) {
+            cand_cell.push_back(cell);
+            cand_off.push_back((uint32_t)cand_objs.size());
+        }
+        cand_objs.push_back((uint32_t)(pr & 0xFFFFFFFFu));
+    }
+    cand_off.push_back((uint32_t)cand_objs.size());
+    const uint32_t ncand = (uint32_t)cand_cell.size();
+
+    rkfm_sparse_scene *s = new rkfm_sparse_scene();
+    s->owner = c;
+    for (int a = 0; a < 3; a++) {
+        s->info.origin[a] = bmin[a];
+        s->info.cells[a] = cells[a];
+    }
+    s->info.voxel_size = vs;
+    TmpBufs tmp;
+    DevBuf *d_cc = tmp.make(), *d_co = tmp.make(), *d_cg = tmp.make(), *d_flags = tmp.make(), *d_cnt = tmp.make();
+    DevBuf *d_blk = tmp.make(), *d_base = tmp.make(), *d_scand = tmp.make(), *d_sbrick = tmp.make();
+    DevBuf *d_any = tmp.make();
+    auto fail = [&](int e) {
+        release_scene(s);
+        delete s;
+        return e;
+    };
+    rc = put(c->stream, *d_cc, cand_cell.data(), cand_cell.size() * 4 + 4);
+    if (!rc) rc = put(c->stream, *d_co, cand_off.data(), cand_off.size() * 4);
+    if (!rc) rc = put(c->stream, *d_cg, cand_objs.data(), cand_objs.size() * 4 + 4);
+    if (!rc) rc = d_flags->reserve((uint64_t)ncand * 512 + 4, false);
+    if (!rc) rc = d_cnt->reserve((uint64_t)ncand * 4 + 4, false);
+    if (!rc) rc = d_any->reserve(256, false);
+    if (rc) return fail(rc);
+    rkf::ObjBuildDev B;
+    memset(&B, 0, sizeof(B));
+    B.R.pool = (const rkf_voxel_sample *)c->pool.ptr;
+    B.R.n_pool_voxels = c->n_pool_voxels;
+    B.R.maps = (const uint32_t *)c->maps.ptr;
+    B.R.n_maps = c->n_maps;
+    B.R.objects = (const rkf_gpu_object *)c->objects.ptr;
+    B.R.su = c->su;
+    B.cand_cell = (const uint32_t *)d_cc->ptr;
+    B.cand_off = (const uint32_t *)d_co->ptr;
+    B.cand_objs = (const uint32_t *)d_cg->ptr;
+    for (int a = 0; a < 3; a++) {
+        B.origin[a] = bmin[a];
+        B.cells[a] = cells[a];
+    }
+    B.vs = vs;
+    std::vector<uint32_t> counts(ncand);
+    if (ncand) {
+        hipLaunchKernelGGL(rkf::obj_alloc_kernel, rkf::item_grid(ncand), dim3(512), 0, c->stream, B, ncand,
+                           (uint8_t *)d_flags->ptr, (uint32_t *)d_cnt->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "obj_alloc launch");
+        if (!rc)
+            rc = rkf_hip_check(hipMemcpyAsync(counts.data(), d_cnt->ptr, (uint64_t)ncand * 4, hipMemcpyDeviceToHost,
+                                              c->stream),
+                               "counts download");
+        if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "counts sync");
+        if (rc) return fail(rc);
+    }
+    std::vector<uint32_t> blk_of(ncand, RKF_EMPTY_SLOT), base(ncand, 0), block_cells;
+    uint64_t nbricks = 0;
+    uint32_t nblocks = 0;
+    for (uint32_t i = 0; i < ncand; i++) {
+        if (!counts[i]) continue;
+        blk_of[i] = nblocks++;
+        block_cells.push_back(cand_cell[i]);
+        base[i] = (uint32_t)nbricks;
+        nbricks += counts[i];
+    }
+    if (nbricks >= 0xFFFFFFF0ull) return fail(rkf_set_error(RKFM_ERR_OOM, "from_objects: too many bricks"));
+    uint64_t need = ncell * 8 + (uint64_t)nblocks * 2048 + nbricks * 2560;
+    if (c->memory_budget && used_bytes(c) + need > c->memory_budget)
+        return fail(rkf_set_error(RKFM_ERR_OOM, "from_objects: %llu bytes over the context budget",
+                                  (unsigned long long)need));
+    rc = s->l2.reserve(ncell * 4, false);
+    if (!rc) rc = s->l1.reserve((uint64_t)nblocks * 2048 + 4, false);
+    if (!rc) rc = s->dist.reserve(nbricks * 1024 + 1024, false);  // the first brick is read by unused loads
+    if (!rc) rc = s->mat.reserve(nbricks * 512 + 4, false);
+    if (!rc) rc = s->aux.reserve(nbricks * 1024 + 1024, false);
+    if (!rc) rc = put(c->stream, *d_blk, blk_of.data(), (uint64_t)ncand * 4 + 4);
+    if (!rc) rc = put(c->stream, *d_base, base.data(), (uint64_t)ncand * 4 + 4);
+    if (!rc) rc = d_scand->reserve(nbricks * 4 + 4, false);
+    if (!rc) rc = d_sbrick->reserve(nbricks * 4 + 4, false);
+    if (rc) return fail(rc);
+    s->l2.size = ncell * 4;
+    s->l1.size = (uint64_t)nblocks * 2048;
+    s->dist.size = nbricks * 1024;
+    s->mat.size = nbricks * 512;
+    s->aux.size = nbricks * 1024;
+    rc = rkf_hip_check(hipMemsetAsync(s->l2.ptr, 0xFF, ncell * 4, c->stream), "l2 clear");
+    if (!rc) rc = rkf_hip_check(hipMemsetAsync(d_any->ptr, 0, 4, c->stream), "aux flag clear");
+    if (rc) return fail(rc);
+    rkf::BuildDev A;
+    memset(&A, 0, sizeof(A));
+    A.cand_cell = B.cand_cell;
+    if (ncand) {
+        hipLaunchKernelGGL(rkf::assign_kernel, rkf::item_grid(ncand), dim3(512), 0, c->stream, A, ncand,
+                           (const uint8_t *)d_flags->ptr, (const uint32_t *)d_blk->ptr, (const uint32_t *)d_base->ptr,
+                           (uint32_t *)s->l2.ptr, (uint32_t *)s->l1.ptr, (uint32_t *)d_scand->ptr,
+                           (uint32_t *)d_sbrick->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "assign launch");
+        if (rc) return fail(rc);
+    }
+    uint32_t any_aux = 0;
+    if (nbricks) {
+        hipLaunchKernelGGL(rkf::obj_fill_kernel, rkf::item_grid(nbricks), dim3(512), 0, c->stream, B, nbricks,
+                           (const uint32_t *)d_scand->ptr, (const uint32_t *)d_sbrick->ptr, (uint16_t *)s->dist.ptr,
+                           (uint8_t *)s->mat.ptr, (uint16_t *)s->aux.ptr, (uint32_t *)d_any->ptr);
+        rc = rkf_hip_check(hipGetLastError(), "obj_fill launch");
+        if (!rc)
+            rc = rkf_hip_check(hipMemcpyAsync(&any_aux, d_any->ptr, 4, hipMemcpyDeviceToHost, c->stream),
+                               "aux flag download");
+    }
+    if (!rc) rc = rkf_hip_check(hipStreamSynchronize(c->stream), "from_objects sync");
+    if (rc) return fail(rc);
+    if (!any_aux) s->aux.release();  // no blend data: no aux plane, as for primitive scenes
+    s->info.n_blocks = nblocks;
+    s->info.n_bricks = nbricks;
+    rc = build_free_masks(c, s, block_cells);
+    if (rc) return fail(rc);
+    finish_info(s);
+    c->scenes.push_back(s);
+    *out = s;
+    return RKFM_OK;
+}
+
+int rkfm_sparse_download_aux(rkfm_ctx *c, const rkfm_sparse_scene *s, uint16_t *aux, uint32_t *present) {
+    if (!c || !s || !present) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sparse_download_aux: null argument");
+    *present = s->aux.ptr ? 1u : 0u;
+    int rc = rkf_hip_check(hipStreamSynchronize(c->stream), "download sync");
+    if (!rc && aux && s->aux.ptr && s->info.n_bricks)
+        rc = rkf_hip_check(hipMemcpy(aux, s->aux.ptr, s->info.n_bricks * 1024, hipMemcpyDeviceToHost), "download aux");
+    return rc;
 }
 
 int rkfm_sparse_info_get(const rkfm_sparse_scene *s, rkfm_sparse_info *info) {

@@ -69,6 +69,8 @@ _SIGS = {
     "rkfm_build_bvh": (C.c_int, [_VP, _U32, _VP, _VP]),
     "rkfm_sparse_build": (C.c_int, [_VP, _VP, _U32, _VP, C.POINTER(_VP)]),
     "rkfm_sparse_from_dense": (C.c_int, [_VP, _VP, _VP, _VP, _U64, _F, _VP, C.POINTER(_VP)]),
+    "rkfm_sparse_from_objects": (C.c_int, [_VP, _F, _VP, _VP, C.POINTER(_VP)]),
+    "rkfm_sparse_download_aux": (C.c_int, [_VP, _VP, _VP, _VP]),
     "rkfm_sparse_info_get": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_destroy": (C.c_int, [_VP, _VP]),
     "rkfm_sparse_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
@@ -479,6 +481,26 @@ class SparseScene:
         _check(lib().rkfm_sparse_from_dense(ctx.h, _p(m), _p(d), _p(b), b.size // 512, voxel_size, _p(o),
                                             C.byref(h)))
         return cls(ctx, h.value)
+
+    @classmethod
+    def from_objects(cls, ctx, voxel_size, bounds_min, bounds_max):
+        """Resident scene resampled from the reference-layout scene uploaded
+        to ctx (rkfm_sparse_from_objects)."""
+        lo = np.asarray(bounds_min, np.float32)
+        hi = np.asarray(bounds_max, np.float32)
+        h = _VP()
+        _check(lib().rkfm_sparse_from_objects(ctx.h, voxel_size, _p(lo), _p(hi), C.byref(h)))
+        return cls(ctx, h.value)
+
+    def download_aux(self):
+        """The aux plane (n_bricks * 512 u16) or None when the scene has none."""
+        present = np.zeros(1, np.uint32)
+        _check(lib().rkfm_sparse_download_aux(self.ctx.h, self.h, None, _p(present)))
+        if not present[0]:
+            return None
+        aux = np.empty(int(self.info["n_bricks"]) * 512, np.uint16)
+        _check(lib().rkfm_sparse_download_aux(self.ctx.h, self.h, _p(aux), _p(present)))
+        return aux
 
     @classmethod
     def from_rkf(cls, ctx, data, lod=0, origin=None):
