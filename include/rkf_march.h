@@ -467,6 +467,50 @@ typedef struct rkfm_point_result {
 int rkfm_sparse_query_points(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                              const float *xyz_dev, uint32_t n,
                              rkfm_point_result *out_dev, void *hip_stream);
+
+/* Sphere casts (no reference counterpart: the reference answers point
+ * queries only, sdf_collision.rs:38-56): a ball of radius r swept from
+ * origin along the unit vector dir is sphere-traced against the stored field
+ * with a conservative distance D(p) at its centre p:
+ *   L(p) = a lower bound on the distance from p to any stored (non-EMPTY)
+ *          brick, from the free-space structures the DDA uses: p outside
+ *          the grid -> the distance to the grid box; p in a free L2 cell
+ *          whose 8 octant extents have minimum E -> the distance to the faces
+ *          of the free box of cells [q - E + 1, q + E) around it; p in a free
+ *          brick (minimum brick extent E_b) -> likewise over bricks; else 0.
+ *          The surface lies in stored bricks only (narrow band), so the true
+ *          distance is >= L(p); where the stored field is capped (EMPTY reads
+ *          8 vs) L lets a ball cross free space in large steps. D never
+ *          exceeds the distance to the surface, so a contact is never late;
+ *          for r above 8 vs it can come up to the narrow band (1.8 bricks)
+ *          early, where L stops at the stored bricks around the surface.
+ *   D(p) = L(p) when L(p) >= 16 vs (the sample there reads only EMPTY
+ *          corners), else max(stepping sample at p, L(p)).
+ * Starting at t = 0, a cast hits at the first t with D(p) - r < 1e-3; else
+ * t += max(D(p) - r, 5e-4), at most 1024 iterations, a miss past max_t.
+ * A hit's normal is the 6-tap central difference (2 vs, gradient sentinels)
+ * at the contact centre p = origin + t dir, normalised ((0, 1, 0) when
+ * shorter than 1e-10); the contact point is p - r * normal. */
+typedef struct rkfm_sphere_cast {
+    float origin[3];
+    float radius;
+    float dir[3];        /* unit vector                                    */
+    float max_t;
+} rkfm_sphere_cast;
+
+typedef struct rkfm_cast_result {
+    float t;             /* centre parameter of the contact (miss: max_t)  */
+    uint32_t hit;
+    uint32_t steps;      /* loop iterations                                */
+    float normal[3];     /* at the contact centre (miss: 0)                */
+    float distance;      /* D at the last centre evaluated                  */
+    uint32_t _pad;
+} rkfm_cast_result;
+
+/* casts and out are device pointers (n records each). */
+int rkfm_sparse_sphere_cast(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                            const rkfm_sphere_cast *casts_dev, uint32_t n,
+                            rkfm_cast_result *out_dev, void *hip_stream);
 /* Contact generation (rkf-physics generate_sdf_contacts,
  * sdf_collision.rs:188-225) for many rigid bodies against the scene's field:
  * each sample point of the body's shape (CollisionShape::sample_points,
@@ -864,6 +908,8 @@ static_assert(sizeof(rkfm_body) == 48, "body record");
 static_assert(sizeof(rkfm_rkf_header) == 128, "RkfFileHeader is 128 bytes");
 static_assert(sizeof(rkfm_rkf_lod) == 40, "LodFileEntry is 40 bytes");
 static_assert(sizeof(rkfm_contact) == 32, "contact record");
+static_assert(sizeof(rkfm_sphere_cast) == 32, "sphere cast record");
+static_assert(sizeof(rkfm_cast_result) == 32, "cast result record");
 #else
 _Static_assert(sizeof(rkf_voxel_sample) == 8, "VoxelSample is 8 bytes");
 _Static_assert(sizeof(rkf_gpu_object) == 256, "GpuObject is 256 bytes");
@@ -881,6 +927,8 @@ _Static_assert(sizeof(rkfm_body) == 48, "body record");
 _Static_assert(sizeof(rkfm_rkf_header) == 128, "RkfFileHeader is 128 bytes");
 _Static_assert(sizeof(rkfm_rkf_lod) == 40, "LodFileEntry is 40 bytes");
 _Static_assert(sizeof(rkfm_contact) == 32, "contact record");
+_Static_assert(sizeof(rkfm_sphere_cast) == 32, "sphere cast record");
+_Static_assert(sizeof(rkfm_cast_result) == 32, "cast result record");
 #endif
 
 #endif /* RKF_MARCH_H */

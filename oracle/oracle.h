@@ -274,6 +274,9 @@ void orc_sparse_tiles(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       int prec, double *out);
 void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t n, const float L[3],
                        float max_dist, float k, int prec, float *visibility);
+/* Sphere casts as rkfm_sparse_sphere_cast (prec 32: FP32 twin, 64: FP64). */
+void orc_sparse_cast(const orc_sparse_scene *s, const rkfm_sphere_cast *casts, uint64_t n, int prec,
+                     rkfm_cast_result *out);
 /* Sparse field value at a world point (stepping semantics). */
 double orc_sparse_sample(const orc_sparse_scene *s, const float p[3], int prec);
 

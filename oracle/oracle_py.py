@@ -376,6 +376,13 @@ def sparse_query(sp: SparseInputs, xyz, prec=32):
     return out
 
 
+def sparse_cast(sp: SparseInputs, casts, prec=32):
+    c = np.ascontiguousarray(casts, L.SPHERE_CAST)
+    out = np.zeros(c.size, L.CAST_RESULT)
+    lib().orc_sparse_cast(C.byref(sp.s), _p(c), C.c_uint64(c.size), C.c_int(prec), _p(out))
+    return out
+
+
 def sparse_shadow(sp: SparseInputs, origins, light, max_dist=25.0, k=16.0, prec=32):
     o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
     L_ = np.asarray(light, np.float32)

@@ -175,6 +175,14 @@ int orc_sparse_from_objects(const orc_ref_scene *s, float vs, const float bmin[3
     return 0;
 }
 
+void orc_sparse_cast(const orc_sparse_scene *s, const rkfm_sphere_cast *casts, uint64_t n, int prec,
+                     rkfm_cast_result *out) {
+    for (uint64_t i = 0; i < n; i++) {
+        if (prec == 64) sp_cast_f64(s, &casts[i], &out[i]);
+        else sp_cast_f32(s, &casts[i], &out[i]);
+    }
+}
+
 int orc_ref_trace_parallel(const orc_ref_scene *s, orc_ray_result *out, int threads) {
     uint64_t n = (uint64_t)s->width * s->height;
     int used = 1;

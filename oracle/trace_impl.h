@@ -1002,6 +1002,86 @@ static void F(sp_pixel)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
     }
 }
 
+/* Sphere casts (rkfm_sparse_sphere_cast's definition, include/rkf_march.h):
+ * L(p) from the free-space structures, D = L when L >= 16 vs, else
+ * max(stepping sample, L); hit at D - r < 1e-3. */
+static R F(sp_free_bound)(const F(sctx) *c, V3 p) {
+    const orc_sparse_scene *s = c->s;
+    R vs = (R)s->voxel_size, be = vs * (R)8.0f;
+    V3 gs = F(mk)((R)c->nb[0] * be, (R)c->nb[1] * be, (R)c->nb[2] * be);
+    V3 gp = F(sub)(p, F(mk)((R)s->origin[0], (R)s->origin[1], (R)s->origin[2]));
+    if (gp.x < (R)0 || gp.y < (R)0 || gp.z < (R)0 || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) {
+        R dx = FMAX(FMAX(-gp.x, gp.x - gs.x), (R)0), dy = FMAX(FMAX(-gp.y, gp.y - gs.y), (R)0),
+          dz = FMAX(FMAX(-gp.z, gp.z - gs.z), (R)0);
+        return SQRT(dx * dx + dy * dy + dz * dz);
+    }
+    R ivs = (R)1.0f / vs;
+    R x[3] = {gp.x * ivs, gp.y * ivs, gp.z * ivs};
+    uint32_t b[3];
+    for (int a = 0; a < 3; a++) b[a] = (uint32_t)F(clampi)((int32_t)FLOOR(x[a]), 0, c->tot[a] - 1) >> 3;
+    uint64_t cell = (uint64_t)(b[0] >> 3) + (uint64_t)(b[1] >> 3) * s->cells[0] +
+                    (uint64_t)(b[2] >> 3) * s->cells[0] * s->cells[1];
+    uint32_t e = 15u;
+    for (uint32_t o = 0; o < 8u; o++) {
+        uint32_t v = (s->oct[cell] >> (4u * o)) & 15u;
+        e = v < e ? v : e;
+    }
+    R unit = (R)64.0f;
+    uint32_t base[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
+    if (e == 0u) {
+        if (s->free_l2[cell]) return 0;
+        uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
+        uint32_t w = s->bext[cell * 512u + lb];
+        e = 15u;
+        for (uint32_t o = 0; o < 8u; o++) {
+            uint32_t v = (w >> (4u * o)) & 15u;
+            e = v < e ? v : e;
+        }
+        if (e == 0u) return 0;
+        unit = (R)8.0f;
+        for (int a = 0; a < 3; a++) base[a] = b[a];
+    }
+    R m = K_MAX_FLOAT;
+    for (int a = 0; a < 3; a++) {
+        R lo = ((R)base[a] - (R)(e - 1u)) * unit, hi = ((R)base[a] + (R)e) * unit;
+        m = FMIN(m, FMIN(x[a] - lo, hi - x[a]));
+    }
+    return FMAX(m, (R)0) * vs;
+}
+
+static void F(sp_cast)(const orc_sparse_scene *s, const rkfm_sphere_cast *q, rkfm_cast_result *out) {
+    F(sctx) c;
+    F(sp_init)(&c, s);
+    V3 o = F(mk)((R)q->origin[0], (R)q->origin[1], (R)q->origin[2]);
+    V3 dir = F(mk)((R)q->dir[0], (R)q->dir[1], (R)q->dir[2]);
+    R r = (R)q->radius, vs = (R)s->voxel_size, t = 0;
+    memset(out, 0, sizeof(*out));
+    out->t = q->max_t;
+    for (uint32_t i = 0; i < K_HARD_MAX; i++) {
+        if (t > (R)q->max_t) break;
+        out->steps++;
+        V3 p = F(add)(o, F(muls)(dir, t));
+        R L = F(sp_free_bound)(&c, p);
+        R D = L >= vs * (R)16.0f ? L : FMAX(F(sp_sample)(&c, p, 0, 1), L);
+        out->distance = (float)D;
+        if (D - r < (R)1e-3f) {
+            R e = vs * (R)2.0f;
+            R gx = F(sp_sample)(&c, F(add)(p, F(mk)(e, 0, 0)), 1, 0) - F(sp_sample)(&c, F(sub)(p, F(mk)(e, 0, 0)), 1, 0);
+            R gy = F(sp_sample)(&c, F(add)(p, F(mk)(0, e, 0)), 1, 0) - F(sp_sample)(&c, F(sub)(p, F(mk)(0, e, 0)), 1, 0);
+            R gz = F(sp_sample)(&c, F(add)(p, F(mk)(0, 0, e)), 1, 0) - F(sp_sample)(&c, F(sub)(p, F(mk)(0, 0, e)), 1, 0);
+            R len = F(length)(F(mk)(gx, gy, gz));
+            V3 n = len < (R)1e-10f ? F(mk)(0, 1, 0) : F(mk)(gx / len, gy / len, gz / len);
+            out->t = (float)t;
+            out->hit = 1;
+            out->normal[0] = (float)n.x;
+            out->normal[1] = (float)n.y;
+            out->normal[2] = (float)n.z;
+            return;
+        }
+        t += FMAX(D - r, K_MIN_STEP);
+    }
+}
+
 /* Point query: SdfQueryable::evaluate + gradient_central_diff
  * (sdf_collision.rs:38-56; eps 0.01, normalize_or_zero = v * (1/len) if that
  * reciprocal is finite and > 0). */

@@ -119,6 +119,8 @@ hipError_t rkf_launch_sparse_march_split(const rkf::SparseMarchParams *P, void *
                                          uint32_t shade_waves, hipStream_t stream);
 hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const float *xyz, uint32_t n, rkfm_point_result *out,
                                    hipStream_t stream);
+hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm_sphere_cast *casts, uint32_t n,
+                                  rkfm_cast_result *out, hipStream_t stream);
 // Per-tile primary-ray intervals: clear, bin every L1 block's cell, then the
 // cells whose rectangle exceeds kTileInlineMax tiles (big holds up to
 // n_blocks records, big_count one counter).

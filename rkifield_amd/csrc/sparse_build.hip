@@ -1412,6 +1412,14 @@ int rkfm_sparse_contacts(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkfm_bod
     return rkf_hip_check(rkf_launch_sparse_contacts(&D, bodies, n, contacts, counts, st), "contacts launch");
 }
 
+int rkfm_sparse_sphere_cast(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkfm_sphere_cast *casts, uint32_t n,
+                            rkfm_cast_result *out, void *stream) {
+    if (!c || !s || (n && (!casts || !out))) return rkf_set_error(RKFM_ERR_INVALID_ARG, "sphere_cast: null argument");
+    rkf::SparseDev D = s->dev();
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    return rkf_hip_check(rkf_launch_sparse_cast(&D, casts, n, out, st), "sphere cast launch");
+}
+
 int rkfm_sparse_query_points(rkfm_ctx *c, const rkfm_sparse_scene *s, const float *xyz, uint32_t n,
                              rkfm_point_result *out, void *stream) {
     if (!c || !s || (n && (!xyz || !out))) return rkf_set_error(RKFM_ERR_INVALID_ARG, "query_points: null argument");

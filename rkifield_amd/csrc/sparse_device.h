@@ -941,6 +941,96 @@ RKF_DEV void point_query(const SparseDev &S, f3 p, rkfm_point_result &out) {
     out = r;
 }
 
+// Sphere casts (declared with rkfm_sparse_sphere_cast): L(p), a lower bound
+// on the distance from p to any stored brick, from the cell word's octant
+// extents (free cells) or the brick's row word (free bricks). A free brick b
+// has every brick of {b..b+1}^3 EMPTY, so the free box [b - E + 1, b + E - 1]
+// (E = the minimum of the 8 octant extents) leaves [b - E + 1, b + E] EMPTY;
+// likewise for cells.
+RKF_DEV float sp_free_bound(const SparseDev &S, f3 p) {
+    const float vs = S.vs, be = vs * 8.0f;
+    const f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
+    const f3 gp = sub(p, mk(S.origin[0], S.origin[1], S.origin[2]));
+    if (gp.x < 0.0f || gp.y < 0.0f || gp.z < 0.0f || gp.x >= gs.x || gp.y >= gs.y || gp.z >= gs.z) {
+        const float dx = fmaxf(fmaxf(-gp.x, gp.x - gs.x), 0.0f), dy = fmaxf(fmaxf(-gp.y, gp.y - gs.y), 0.0f),
+                    dz = fmaxf(fmaxf(-gp.z, gp.z - gs.z), 0.0f);
+        return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+    const float x[3] = {gp.x * S.inv_vs, gp.y * S.inv_vs, gp.z * S.inv_vs};
+    uint32_t b[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) b[a] = (uint32_t)clampi((int32_t)floorf(x[a]), 0, S.tot[a] - 1) >> 3;
+    const uint64_t cell = (uint64_t)(b[0] >> 3) + (uint64_t)(b[1] >> 3) * S.cells[0] +
+                          (uint64_t)(b[2] >> 3) * S.cells[0] * S.cells[1];
+    const uint64_t cw = S.cellw[cell];
+    uint32_t e = 15u;
+#pragma unroll
+    for (uint32_t o = 0; o < 8u; o++) e = min(e, ((uint32_t)cw >> (4u * o)) & 15u);
+    float unit = 64.0f;
+    uint32_t base[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
+    if (e == 0u) {
+        const uint32_t row = (uint32_t)(cw >> 32);
+        if (row == kFreeCell) return 0.0f;
+        const uint64_t rw = S.roww[(uint64_t)row * 512u + (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u];
+        e = 15u;
+#pragma unroll
+        for (uint32_t o = 0; o < 8u; o++) e = min(e, (uint32_t)(rw >> (32u + 4u * o)) & 15u);
+        if (e == 0u) return 0.0f;
+        unit = 8.0f;
+        base[0] = b[0];
+        base[1] = b[1];
+        base[2] = b[2];
+    }
+    float m = kMaxFloat;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float lo = ((float)base[a] - (float)(e - 1u)) * unit, hi = ((float)base[a] + (float)e) * unit;
+        m = fminf(m, fminf(x[a] - lo, hi - x[a]));
+    }
+    return fmaxf(m, 0.0f) * vs;
+}
+
+RKF_DEV void sphere_cast(const SparseDev &S, const rkfm_sphere_cast &q, rkfm_cast_result &out) {
+    const f3 o = mk(q.origin[0], q.origin[1], q.origin[2]), dir = mk(q.dir[0], q.dir[1], q.dir[2]);
+    const float r = q.radius, vs = S.vs;
+    SpCtx c{&S, 0u};
+    rkfm_cast_result res;
+    res.t = q.max_t;
+    res.hit = 0u;
+    res.steps = 0u;
+    res.normal[0] = res.normal[1] = res.normal[2] = 0.0f;
+    res.distance = 0.0f;
+    res._pad = 0u;
+    float t = 0.0f;
+    for (uint32_t i = 0; i < kHardMax; i++) {
+        if (t > q.max_t) break;
+        res.steps++;
+        const f3 p = add(o, muls(dir, t));
+        const float L = sp_free_bound(S, p);
+        const float D = L >= vs * 16.0f ? L : fmaxf(sp_sample(c, p, false, true), L);
+        res.distance = D;
+        if (D - r < 1e-3f) {
+            const float e = vs * 2.0f;
+            const float gx = sp_sample(c, add(p, mk(e, 0.0f, 0.0f)), true, false) -
+                             sp_sample(c, sub(p, mk(e, 0.0f, 0.0f)), true, false);
+            const float gy = sp_sample(c, add(p, mk(0.0f, e, 0.0f)), true, false) -
+                             sp_sample(c, sub(p, mk(0.0f, e, 0.0f)), true, false);
+            const float gz = sp_sample(c, add(p, mk(0.0f, 0.0f, e)), true, false) -
+                             sp_sample(c, sub(p, mk(0.0f, 0.0f, e)), true, false);
+            const float len = length(mk(gx, gy, gz));
+            const f3 n = len < 1e-10f ? mk(0.0f, 1.0f, 0.0f) : mk(gx / len, gy / len, gz / len);
+            res.t = t;
+            res.hit = 1u;
+            res.normal[0] = n.x;
+            res.normal[1] = n.y;
+            res.normal[2] = n.z;
+            break;
+        }
+        t += fmaxf(D - r, kMinStep);
+    }
+    out = res;
+}
+
 // CollisionShape::sample_points (sdf_collision.rs:79-148): sample k of a
 // body's shape in body-local space; false when the shape has no sample k.
 RKF_DEV bool body_sample(const rkfm_body &b, uint32_t k, f3 &p) {

@@ -334,6 +334,13 @@ __global__ __launch_bounds__(256) void sparse_query_kernel(SparseDev S, const fl
     point_query(S, p, out[i]);
 }
 
+__global__ __launch_bounds__(256) void sparse_cast_kernel(SparseDev S, const rkfm_sphere_cast *casts, uint32_t n,
+                                                          rkfm_cast_result *out) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    sphere_cast(S, casts[i], out[i]);
+}
+
 // Contacts: 16 lanes per body (4 bodies per wavefront), one lane per sample
 // point; a lane's rank among the body's contacts (deeper first, then lower
 // sample index) comes from the 16 lanes' penetrations by shuffles.
@@ -440,6 +447,14 @@ extern "C" hipError_t rkf_launch_sparse_query(const rkf::SparseDev *S, const flo
                                               rkfm_point_result *out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(rkf::sparse_query_kernel, dim3((n + 255u) / 256u, 1, 1), dim3(256, 1, 1), 0, stream, *S, xyz,
+                       n, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm_sphere_cast *casts, uint32_t n,
+                                             rkfm_cast_result *out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rkf::sparse_cast_kernel, dim3((n + 255u) / 256u, 1, 1), dim3(256, 1, 1), 0, stream, *S, casts,
                        n, out);
     return hipGetLastError();
 }

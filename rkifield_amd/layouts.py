@@ -137,6 +137,10 @@ POINT_RESULT = np.dtype([
     ("inside", "<u4"),
     ("gradient", "<f4", (3,)),
 ])
+SPHERE_CAST = np.dtype([("origin", "<f4", (3,)), ("radius", "<f4"), ("dir", "<f4", (3,)), ("max_t", "<f4")])
+CAST_RESULT = np.dtype([("t", "<f4"), ("hit", "<u4"), ("steps", "<u4"), ("normal", "<f4", (3,)),
+                        ("distance", "<f4"), ("_pad", "<u4")])
+assert SPHERE_CAST.itemsize == 32 and CAST_RESULT.itemsize == 32
 
 GBUFFER_DESC = np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<u8"), ("motion", "<u8"),
                          ("debug", "<u8"), ("shadow", "<u8"), ("width", "<u4"), ("height", "<u4")])

@@ -78,6 +78,7 @@ _SIGS = {
     "rkfm_sparse_march": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_march_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _U32, _VP]),
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
+    "rkfm_sparse_sphere_cast": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_tile_intervals": (C.c_int, [_VP, _VP, _VP, _VP, _U32, _U32, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
     "rkfm_sparse_contacts": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
@@ -588,6 +589,10 @@ class SparseScene:
 
     def query(self, xyz_dev, out_dev, n, stream=None):
         _check(lib().rkfm_sparse_query_points(self.ctx.h, self.h, _p(xyz_dev), n, _p(out_dev), stream))
+
+    def sphere_cast(self, casts_dev, out_dev, n, stream=None):
+        """rkfm_sparse_sphere_cast: casts_dev SPHERE_CAST records -> out_dev CAST_RESULT records."""
+        _check(lib().rkfm_sparse_sphere_cast(self.ctx.h, self.h, _p(casts_dev), n, _p(out_dev), stream))
 
 
 class Batch:

@@ -278,3 +278,27 @@ def test_sparse_errors(ctx):
         M.SparseScene.from_dense(ctx, (8 * 32768 + 8, 1, 1), np.full(8 * 32768 + 8, 0xFFFFFFFF, np.uint32),
                                  np.zeros(512, L.VOXEL_SAMPLE), 0.05, (0.0, 0.0, 0.0))
     assert e.value.code == M.ERR["INVALID_ARG"]
+
+
+def test_sphere_casts_match_oracle(ctx, small_scene):
+    """rkfm_sparse_sphere_cast: every result field bit-exact against the FP32
+    twin; radii from below a voxel to above the 8 vs field cap, casts from
+    outside and inside the grid, hits and misses."""
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    rng = np.random.default_rng(8)
+    n = 6000
+    casts = np.zeros(n, L.SPHERE_CAST)
+    casts["origin"] = rng.uniform(-5.0, 5.0, (n, 3))
+    d = rng.normal(size=(n, 3))
+    casts["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    casts["radius"] = rng.uniform(0.01, 0.8, n)
+    casts["max_t"] = rng.uniform(1.0, 12.0, n)
+    _say(f"case: sphere casts {n}")
+    cd, rd = ctx.to_device(casts), ctx.array((n,), L.CAST_RESULT)
+    sc.sphere_cast(cd, rd, n)
+    got = rd.download()
+    want = O.sparse_cast(sp, casts)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    assert 500 < got["hit"].sum() < n - 500
+    assert (got["steps"] > 5).sum() > 500

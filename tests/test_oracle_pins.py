@@ -369,3 +369,52 @@ def test_point_query_on_the_ground_plane():
         assert np.array_equal(r["inside"] == 1, q[:, 1] < 0) or (np.abs(q[:, 1]) < 2e-4).any()
         assert np.allclose(r["gradient"], [0.0, 1.0, 0.0], atol=2e-3)
         assert np.all(r["material"] == 1)
+
+
+def _cast(o, d, r, max_t=5.0):
+    c = np.zeros((), L.SPHERE_CAST)
+    c["origin"], c["radius"], c["dir"], c["max_t"] = o, r, np.asarray(d) / np.linalg.norm(d), max_t
+    return c
+
+
+def test_sphere_cast_closed_forms_on_the_ground_plane():
+    """Sphere casts (no reference counterpart) over the voxelised plane y = 0:
+    a ball of radius r dropped from height h stops with its centre at y = r
+    (t = (h - r) / |dir.y| for an oblique cast), normal (0, 1, 0); a cast
+    parallel to the plane above r misses; the contact is never past the
+    exact one (the distance used is a lower bound)."""
+    sp = _ground_plane()
+    casts, want = [], []
+    for r in (0.05, 0.12, 0.3):
+        for h in (0.5, 0.9):
+            for d in ((0.0, -1.0, 0.0), (0.5, -1.0, 0.2), (-0.3, -1.0, -0.4)):
+                dn = np.asarray(d) / np.linalg.norm(d)
+                casts.append(_cast((0.1, h, -0.2), d, r))
+                want.append((h - r) / -dn[1])
+    casts = np.array(casts, L.SPHERE_CAST)
+    for prec in (32, 64):
+        res = O.sparse_cast(sp, casts, prec=prec)
+        assert np.all(res["hit"] == 1)
+        t = res["t"].astype(np.float64)
+        assert np.all(t <= np.asarray(want) + 1e-4), (t, want)
+        assert np.allclose(t, want, atol=3e-3), np.abs(t - want).max()
+        assert np.allclose(res["normal"], [0.0, 1.0, 0.0], atol=2e-3)
+    # parallel to the plane at 0.2 above a 0.1 ball: no contact within max_t
+    par = np.array([_cast((-1.0, 0.2, 0.0), (1.0, 0.0, 0.0), 0.1, max_t=1.8)], L.SPHERE_CAST)
+    r = O.sparse_cast(sp, par)
+    assert r["hit"][0] == 0 and r["t"][0] == np.float32(1.8)
+
+
+def test_sphere_cast_large_radius_is_conservative():
+    """A 0.6 m ball (above the 8 vs = 0.4 m cap of the stored field) cast at
+    the plane: the contact is reported no later than the exact one and at
+    most a narrow band (1.8 bricks) early: the distance bound stops at the
+    stored bricks, not at the surface inside them."""
+    sp = _ground_plane()
+    res = O.sparse_cast(sp, np.array([_cast((0.0, 1.1, 0.0), (0.0, -1.0, 0.0), 0.6)], L.SPHERE_CAST), prec=64)
+    assert res["hit"][0] == 1
+    exact = 1.1 - 0.6
+    assert exact - 1.8 * 0.4 - 1e-3 <= res["t"][0] <= exact + 1e-4
+    # from higher up the free space above the band is crossed with large steps
+    res = O.sparse_cast(sp, np.array([_cast((0.0, 1.1, 0.0), (0.0, -1.0, 0.0), 0.2)], L.SPHERE_CAST), prec=64)
+    assert res["hit"][0] == 1 and abs(res["t"][0] - 0.9) < 3e-3
