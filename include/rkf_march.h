@@ -677,6 +677,7 @@ int rkfm_sparse_from_rkf(rkfm_ctx *ctx, const void *file, uint64_t len,
 #define RKFM_BATCH_VIEW 0u   /* camera view: G-buffer (+ shadow plane)        */
 #define RKFM_BATCH_SHADOW 1u /* soft-shadow rays from given origins          */
 #define RKFM_BATCH_POINTS 2u /* point queries                                */
+#define RKFM_BATCH_SPHERES 3u /* sphere casts (rkfm_sparse_sphere_cast)      */
 
 typedef struct rkfm_batch_item {
     uint32_t kind;                    /* RKFM_BATCH_*                         */
@@ -697,6 +698,9 @@ typedef struct rkfm_batch_item {
     /* points: xyz (3 floats each, device) -> results (device) */
     const float *xyz;
     rkfm_point_result *results;
+    /* sphere casts: casts (device) -> cast results (device) */
+    const rkfm_sphere_cast *casts;
+    rkfm_cast_result *cast_results;
 } rkfm_batch_item;
 
 #define RKFM_SCHEDULE_GRID 0u       /* one wavefront per work unit           */
@@ -791,9 +795,10 @@ typedef struct rkfm_request {
     rkf_scene_uniforms su;
     rkfm_scene_ext ext;        /* views: shadow plane written iff            */
                                /* ext.shadow_enable; shadow batches: light   */
-    const float *in;           /* host: shadow origins / points (3 floats)   */
+    const float *in;           /* host: shadow origins / points (3 floats) / */
+                               /* rkfm_sphere_cast records (8 floats)        */
     void *out;                 /* host: float visibility[n] /                */
-                               /* rkfm_point_result[n]                       */
+                               /* rkfm_point_result[n] / rkfm_cast_result[n] */
     rkfm_gbuffer view_out;     /* views: host planes (NULL plane = skipped;  */
                                /* width/height ignored)                     */
     uint64_t tag;              /* caller's value, returned in the status     */
@@ -898,7 +903,7 @@ static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes")
 static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
 static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
 static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
-static_assert(sizeof(rkfm_batch_item) == 312, "batch item record");
+static_assert(sizeof(rkfm_batch_item) == 328, "batch item record");
 static_assert(sizeof(rkfm_request) == 312, "request record");
 static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 static_assert(sizeof(rkfm_request_status) == 64, "request status record");
@@ -917,7 +922,7 @@ _Static_assert(sizeof(rkf_camera_uniforms) == 144, "CameraUniforms is 144 bytes"
 _Static_assert(sizeof(rkf_scene_uniforms) == 16, "SceneUniforms is 16 bytes");
 _Static_assert(sizeof(rkf_coarse_field_uniforms) == 48, "CoarseFieldUniforms is 48 bytes");
 _Static_assert(sizeof(rkf_bvh_node) == 32, "BvhNode is 32 bytes");
-_Static_assert(sizeof(rkfm_batch_item) == 312, "batch item record");
+_Static_assert(sizeof(rkfm_batch_item) == 328, "batch item record");
 _Static_assert(sizeof(rkfm_request) == 312, "request record");
 _Static_assert(sizeof(rkfm_server_config) == 56, "server config record");
 _Static_assert(sizeof(rkfm_request_status) == 64, "request status record");

@@ -1,12 +1,14 @@
 // batch.hip — one launch over many uneven batches (views, shadow-ray batches,
-// point-query batches), possibly over different resident sparse scenes.
+// point-query batches, sphere-cast batches), possibly over different resident
+// sparse scenes.
 //
 // Work units: a view is cut into tile_w x tile_h pixel packets (64 lanes), a
 // shadow or point batch into packets of 64. Unit u belongs to the batch whose
 // [unit_begin, unit_begin + n_units) holds it (binary search over <= a few
 // hundred wave-uniform begins, through the scalar cache). Each unit runs the
 // same per-ray code as the single-batch entry points (view_pixel,
-// shadow_step, point_query), so every result equals the batch run alone.
+// shadow_step, point_query, sphere_cast), so every result equals the batch
+// run alone.
 //
 // Schedules: GRID launches one wavefront per unit and lets the hardware
 // dispatcher balance; PERSISTENT launches waves_per_cu x CUs wavefronts that
@@ -53,11 +55,15 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
         while (!shadow_step<kLowLat>(B.P.S, sr, maxd, k)) {
         }
         B.vis[i] = sr.prev_dist;
-    } else {
+    } else if (B.kind == RKFM_BATCH_POINTS) {
         uint32_t i = u * 64u + lane;
         if (i >= B.n) return;
         f3 p = mk(B.in[3 * (uint64_t)i], B.in[3 * (uint64_t)i + 1], B.in[3 * (uint64_t)i + 2]);
         point_query(B.P.S, p, B.res[i]);
+    } else {
+        uint32_t i = u * 64u + lane;
+        if (i >= B.n) return;
+        sphere_cast(B.P.S, B.casts[i], B.cres[i]);
     }
 }
 
@@ -187,6 +193,12 @@ int rkf_batch_prepare(rkfm_ctx *c, const rkfm_batch_item *items, uint32_t n, con
                 return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: null point buffers", i);
             d.in = it.xyz;
             d.res = it.results;
+            units = (it.n + 63u) / 64u;
+        } else if (it.kind == RKFM_BATCH_SPHERES) {
+            if (it.n && (!it.casts || !it.cast_results))
+                return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: null sphere-cast buffers", i);
+            d.casts = it.casts;
+            d.cres = it.cast_results;
             units = (it.n + 63u) / 64u;
         } else {
             return rkf_set_error(RKFM_ERR_INVALID_ARG, "batch item %u: kind %u", i, it.kind);

@@ -99,6 +99,8 @@ struct BatchDev {
     const float *in;
     float *vis;
     rkfm_point_result *res;
+    const rkfm_sphere_cast *casts;
+    rkfm_cast_result *cres;
 };
 
 }  // namespace rkf

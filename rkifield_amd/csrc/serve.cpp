@@ -164,6 +164,13 @@ struct rkfm_server {
 
 namespace {
 
+// Bytes per input / output record of a query kind.
+uint64_t in_record_bytes(uint32_t kind) { return kind == RKFM_BATCH_SPHERES ? sizeof(rkfm_sphere_cast) : 12u; }
+uint64_t out_record_bytes(uint32_t kind) {
+    return kind == RKFM_BATCH_SHADOW ? 4u : kind == RKFM_BATCH_POINTS ? sizeof(rkfm_point_result)
+                                                                      : sizeof(rkfm_cast_result);
+}
+
 // Footprint layout of one request inside its arena region.
 int plan_request(const rkfm_server *s, const rkfm_request &r, Req &q) {
     if (r.scene >= s->scenes.size())
@@ -188,12 +195,12 @@ int plan_request(const rkfm_server *s, const rkfm_request &r, Req &q) {
             off += align_up(px * kPlaneBytes[p]);
         }
         q.work = px;
-    } else if (r.kind == RKFM_BATCH_SHADOW || r.kind == RKFM_BATCH_POINTS) {
+    } else if (r.kind == RKFM_BATCH_SHADOW || r.kind == RKFM_BATCH_POINTS || r.kind == RKFM_BATCH_SPHERES) {
         if (r.n && (!r.in || !r.out)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "server: null host buffer");
         q.in_off = 0;
-        off = align_up((uint64_t)r.n * 12u);
+        off = align_up((uint64_t)r.n * in_record_bytes(r.kind));
         q.out_off = off;
-        off += align_up((uint64_t)r.n * (r.kind == RKFM_BATCH_SHADOW ? 4u : sizeof(rkfm_point_result)));
+        off += align_up((uint64_t)r.n * out_record_bytes(r.kind));
         q.work = r.n;
     } else {
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "server: request kind %u", r.kind);
@@ -276,8 +283,8 @@ bool rkfm_server::reap_locked() {
 int rkfm_server::copies_in(Slot &sl) {
     for (Req *q : sl.reqs) {
         if (q->r.kind == RKFM_BATCH_VIEW || q->r.n == 0) continue;
-        hipError_t e = hipMemcpyAsync(arena_base + q->off + q->in_off, q->r.in, (uint64_t)q->r.n * 12u,
-                                      hipMemcpyHostToDevice, sl.stream);
+        hipError_t e = hipMemcpyAsync(arena_base + q->off + q->in_off, q->r.in,
+                                      (uint64_t)q->r.n * in_record_bytes(q->r.kind), hipMemcpyHostToDevice, sl.stream);
         if (e != hipSuccess) return rkf_hip_check(e, "server input copy");
     }
     return RKFM_OK;
@@ -299,7 +306,7 @@ int rkfm_server::copies_out(Slot &sl) {
                 if (e != hipSuccess) return rkf_hip_check(e, "server view copy");
             }
         } else if (r.n) {
-            uint64_t bytes = (uint64_t)r.n * (r.kind == RKFM_BATCH_SHADOW ? 4u : sizeof(rkfm_point_result));
+            uint64_t bytes = (uint64_t)r.n * out_record_bytes(r.kind);
             hipError_t e = hipMemcpyAsync(r.out, base + q->out_off, bytes, hipMemcpyDeviceToHost, sl.stream);
             if (e != hipSuccess) return rkf_hip_check(e, "server output copy");
         }
@@ -331,9 +338,12 @@ rkfm_batch_item item_of(const rkfm_server &s, const Req &q) {
     } else if (r.kind == RKFM_BATCH_SHADOW) {
         it.origins = (const float *)(base + q.in_off);
         it.visibility = (float *)(base + q.out_off);
-    } else {
+    } else if (r.kind == RKFM_BATCH_POINTS) {
         it.xyz = (const float *)(base + q.in_off);
         it.results = (rkfm_point_result *)(base + q.out_off);
+    } else {
+        it.casts = (const rkfm_sphere_cast *)(base + q.in_off);
+        it.cast_results = (rkfm_cast_result *)(base + q.out_off);
     }
     return it;
 }
@@ -386,6 +396,11 @@ int rkfm_server::launch_plain(Slot &sl) {
     if (q.r.kind == RKFM_BATCH_POINTS) {
         rkf::SparseDev D = it.scene->dev();
         return rkf_hip_check(rkf_launch_sparse_query(&D, it.xyz, it.n, it.results, sl.stream), "server plain query");
+    }
+    if (q.r.kind == RKFM_BATCH_SPHERES) {
+        rkf::SparseDev D = it.scene->dev();
+        return rkf_hip_check(rkf_launch_sparse_cast(&D, it.casts, it.n, it.cast_results, sl.stream),
+                             "server plain sphere cast");
     }
     return launch_batched(sl);
 }

@@ -145,14 +145,16 @@ assert SPHERE_CAST.itemsize == 32 and CAST_RESULT.itemsize == 32
 GBUFFER_DESC = np.dtype([("position", "<u8"), ("normal", "<u8"), ("material", "<u8"), ("motion", "<u8"),
                          ("debug", "<u8"), ("shadow", "<u8"), ("width", "<u4"), ("height", "<u4")])
 
-BATCH_VIEW, BATCH_SHADOW, BATCH_POINTS = 0, 1, 2
+BATCH_VIEW, BATCH_SHADOW, BATCH_POINTS, BATCH_SPHERES = 0, 1, 2, 3
 SCHEDULE_GRID, SCHEDULE_PERSISTENT, SCHEDULE_GRID_LOWLAT = 0, 1, 2
 
 BATCH_ITEM = np.dtype([
     ("kind", "<u4"), ("n", "<u4"), ("scene", "<u8"),
     ("camera", CAMERA_UNIFORMS), ("su", SCENE_UNIFORMS), ("ext", SCENE_EXT), ("out", GBUFFER_DESC),
     ("origins", "<u8"), ("visibility", "<u8"), ("xyz", "<u8"), ("results", "<u8"),
+    ("casts", "<u8"), ("cast_results", "<u8"),
 ])
+assert BATCH_ITEM.itemsize == 328
 
 BATCH_TUNING = np.dtype([("schedule", "<u4"), ("tile_w", "<u4"), ("tile_h", "<u4"), ("waves_per_cu", "<u4")])
 
@@ -219,7 +221,7 @@ assert DEBUG_PX.itemsize == 48
 assert PRIM.itemsize == 80
 assert POINT_RESULT.itemsize == 24
 assert GBUFFER_DESC.itemsize == 56
-assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 32
+assert BATCH_ITEM.itemsize == 8 + 8 + 144 + 16 + 48 + 56 + 48
 assert EDIT.itemsize == 64
 assert RKF_HEADER.itemsize == 128 and RKF_LOD.itemsize == 40 and RKF_LOD_SRC.itemsize == 40
 assert BODY.itemsize == 48

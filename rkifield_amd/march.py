@@ -640,6 +640,17 @@ class Batch:
         self.keep += [xyz_dev, out_dev]
         return len(self.items) - 1
 
+    def add_spheres(self, scene, casts_dev, out_dev, n):
+        it = np.zeros((), L.BATCH_ITEM)
+        it["kind"] = L.BATCH_SPHERES
+        it["n"] = n
+        it["scene"] = scene.h
+        it["casts"] = casts_dev.ptr
+        it["cast_results"] = out_dev.ptr
+        self.items.append(it)
+        self.keep += [casts_dev, out_dev]
+        return len(self.items) - 1
+
     def array(self):
         return np.ascontiguousarray(np.array(self.items, L.BATCH_ITEM))
 
@@ -770,6 +781,17 @@ class Server:
         r["scene"] = scene
         r["n"] = xyz.shape[0] if n is None else n
         r["in"], r["out"] = xyz.ptr, out.ptr
+        r["tag"] = tag
+        return r
+
+    @staticmethod
+    def cast_request(scene, casts, out, n=None, tag=0):
+        """casts: PinnedArray of SPHERE_CAST records, out: PinnedArray of CAST_RESULT."""
+        r = np.zeros((), L.REQUEST)
+        r["kind"] = L.BATCH_SPHERES
+        r["scene"] = scene
+        r["n"] = casts.shape[0] if n is None else n
+        r["in"], r["out"] = casts.ptr, out.ptr
         r["tag"] = tag
         return r
 

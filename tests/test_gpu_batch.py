@@ -68,7 +68,22 @@ def _make(ctx, two_scenes):
         xd, rd = ctx.to_device(x), ctx.array((n,), L.POINT_RESULT)
         b.add_points(two_scenes[sc], xd, rd, n)
         points.append((two_scenes[sc], x, xd, rd))
+    for sc, n in ((0, 2000), (1, 33)):
+        casts = _casts(rng, n)
+        cd, rd = ctx.to_device(casts), ctx.array((n,), L.CAST_RESULT)
+        b.add_spheres(two_scenes[sc], cd, rd, n)
+        points.append((two_scenes[sc], casts, cd, rd))
     return b, views, shadows, points
+
+
+def _casts(rng, n):
+    c = np.zeros(n, L.SPHERE_CAST)
+    c["origin"] = rng.uniform(-5, 5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    c["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    c["radius"] = rng.uniform(0.02, 0.6, n)
+    c["max_t"] = 10.0
+    return c
 
 
 def _alone(ctx, views, points):
@@ -79,8 +94,12 @@ def _alone(ctx, views, points):
         ref_views.append(gb.read())
     ref_points = []
     for sc, x, xd, _ in points:
-        out = ctx.array((x.shape[0],), L.POINT_RESULT)
-        sc.query(xd, out, x.shape[0])
+        if x.dtype == L.SPHERE_CAST:
+            out = ctx.array((x.shape[0],), L.CAST_RESULT)
+            sc.sphere_cast(xd, out, x.shape[0])
+        else:
+            out = ctx.array((x.shape[0],), L.POINT_RESULT)
+            sc.query(xd, out, x.shape[0])
         ref_points.append(out.download())
     return ref_views, ref_points
 

@@ -61,6 +61,14 @@ class Mix:
             x = M.PinnedArray((n, 3), np.float32)
             x.a[:] = rng.uniform(-5, 5, (n, 3))
             self.items.append(("points", s, dict(n=n), dict(xyz=x, out=M.PinnedArray((n,), L.POINT_RESULT))))
+        for s, n in ((1, 1500), (2, 40)):
+            c = M.PinnedArray((n,), L.SPHERE_CAST)
+            c.a["origin"] = rng.uniform(-5, 5, (n, 3))
+            d = rng.normal(size=(n, 3))
+            c.a["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+            c.a["radius"] = rng.uniform(0.02, 0.6, n)
+            c.a["max_t"] = 10.0
+            self.items.append(("casts", s, dict(n=n), dict(casts=c, out=M.PinnedArray((n,), L.CAST_RESULT))))
         for s, n in ((0, 1000), (2, 77)):
             o = M.PinnedArray((n, 3), np.float32)
             o.a[:] = rng.uniform(-4, 4, (n, 3))
@@ -80,6 +88,11 @@ class Mix:
             out = ctx.array((p["n"],), L.POINT_RESULT)
             sc[s].query(xd, out, p["n"])
             return out.download()
+        if kind == "casts":
+            cd = ctx.to_device(host["casts"].a)
+            out = ctx.array((p["n"],), L.CAST_RESULT)
+            sc[s].sphere_cast(cd, out, p["n"])
+            return out.download()
         od = ctx.to_device(host["origins"].a)
         vd = ctx.array((p["n"],), np.float32)
         b = M.Batch(ctx)
@@ -94,13 +107,15 @@ class Mix:
                                          host=host, tag=k)
         if kind == "points":
             return M.Server.points_request(s, host["xyz"], host["out"], tag=k)
+        if kind == "casts":
+            return M.Server.cast_request(s, host["casts"], host["out"], tag=k)
         return M.Server.shadow_request(s, host["origins"], host["vis"], p["ext"], tag=k)
 
     def clear(self):
         """Poisons every output buffer (inputs are left alone)."""
         for _, _, _, host in self.items:
             for name, arr in (host or {}).items():
-                if name not in ("xyz", "origins"):
+                if name not in ("xyz", "origins", "casts"):
                     arr.a.view(np.uint8)[...] = 0xAB
 
     def check(self, srv, k, rid):
@@ -111,7 +126,7 @@ class Mix:
             names = PLANES + (("shadow",) if p["ext"]["shadow_enable"] else ())
             for name in names:
                 assert np.array_equal(got[name].view(np.uint8), ref[name].view(np.uint8)), (k, name)
-        elif kind == "points":
+        elif kind in ("points", "casts"):
             assert np.array_equal(host["out"].a.view(np.uint8), ref.view(np.uint8)), k
         else:
             assert np.array_equal(host["vis"].a.view(np.uint32), ref.view(np.uint32)), k
