@@ -78,12 +78,35 @@ RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, const rkf_gpu_ob
     r.obj_idx = oi;
 }
 
+constexpr uint32_t kNoObj = 0xFFFFFFFFu;
+
+// The L lanes of one ray each hold the first minimum of their share of the
+// step's objects (lane s: objects s, s + L, ...), or (kMaxFloat, kNoObj).
+// Combined, every lane holds the first object attaining the minimum over
+// all of them, the object ray_march.wgsl:929-938 keeps (strictly smaller
+// value, or the same value at a lower index).
+template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j) {
+#pragma unroll
+    for (uint32_t off = 1; off < L; off <<= 1) {
+        const float pd = __shfl_xor(d, (int)off);
+        const uint32_t pm = (uint32_t)__shfl_xor((int)m, (int)off);
+        const uint32_t pj = (uint32_t)__shfl_xor((int)j, (int)off);
+        if (pd < d || (pd == d && pj < j)) {
+            d = pd;
+            m = pm;
+            j = pj;
+        }
+    }
+}
+
 // ray_march.wgsl:641-985. Tiled mode: the tile's `count` object records and
-// indices, staged in LDS by the workgroup (an 8x8 pixel workgroup lies in one
-// 16x16 object tile), so a step's object loop reads no global memory.
-template <uint32_t MODE>
-RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
-                   MarchRes &r) {
+// indices, staged in LDS by the workgroup (the workgroup's pixels lie in one
+// 16x16 object tile), so a step's object loop reads no global memory; the
+// loop is shared by the ray's L lanes (lane `share` takes every L-th object).
+// bis: evaluations of the bisection (made by every lane of the ray).
+template <uint32_t MODE, uint32_t L>
+RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
+                        uint32_t share, MarchRes &r, uint32_t &bis) {
     const RefParams &P = *c.P;
     r.t = kMaxFloat;
     r.hit = false;
@@ -113,17 +136,25 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
         float min_dist = kMaxFloat;
         uint32_t best_mat = 0, best_id = 0, best_idx = 0, best_j = 0;
         if (MODE == 0) {
-            for (uint32_t i = 0; i < count; i++) {
-                const uint32_t oi = toi[i];
+            float bd = kMaxFloat;
+            uint32_t bm = 0, bj = kNoObj;
+            for (uint32_t i = share; i < count; i += L) {
                 uint32_t m;
-                float d = eval_obj(c, pos, tobj[i], oi, m);
-                if (d < min_dist) {
-                    min_dist = d;
-                    best_mat = m;
-                    best_id = oi < P.su.num_objects ? tobj[i].object_id : 0u;
-                    best_idx = oi;
-                    best_j = i;
+                const float d = eval_obj(c, pos, tobj[i], toi[i], m);
+                if (d < bd) {
+                    bd = d;
+                    bm = m;
+                    bj = i;
                 }
+            }
+            split_min<L>(bd, bm, bj);
+            if (bj != kNoObj) {
+                const uint32_t oi = toi[bj];
+                min_dist = bd;
+                best_mat = bm;
+                best_id = oi < P.su.num_objects ? tobj[bj].object_id : 0u;
+                best_idx = oi;
+                best_j = bj;
             }
         } else if (MODE == 1) {
             uint32_t stack[32];
@@ -169,7 +200,9 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
         if (min_dist < 0.0f && prev_dist > hit_eps(P.eps_mode, P.su.hit_threshold, prev_t)) {
             const rkf_gpu_object &bo =
                 MODE == 0 ? tobj[best_j] : P.objects[best_idx < P.su.num_objects ? best_idx : 0u];
+            const uint32_t e0 = c.evals;
             bisect(c, o, sd, prev_t, t, bo, best_idx, best_mat, best_id, r);
+            bis = c.evals - e0;
             return;
         }
         if (min_dist < hit_eps(P.eps_mode, P.su.hit_threshold, t)) {
@@ -187,50 +220,87 @@ RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const ui
     }
 }
 
+// march_core for one ray; c.evals then counts the ray's evaluations once
+// (the L lanes' shares of the object loops plus one bisection).
+template <uint32_t MODE, uint32_t L>
+RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
+                   uint32_t share, MarchRes &r) {
+    uint32_t bis = 0;
+    march_core<MODE, L>(c, o, dir, tobj, toi, count, share, r, bis);
+    if (L > 1) {
+        uint32_t e = c.evals - bis;
+#pragma unroll
+        for (uint32_t off = 1; off < L; off <<= 1) e += (uint32_t)__shfl_xor((int)e, (int)off);
+        c.evals = e + bis;
+    }
+}
+
+// Central-difference tap k of the normal (0 +x, 1 -x, 2 +y, 3 -y, 4 +z,
+// 5 -z), the same additions ray_march.wgsl:1008-1013 / :1024-1029 make.
+RKF_DEV f3 tap_point(f3 p, float e, uint32_t k) {
+    const f3 v = k < 2u ? mk(e, 0.0f, 0.0f) : k < 4u ? mk(0.0f, e, 0.0f) : mk(0.0f, 0.0f, e);
+    return (k & 1u) ? sub(p, v) : add(p, v);
+}
+
+// The six taps shared by the ray's L lanes (lane `share` takes taps share,
+// share + L, ...); every lane receives all six. base: the ray's first lane.
+template <uint32_t L, class F> RKF_DEV void shared_taps(uint32_t share, uint32_t base, F &&tap, float tv[6]) {
+    constexpr uint32_t Q = (6u + L - 1u) / L;
+    float mine[Q];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+        const uint32_t k = share + q * L;
+        mine[q] = k < 6u ? tap(k) : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 6u; k++) tv[k] = L == 1u ? mine[k] : __shfl(mine[k / L], (int)(base + k % L));
+}
+
 // ray_march.wgsl:1003-1032
-RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t oi) {
+template <uint32_t L> RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t oi, uint32_t share, uint32_t base) {
     const RefParams &P = *c.P;
     const rkf_gpu_object &o = P.objects[oi];
+    float tv[6];
     if (o.sdf_type == RKF_SDF_TYPE_VOXELIZED) {
         f3 lp = xform_point(o.inverse_world, pos);
         float e = o.voxel_size * 2.0f;
-        float gx = sample_voxelized(c, add(lp, mk(e, 0.0f, 0.0f)), o, true) -
-                   sample_voxelized(c, sub(lp, mk(e, 0.0f, 0.0f)), o, true);
-        float gy = sample_voxelized(c, add(lp, mk(0.0f, e, 0.0f)), o, true) -
-                   sample_voxelized(c, sub(lp, mk(0.0f, e, 0.0f)), o, true);
-        float gz = sample_voxelized(c, add(lp, mk(0.0f, 0.0f, e)), o, true) -
-                   sample_voxelized(c, sub(lp, mk(0.0f, 0.0f, e)), o, true);
-        f3 wg = xform_grad(o.inverse_world, mk(gx, gy, gz));
+        shared_taps<L>(share, base, [&](uint32_t k) { return sample_voxelized(c, tap_point(lp, e, k), o, true); }, tv);
+        f3 wg = xform_grad(o.inverse_world, mk(tv[0] - tv[1], tv[2] - tv[3], tv[4] - tv[5]));
         float len = length(wg);
         if (len < 1e-10f) return mk(0.0f, 1.0f, 0.0f);
         return mk(wg.x / len, wg.y / len, wg.z / len);
     }
     float e = P.su.hit_threshold * 10.0f;
-    uint32_t m;
-    float nx = eval_object(c, add(pos, mk(e, 0.0f, 0.0f)), oi, m) - eval_object(c, sub(pos, mk(e, 0.0f, 0.0f)), oi, m);
-    float ny = eval_object(c, add(pos, mk(0.0f, e, 0.0f)), oi, m) - eval_object(c, sub(pos, mk(0.0f, e, 0.0f)), oi, m);
-    float nz = eval_object(c, add(pos, mk(0.0f, 0.0f, e)), oi, m) - eval_object(c, sub(pos, mk(0.0f, 0.0f, e)), oi, m);
-    return normalize(mk(nx, ny, nz));
+    shared_taps<L>(share, base, [&](uint32_t k) {
+        uint32_t m;
+        return eval_object(c, tap_point(pos, e, k), oi, m);
+    }, tv);
+    return normalize(mk(tv[0] - tv[1], tv[2] - tv[3], tv[4] - tv[5]));
 }
 
 // ray_march.wgsl:1036-1111, one instance per strategy (MODE 0 tiled, 1 BVH,
 // 2 brute; :1056-1063 picks it, the host launches that instance). Block = 64
-// lanes = 8x8 pixels. Held to 128 VGPRs (4 waves/SIMD): a 640x360 view is
-// 3600 waves, which then all fit the 256 CUs at once.
-template <uint32_t MODE>
+// lanes = 64 / L rays of L lanes each: 8x8 pixels (L 1), 8x4 (2), 4x4 (4),
+// 4x2 (8), 2x2 (16). Held to 128 VGPRs (4 waves/SIMD).
+template <uint32_t L> struct RefBlock {
+    static constexpr uint32_t w = L <= 2u ? 8u : L <= 8u ? 4u : 2u, h = (64u / L) / w;
+};
+template <uint32_t MODE, uint32_t L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
+    static_assert(MODE == 0 || L == 1, "only the tiled object loop is split across lanes");
+    constexpr uint32_t BW = RefBlock<L>::w, BH = RefBlock<L>::h;
     __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
     __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
-    const uint32_t lane = threadIdx.x;
-    uint32_t px = blockIdx.x * 8u + (lane & 7u);
-    uint32_t py = blockIdx.y * 8u + (lane >> 3);
+    const uint32_t lane = threadIdx.x, ray = lane / L, share = lane % L;
+    uint32_t px = blockIdx.x * BW + ray % BW;
+    uint32_t py = blockIdx.y * BH + ray / BW;
     const uint32_t W = P.g.width, H = P.g.height;
     uint32_t count = 0;
     if (MODE == 0) {
         // Stage the tile's object list and records (64 lanes, one dword of a
         // 256-byte record each; records past num_objects are never read).
         const uint32_t ntx = (W + 15u) / 16u;
-        const uint32_t tile = ((blockIdx.y * 8u) / 16u) * ntx + (blockIdx.x * 8u) / 16u;
+        const uint32_t tile = ((blockIdx.y * BH) / 16u) * ntx + (blockIdx.x * BW) / 16u;
         count = tile < P.n_tiles ? P.tile_counts[tile] : 0u;
         if (count > RKF_TILE_MAX_OBJECTS) count = RKF_TILE_MAX_OBJECTS;
         if (lane < count) s_oi[lane] = P.tile_idx[(uint64_t)tile * RKF_TILE_MAX_OBJECTS + lane];
@@ -257,15 +327,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     f3 o, dir;
     camera_ray(P.cam, W, H, px, py, o, dir);
     MarchRes r;
-    march<MODE>(c, o, dir, s_obj, s_oi, count, r);
+    march<MODE, L>(c, o, dir, s_obj, s_oi, count, share, r);
+    // the ray's L lanes hold the same results from here on; the first stores
     if (!r.hit) {
-        store_miss(P.g, p);
-        store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
+        if (share == 0) {
+            store_miss(P.g, p);
+            store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, r.steps, c.evals);
+        }
         return;
     }
     f3 hp = add(o, muls(dir, r.t));
     uint32_t evals_march = c.evals;
-    f3 n = normal_for_object(c, hp, r.obj_idx);
+    f3 n = normal_for_object<L>(c, hp, r.obj_idx, share, lane - share);
     const rkf_gpu_object &ho = P.objects[r.obj_idx];
     uint32_t sec = 0, blend = 0, slot = RKF_EMPTY_SLOT, vox = 0xFFFFFFFFu;
     float sdf_at = 0.0f;
@@ -281,19 +354,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     uint32_t packed = (r.mat & 0x3Fu) | (sec << 6) | (blend << 12) | (r.obj_id << 24);
     float mx, my;
     motion_vec(P.cam, W, H, px, py, hp, mx, my);
+    if (share != 0) return;
     store_hit(P.g, p, hp, r.t, n, packed, mx, my, sdf_at);
     store_debug(P.g, p, n, slot, vox, r.obj_idx, r.steps, evals_march);
 }
 
 }  // namespace rkf
 
+// Lanes per ray of the tiled instance.
+constexpr uint32_t kTiledLanes = 8;
+
 extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream) {
     dim3 grid((P->g.width + 7u) / 8u, (P->g.height + 7u) / 8u, 1);
-    if (P->mode == 0)
-        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<0>, grid, dim3(64, 1, 1), 0, stream, *P);
-    else if (P->mode == 1)
-        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<1>, grid, dim3(64, 1, 1), 0, stream, *P);
-    else
-        hipLaunchKernelGGL(rkf::ray_march_ref_kernel<2>, grid, dim3(64, 1, 1), 0, stream, *P);
+    if (P->mode == 0) {
+        using B = rkf::RefBlock<kTiledLanes>;
+        dim3 tg((P->g.width + B::w - 1u) / B::w, (P->g.height + B::h - 1u) / B::h, 1);
+        hipLaunchKernelGGL((rkf::ray_march_ref_kernel<0, kTiledLanes>), tg, dim3(64, 1, 1), 0, stream, *P);
+    } else if (P->mode == 1) {
+        hipLaunchKernelGGL((rkf::ray_march_ref_kernel<1, 1>), grid, dim3(64, 1, 1), 0, stream, *P);
+    } else {
+        hipLaunchKernelGGL((rkf::ray_march_ref_kernel<2, 1>), grid, dim3(64, 1, 1), 0, stream, *P);
+    }
     return hipGetLastError();
 }
