@@ -350,8 +350,9 @@ def run_config3(args, ctx, M, L, scenes):
     client against the batched server and against the per-request policy
     (one plain launch per request), first at its arrival times (latency), then
     all at once (saturation throughput). Both use the server's default shape
-    (4 rounds in flight, rounds capped at 2^20 rays + queries, a round of one
-    request on its plain kernel). Point results are copied to pinned host
+    (4 rounds in flight, rounds of 2^20 rays + queries or a quarter of the
+    admitted queue's work if that is more, a round of one request on its
+    plain kernel). Point results are copied to pinned host
     memory; view G-buffers stay in HBM for their GPU consumer and are
     released on completion."""
     n_scenes = 8

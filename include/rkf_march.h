@@ -770,8 +770,9 @@ typedef struct rkfm_server_config {
     uint32_t max_requests;     /* admitted-request slots (0 = 1024)          */
     uint32_t rounds_in_flight; /* 1..4 (0 = 4); PER_REQUEST forces 1         */
     uint64_t max_round_work;   /* rays + queries per round (0 = 2^20; a      */
-                               /* round always takes at least one request;  */
-                               /* UINT64_MAX = no cap)                      */
+                               /* round always takes at least one request,  */
+                               /* and at least the admitted queue's work /  */
+                               /* rounds_in_flight; UINT64_MAX = no cap)    */
     uint32_t max_round_requests; /* requests per round (0 = 512)             */
     uint32_t policy;           /* RKFM_SERVE_*                               */
     rkfm_batch_tuning tuning;  /* batched launch shape (zero = 8x8 grid,     */

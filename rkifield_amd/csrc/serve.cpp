@@ -145,6 +145,7 @@ struct rkfm_server {
     std::deque<Req *> pending, admitted;
     std::unordered_map<uint64_t, Req *> reqs;
     uint32_t running = 0;
+    uint64_t admitted_work = 0;  // rays + queries of the admitted queue
     uint64_t next_id = 1, round_seq = 0, completions = 0;
     bool stop = false;
     rkfm_server_stats stats{};
@@ -229,6 +230,7 @@ bool rkfm_server::admit_locked() {
         q->st.t_admit = now();
         pending.pop_front();
         admitted.push_back(q);
+        admitted_work += q->work;
         any = true;
     }
     stats.arena_in_use = arena.used();
@@ -438,6 +440,7 @@ void rkfm_server::run() {
                 }
                 qq->clear();
             }
+            admitted_work = 0;
             cv_done.notify_all();
             bool busy = false;
             for (uint32_t i = 0; i < n_slots; i++) busy |= slots[i].busy;
@@ -456,10 +459,14 @@ void rkfm_server::run() {
             sl.reqs.clear();
             uint64_t work = 0;
             uint32_t cap = cfg.policy == RKFM_SERVE_PER_REQUEST ? 1u : cfg.max_round_requests;
+            // a round takes at least its share of the queued work, so a deep
+            // queue (a burst) runs in fewer, larger launches
+            const uint64_t cap_work = std::max<uint64_t>(cfg.max_round_work, admitted_work / n_slots);
             while (!admitted.empty() && sl.reqs.size() < cap) {
                 Req *q = admitted.front();
-                if (!sl.reqs.empty() && cfg.max_round_work && work + q->work > cfg.max_round_work) break;
+                if (!sl.reqs.empty() && cap_work && work + q->work > cap_work) break;
                 admitted.pop_front();
+                admitted_work -= q->work;
                 work += q->work;
                 sl.reqs.push_back(q);
             }
