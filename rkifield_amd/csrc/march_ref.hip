@@ -101,9 +101,10 @@ template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j)
 
 // ray_march.wgsl:641-985. Tiled mode: the tile's `count` object records and
 // indices, staged in LDS by the workgroup (the workgroup's pixels lie in one
-// 16x16 object tile), so a step's object loop reads no global memory; the
-// loop is shared by the ray's L lanes (lane `share` takes every L-th object).
-// bis: evaluations of the bisection (made by every lane of the ray).
+// 16x16 object tile), so a step's object loop reads no global memory. The
+// tiled and brute object loops are shared by the ray's L lanes (lane `share`
+// takes every L-th object). bis: evaluations of the bisection (made by every
+// lane of the ray).
 template <uint32_t MODE, uint32_t L>
 RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
                         uint32_t share, MarchRes &r, uint32_t &bis) {
@@ -186,15 +187,23 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
                 }
             }
         } else {
-            for (uint32_t i = 0; i < P.su.num_objects; i++) {
+            float bd = kMaxFloat;
+            uint32_t bm = 0, bj = kNoObj;
+            for (uint32_t i = share; i < P.su.num_objects; i += L) {
                 uint32_t m;
-                float d = eval_object(c, pos, i, m);
-                if (d < min_dist) {
-                    min_dist = d;
-                    best_mat = m;
-                    best_id = P.objects[i].object_id;
-                    best_idx = i;
+                const float d = eval_object(c, pos, i, m);
+                if (d < bd) {
+                    bd = d;
+                    bm = m;
+                    bj = i;
                 }
+            }
+            split_min<L>(bd, bm, bj);
+            if (bj != kNoObj) {
+                min_dist = bd;
+                best_mat = bm;
+                best_id = P.objects[bj].object_id;
+                best_idx = bj;
             }
         }
         if (min_dist < 0.0f && prev_dist > hit_eps(P.eps_mode, P.su.hit_threshold, prev_t)) {
@@ -287,7 +296,7 @@ template <uint32_t L> struct RefBlock {
 };
 template <uint32_t MODE, uint32_t L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
-    static_assert(MODE == 0 || L == 1, "only the tiled object loop is split across lanes");
+    static_assert(MODE != 1 || L == 1, "the BVH walk prunes by the running minimum: one lane per ray");
     constexpr uint32_t BW = RefBlock<L>::w, BH = RefBlock<L>::h;
     __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
     __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 
 }  // namespace rkf
 
-// Lanes per ray of the tiled instance.
+// Lanes per ray of the tiled and brute instances.
 constexpr uint32_t kTiledLanes = 8;
 
 extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream) {
@@ -373,7 +382,9 @@ extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t 
     } else if (P->mode == 1) {
         hipLaunchKernelGGL((rkf::ray_march_ref_kernel<1, 1>), grid, dim3(64, 1, 1), 0, stream, *P);
     } else {
-        hipLaunchKernelGGL((rkf::ray_march_ref_kernel<2, 1>), grid, dim3(64, 1, 1), 0, stream, *P);
+        using B = rkf::RefBlock<kTiledLanes>;
+        dim3 tg((P->g.width + B::w - 1u) / B::w, (P->g.height + B::h - 1u) / B::h, 1);
+        hipLaunchKernelGGL((rkf::ray_march_ref_kernel<2, kTiledLanes>), tg, dim3(64, 1, 1), 0, stream, *P);
     }
     return hipGetLastError();
 }
