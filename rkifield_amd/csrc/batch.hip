@@ -44,7 +44,7 @@ RKF_DEV void run_unit(const BatchDev &B, uint32_t u, uint32_t lane) {
         uint32_t tx = u % B.tiles_x, ty = u / B.tiles_x;
         uint32_t px = tx * B.tile_w + lane % B.tile_w;
         uint32_t py = ty * B.tile_h + lane / B.tile_w;
-        if (px < B.P.g.width && py < B.P.g.height) view_pixel<kLowLat>(B.P, px, py);
+        view_pixel<kLowLat>(B.P, px, py, px < B.P.g.width && py < B.P.g.height);
     } else if (B.kind == RKFM_BATCH_SHADOW) {
         uint32_t i = u * 64u + lane;
         if (i >= B.n) return;

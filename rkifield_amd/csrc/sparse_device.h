@@ -626,6 +626,146 @@ RKF_DEV float sp_sample_nb(SpCtx &c, const BrickNbhd &n, f3 p) {
     return sp_sample(c, p, false, true);
 }
 
+// ---- Sub-wave groups: 8 lanes per ray ----
+// Once at most kGroupRays rays of a wavefront are still marching, each is
+// handed to a group of 8 lanes (lanes 8g..8g+7 take the g-th unfinished ray
+// in lane order). Every lane of a group runs the ray's step logic on the same
+// values; the step's loads are split across the group (lane k: the L2 entry
+// of neighbour cell k, the slot of corner brick k, corner voxel k) and the
+// trilinear mix is reduced across it by lane exchanges, with the operations
+// and operand order of nb_sample, so a ray's results are those of its
+// one-lane march. A lone ray then issues a step's 8 loads and their address
+// arithmetic as one instruction each instead of eight, which shortens the
+// wavefront's remaining steps (its few long rays set the launch's end).
+// The neighbourhood cache is spread over the group: lane k keeps the slot of
+// corner brick b0 + (k & 1, k >> 1 & 1, k >> 2) in nb.slot[0] and, for
+// k >= 1, the L1 block of neighbour cell (k & 1, k >> 1 & 1, k >> 2) of
+// nb.cell in nb.slot[1].
+constexpr uint32_t kGroupRays = 8u;
+
+// DPP lane exchanges inside 8-lane groups: xor 1 and xor 2 (quad
+// permutations) and the half-row mirror (lane i <-> 7 - i), which equals
+// xor 4 once lanes 0-3 and lanes 4-7 each hold one value.
+RKF_DEV float dpp_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+RKF_DEV float dpp_xor2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+RKF_DEV float dpp_half_mirror(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+}
+RKF_DEV uint32_t lane_read(uint32_t v, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+
+// sp_locate for a group (lane k of the group whose first lane is `base`):
+// same decisions, same cache contents (spread as above).
+RKF_DEV int sp_locate_g(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, BrickNbhd &n,
+                        float g[3], int32_t v0[3], uint32_t k, uint32_t base) {
+    if (!sp_grid_pos(S, pos, g, v0)) return kLocEdge;
+    const uint64_t key = nb_key(S, v0);
+    if (key == n.key) return kLocCached;
+    const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
+    const uint32_t b[3] = {(uint32_t)v0[0] >> 3, (uint32_t)v0[1] >> 3, (uint32_t)v0[2] >> 3};
+    const uint32_t q[3] = {b[0] >> 3, b[1] >> 3, b[2] >> 3};
+    const uint64_t cell = (uint64_t)q[0] + (uint64_t)q[1] * S.cells[0] + (uint64_t)q[2] * S.cells[0] * S.cells[1];
+    const uint64_t cxy = (uint64_t)S.cells[0] * S.cells[1];
+    float lo_b[3], hi_b[3];
+    const uint32_t octant = (sd.x < 0.0f ? 1u : 0u) | (sd.y < 0.0f ? 2u : 0u) | (sd.z < 0.0f ? 4u : 0u);
+    uint32_t e = 0u;
+    if ((uint32_t)cell != n.cell) {
+        // every lane: the cell word (one address); lane k >= 1: the L2 entry of neighbour cell k
+        const bool ing = k != 0u && q[0] + kx < S.cells[0] && q[1] + ky < S.cells[1] && q[2] + kz < S.cells[2];
+        const uint64_t cw = S.cellw[cell];
+        uint32_t nl = S.l2[ing ? cell + kx + (uint64_t)ky * S.cells[0] + (uint64_t)kz * cxy : 0u];
+        uint32_t cw_lo = (uint32_t)cw, cw_hi = (uint32_t)(cw >> 32);
+        loaded(cw_lo);
+        loaded(cw_hi);
+        loaded(nl);
+        e = (cw_lo >> (4u * octant)) & 15u;
+        if (e == 0) {
+            n.cell = (uint32_t)cell;
+            n.row = cw_hi;
+            n.slot[1] = ing ? nl : RKF_EMPTY_SLOT;
+        }
+    }
+    if (e > 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((float)q[a] - (float)(e - 1u)) * 64.0f;
+            hi_b[a] = ((float)q[a] + (float)e) * 64.0f;
+        }
+    } else {
+        // lane k: the slot of corner brick k from the row word in the row of
+        // the cell it lies in (the neighbour cell's, read from lane j of the
+        // group, when it crosses a face); every lane: the brick's own row word
+        const bool fx = (b[0] & 7u) == 7u, fy = (b[1] & 7u) == 7u, fz = (b[2] & 7u) == 7u;
+        const uint32_t j = ((kx && fx) ? 1u : 0u) | ((ky && fy) ? 2u : 0u) | ((kz && fz) ? 4u : 0u);
+        const uint32_t nrow = lane_read(n.slot[1], base + j);
+        const uint32_t rk = j == 0u ? n.row : nrow;
+        const uint32_t bx = b[0] + kx, by = b[1] + ky, bz = b[2] + kz;
+        const bool ok = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2] && rk != RKF_EMPTY_SLOT;
+        const uint32_t lb = (b[0] & 7u) + (b[1] & 7u) * 8u + (b[2] & 7u) * 64u;
+        const uint32_t lbk = (bx & 7u) + (by & 7u) * 8u + (bz & 7u) * 64u;
+        const uint64_t rw = S.roww[(uint64_t)n.row * 512u + lb];
+        uint32_t va = roww_slots(S)[ok ? ((uint64_t)rk * 512u + lbk) * 2u : 0u];
+        uint32_t rw_lo = (uint32_t)rw, rw_hi = (uint32_t)(rw >> 32);
+        loaded(rw_lo);
+        loaded(rw_hi);
+        loaded(va);
+        const uint32_t eb = (rw_hi >> (4u * octant)) & 15u;
+        if (eb == 0) {
+            n.key = key;
+            n.slot[0] = ok ? va : RKF_EMPTY_SLOT;
+            return kLocCached;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo_b[a] = ((float)b[a] - (float)(eb - 1u)) * 8.0f;
+            hi_b[a] = ((float)b[a] + (float)eb) * 8.0f;
+        }
+    }
+    const float dv[3] = {sd.x, sd.y, sd.z};
+    const float iv[3] = {inv.x, inv.y, inv.z};
+    float dt = kMaxFloat;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float bound = dv[a] > 0.0f ? hi_b[a] : lo_b[a];
+        const float da = (bound - g[a]) * iv[a];
+        dt = fminf(dt, da);
+    }
+    t_exit = t + fmaxf(dt, 0.0f);
+    return kLocFree;
+}
+
+// nb_sample for a group: lane k loads corner k (k = x + 2y + 4z), the
+// group reduces x, then y, then z with nb_sample's mixes.
+RKF_DEV float nb_sample_g(const SparseDev &S, const BrickNbhd &n, const float g[3], const int32_t v0[3], uint32_t k,
+                          uint32_t base) {
+    const float vs = S.vs;
+    const float v_empty = vs * 8.0f, v_interior = -(vs * 2.0f);
+    const int32_t x0 = v0[0], y0 = v0[1], z0 = v0[2];
+    const float tx = g[0] - (float)x0, ty = g[1] - (float)y0, tz = g[2] - (float)z0;
+    const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
+    const bool ox = (x0 & 7) == 7, oy = (y0 & 7) == 7, oz = (z0 & 7) == 7;
+    const uint32_t j = ((kx && ox) ? 1u : 0u) | ((ky && oy) ? 2u : 0u) | ((kz && oz) ? 4u : 0u);
+    const uint32_t sl = lane_read(n.slot[0], base + j);
+    const bool live = sl < RKF_INTERIOR_SLOT;
+    const uint32_t loc = (uint32_t)((x0 + (int32_t)kx) & 7) + (uint32_t)((y0 + (int32_t)ky) & 7) * 8u +
+                         (uint32_t)((z0 + (int32_t)kz) & 7) * 64u;
+    uint32_t raw = S.dist[(uint64_t)(live ? sl : 0u) * 512u + (live ? loc : 0u)];
+    loaded(raw);
+    const uint32_t sent = __float_as_uint(sl == RKF_EMPTY_SLOT ? v_empty : v_interior);
+    const float c = __uint_as_float(live ? __float_as_uint(h2f(raw)) : sent);
+    const float p1 = dpp_xor1(c);
+    const float c1 = kx ? mix(p1, c, tx) : mix(c, p1, tx);
+    const float p2 = dpp_xor2(c1);
+    const float c2 = ky ? mix(p2, c1, ty) : mix(c1, p2, ty);
+    const float p3 = dpp_half_mirror(c2);
+    return (kz ? mix(p3, c2, tz) : mix(c2, p3, tz)) + 0.0f;
+}
+
 struct SpRes {
     float t, mat_d;
     f3 mat_p;
@@ -674,22 +814,30 @@ RKF_DEV void prim_cull(PrimRay &r, uint2 iv, float vs) {
     }
 }
 
+// Step codes: kModeThroughput (sp_free + sp_sample with the prefetched
+// slot: fewer memory instructions, for launches with enough rays in flight
+// to hide latency), kModeLowLat (the brick-neighbourhood cache, sp_locate +
+// nb_sample: fewer dependent loads per step, for launches bounded by their
+// slowest rays), kModeGroup (kModeLowLat spread over an 8-lane group). All
+// give the same values.
+enum : int { kModeThroughput = 0, kModeLowLat = 1, kModeGroup = 2 };
+
 // One step's field query: false for a free-space skip (t_exit set), else
-// the stepping sample d. kLowLat: the brick-neighbourhood cache (sp_locate +
-// nb_sample), fewer dependent loads per step, for launches bounded by their
-// slowest rays; otherwise sp_free + sp_sample with the prefetched slot, fewer
-// memory instructions, for launches with enough rays in flight to hide
-// latency. Both give the same values.
-template <bool kLowLat>
+// the stepping sample d.
+template <int kMode>
 RKF_DEV bool step_field(const SparseDev &S, PrimRay &r, f3 pos, float &t_exit, SpCtx &c, float &d) {
-    if constexpr (kLowLat) {
+    if constexpr (kMode != kModeThroughput) {
         float g[3];
         int32_t v0[3];
-        const int loc = sp_locate(S, pos, r.sd, r.inv, r.t, t_exit, r.nb, g, v0);
+        const uint32_t k = threadIdx.x & 7u, base = (threadIdx.x & 63u) & ~7u;
+        int loc;
+        if constexpr (kMode == kModeGroup) loc = sp_locate_g(S, pos, r.sd, r.inv, r.t, t_exit, r.nb, g, v0, k, base);
+        else loc = sp_locate(S, pos, r.sd, r.inv, r.t, t_exit, r.nb, g, v0);
         if (loc == kLocFree) return false;
         if (loc == kLocCached) {
             c.evals++;
-            d = nb_sample(S, r.nb, g, v0);
+            if constexpr (kMode == kModeGroup) d = nb_sample_g(S, r.nb, g, v0, k, base);
+            else d = nb_sample(S, r.nb, g, v0);
         } else {
             d = sp_sample(c, pos, false, true);
         }
@@ -702,7 +850,7 @@ RKF_DEV bool step_field(const SparseDev &S, PrimRay &r, f3 pos, float &t_exit, S
     }
 }
 
-template <bool kLowLat>
+template <int kMode>
 RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &su, uint32_t eps_mode, SpRes &out) {
     if (r.steps >= kHardMax || r.t > r.tmax || r.near >= su.max_steps) return kRayMiss;
     const float vs = S.vs;
@@ -711,7 +859,7 @@ RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &
     float tx;
     SpCtx c{&S, r.evals};
     float d;
-    if (!step_field<kLowLat>(S, r, pos, tx, c, d)) {
+    if (!step_field<kMode>(S, r, pos, tx, c, d)) {
         r.prev_t = tx;
         r.prev_dist = vs * 8.0f;
         r.t = tx + vs * 0.00390625f;
@@ -722,12 +870,12 @@ RKF_DEV int prim_step(const SparseDev &S, PrimRay &r, const rkf_scene_uniforms &
         // low latency: the bricks within 3 voxels of the segment's midpoint,
         // looked up once for the 8 bisection samples
         BrickCache bc;
-        if constexpr (kLowLat) bc_init(S, add(r.o, muls(r.sd, (lo + hi) * 0.5f)), 3, bc);
+        if constexpr (kMode != kModeThroughput) bc_init(S, add(r.o, muls(r.sd, (lo + hi) * 0.5f)), 3, bc);
         for (uint32_t b = 0; b < kBisect; b++) {
             float mid = (lo + hi) * 0.5f;
             const f3 mp = add(r.o, muls(r.sd, mid));
             float mdd;
-            if constexpr (kLowLat) {
+            if constexpr (kMode != kModeThroughput) {
                 c.evals++;
                 mdd = sp_sample_bc(S, bc, mp, false);
             } else {
@@ -781,7 +929,7 @@ RKF_DEV void shadow_init(ShadowRay &r, f3 o, f3 L, float vs) {
 }
 
 // Returns true when finished; r.prev_dist then holds the visibility.
-template <bool kLowLat>
+template <int kMode>
 RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) {
     if (r.steps >= 256u || r.near >= 32u || r.t > maxd) {
         r.prev_dist = clampf(r.prev_dist, 0.0f, 1.0f);
@@ -792,7 +940,7 @@ RKF_DEV bool shadow_step(const SparseDev &S, ShadowRay &r, float maxd, float k) 
     float tx;
     SpCtx c{&S, r.evals};
     float d;
-    if (!step_field<kLowLat>(S, r, p, tx, c, d)) {
+    if (!step_field<kMode>(S, r, p, tx, c, d)) {
         r.t = tx + S.vs * 0.00390625f;
         return false;
     }
@@ -875,11 +1023,110 @@ RKF_DEV void shadow_store(const SparseMarchParams &P, uint64_t p, const ShadowRa
     }
 }
 
-// One pixel of a view, marched to completion, then its shadow ray.
+// Group of lane `lane` when the unfinished rays are the set bits of `pend`
+// (wave-uniform): the lane holding the group's ray (the g-th set bit).
+RKF_DEV uint32_t group_source(uint64_t pend, uint32_t grp) {
+    uint32_t src = 0u;
+    uint64_t m = pend;
+#pragma unroll
+    for (uint32_t i = 0; i < kGroupRays; i++) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(m | 0x8000000000000000ull);
+        src = grp == i ? b : src;
+        m &= m - 1ull;
+    }
+    return src;
+}
+
+RKF_DEV float lane_readf(float v, uint32_t src) { return __uint_as_float(lane_read(__float_as_uint(v), src)); }
+RKF_DEV f3 lane_read3(f3 v, uint32_t src) { return mk(lane_readf(v.x, src), lane_readf(v.y, src), lane_readf(v.z, src)); }
+
+// The ray state a group needs from the ray's own lane (the neighbourhood
+// cache starts empty: it never changes a value).
+RKF_DEV void group_gather(const PrimRay &r, uint32_t src, PrimRay &q) {
+    q.o = lane_read3(r.o, src);
+    q.sd = lane_read3(r.sd, src);
+    q.inv = lane_read3(r.inv, src);
+    q.t = lane_readf(r.t, src);
+    q.prev_t = lane_readf(r.prev_t, src);
+    q.prev_dist = lane_readf(r.prev_dist, src);
+    q.tmax = lane_readf(r.tmax, src);
+    q.near = lane_read(r.near, src);
+    q.steps = lane_read(r.steps, src);
+    q.evals = lane_read(r.evals, src);
+    nb_reset(q.nb);
+}
+
+// Primary rays still marching (st == kRayContinue) when at most kGroupRays
+// of the wave are: finished in 8-lane groups, results back in their lanes.
+// All 64 lanes of the wavefront must be active.
+RKF_DEV void prim_groups(const SparseMarchParams &P, PrimRay &r, int &st, SpRes &res) {
+    const uint64_t pend = __ballot(st == kRayContinue);
+    if (pend == 0ull) return;
+    const uint32_t lane = threadIdx.x & 63u, grp = lane >> 3;
+    PrimRay q;
+    group_gather(r, group_source(pend, grp), q);
+    SpRes qr;
+    qr.t = qr.mat_d = 0.0f;
+    qr.mat_p = mk(0.0f, 0.0f, 0.0f);
+    qr.hit = false;
+    qr.steps = 0u;
+    int qs = kRayContinue;
+    if (grp < (uint32_t)__popcll(pend)) {
+        do {
+            qs = prim_step<kModeGroup>(P.S, q, P.su, P.ext.hit_eps_mode, qr);
+        } while (qs == kRayContinue);
+    }
+    const uint32_t lead = 8u * (uint32_t)__popcll(pend & ((1ull << lane) - 1ull));
+    const int s2 = (int)lane_read((uint32_t)qs, lead);
+    const uint32_t steps = lane_read(q.steps, lead), evals = lane_read(q.evals, lead);
+    SpRes r2;
+    r2.t = lane_readf(qr.t, lead);
+    r2.mat_d = lane_readf(qr.mat_d, lead);
+    r2.mat_p = lane_read3(qr.mat_p, lead);
+    r2.hit = lane_read(qr.hit ? 1u : 0u, lead) != 0u;
+    r2.steps = lane_read(qr.steps, lead);
+    if ((pend >> lane) & 1ull) {
+        st = s2;
+        r.steps = steps;
+        r.evals = evals;
+        res = r2;
+    }
+}
+
+// Shadow rays still marching (pending) when at most kGroupRays of the wave
+// are: finished in 8-lane groups, visibility and counters back in their
+// lanes. All 64 lanes of the wavefront must be active.
+RKF_DEV void shadow_groups(const SparseDev &S, ShadowRay &r, bool pending, float maxd, float k) {
+    const uint64_t pend = __ballot(pending);
+    if (pend == 0ull) return;
+    const uint32_t lane = threadIdx.x & 63u, grp = lane >> 3;
+    ShadowRay q;
+    group_gather(r, group_source(pend, grp), q);
+    if (grp < (uint32_t)__popcll(pend)) {
+        while (!shadow_step<kModeGroup>(S, q, maxd, k)) {
+        }
+    }
+    const uint32_t lead = 8u * (uint32_t)__popcll(pend & ((1ull << lane) - 1ull));
+    const float vis = lane_readf(q.prev_dist, lead);
+    const uint32_t steps = lane_read(q.steps, lead), evals = lane_read(q.evals, lead);
+    if (pending) {
+        r.prev_dist = vis;
+        r.steps = steps;
+        r.evals = evals;
+    }
+}
+
+// One pixel of a view: its primary ray, the hit's shading and G-buffer
+// record, then its shadow ray. `live` false: a lane with no pixel (past the
+// view's edge), which only takes part in the wave's group hand-offs. With
+// the low-latency step code, the last kGroupRays primary rays and the last
+// kGroupRays shadow rays of the wavefront finish in 8-lane groups (all 64
+// lanes of the wavefront must call this).
 template <bool kLowLat>
-RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
+RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py, bool live = true) {
     uint2 iv = make_uint2(0u, 0u);
-    if (P.tiles) {
+    bool run = live;
+    if (run && P.tiles) {
         iv = P.tiles[(uint64_t)(py >> 3) * P.tiles_x + (px >> 3)];
         if (iv.x == 0x7F800000u) {
             // a tile no occupied cell projects to: prim_cull sets t = +inf
@@ -888,27 +1135,62 @@ RKF_DEV void view_pixel(const SparseMarchParams &P, uint32_t px, uint32_t py) {
             store_miss(P.g, p);
             store_debug(P.g, p, mk(0.0f, 0.0f, 0.0f), RKF_EMPTY_SLOT, 0xFFFFFFFFu, 0xFFFFFFFFu, 1u, 0u);
             if (P.g.shadow) P.g.shadow[p] = 0.0f;
-            return;
+            run = false;
         }
     }
-    f3 o, dir;
-    camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
-    PrimRay r;
-    prim_init(r, o, dir, P.S.vs, P.su.max_distance);
-    if (P.tiles) prim_cull(r, iv, P.S.vs);
-    SpRes res;
-    int st;
-    do {
-        st = prim_step<kLowLat>(P.S, r, P.su, P.ext.hit_eps_mode, res);
-    } while (st == kRayContinue);
-    f3 so;
-    if (prim_finish(P, px, py, r, st, res, so)) {
-        ShadowRay sr;
-        shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
-        float maxd = shadow_maxd(P), k = shadow_k(P);
-        while (!shadow_step<kLowLat>(P.S, sr, maxd, k)) {
+    if constexpr (!kLowLat) {
+        if (!run) return;
+        f3 o, dir;
+        camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
+        PrimRay r;
+        prim_init(r, o, dir, P.S.vs, P.su.max_distance);
+        if (P.tiles) prim_cull(r, iv, P.S.vs);
+        SpRes res;
+        int st;
+        do {
+            st = prim_step<kModeThroughput>(P.S, r, P.su, P.ext.hit_eps_mode, res);
+        } while (st == kRayContinue);
+        f3 so;
+        if (prim_finish(P, px, py, r, st, res, so)) {
+            ShadowRay sr;
+            shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
+            float maxd = shadow_maxd(P), k = shadow_k(P);
+            while (!shadow_step<kModeThroughput>(P.S, sr, maxd, k)) {
+            }
+            shadow_store(P, (uint64_t)py * P.g.width + px, sr);
         }
-        shadow_store(P, (uint64_t)py * P.g.width + px, sr);
+    } else {
+        PrimRay r;
+        SpRes res;
+        int st = kRayMiss;
+        if (run) {
+            f3 o, dir;
+            camera_ray(P.cam, P.g.width, P.g.height, px, py, o, dir);
+            prim_init(r, o, dir, P.S.vs, P.su.max_distance);
+            if (P.tiles) prim_cull(r, iv, P.S.vs);
+            for (;;) {
+                st = prim_step<kModeLowLat>(P.S, r, P.su, P.ext.hit_eps_mode, res);
+                const bool more = st == kRayContinue;
+                const uint64_t m = __ballot(more);
+                if (!more || (uint32_t)__popcll(m) <= kGroupRays) break;
+            }
+        }
+        prim_groups(P, r, st, res);
+        f3 so;
+        const bool sh = run && prim_finish(P, px, py, r, st, res, so);
+        const float maxd = shadow_maxd(P), k = shadow_k(P);
+        ShadowRay sr;
+        bool done = true;
+        if (sh) {
+            shadow_init(sr, so, mk(P.ext.light_dir[0], P.ext.light_dir[1], P.ext.light_dir[2]), P.S.vs);
+            for (;;) {
+                done = shadow_step<kModeLowLat>(P.S, sr, maxd, k);
+                const uint64_t m = __ballot(!done);
+                if (done || (uint32_t)__popcll(m) <= kGroupRays) break;
+            }
+        }
+        shadow_groups(P.S, sr, !done, maxd, k);
+        if (sh) shadow_store(P, (uint64_t)py * P.g.width + px, sr);
     }
 }
 

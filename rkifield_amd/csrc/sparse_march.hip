@@ -9,8 +9,7 @@ namespace rkf {
 __global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
-    if (px >= P.g.width || py >= P.g.height) return;
-    view_pixel<true>(P, px, py);
+    view_pixel<true>(P, px, py, px < P.g.width && py < P.g.height);
 }
 
 // Tuned variant: persistent wavefronts with per-lane ray refill and no
