@@ -346,6 +346,7 @@ struct BrickNbhd {
     uint32_t slot[8];  // brick b0 + (k & 1, k >> 1 & 1, k >> 2); constant indices only
     uint32_t cell;     // last cell met that is not free for this ray's octant, ~0 when none
     uint32_t row;      // its free-mask row
+    uint32_t nrow;     // 8-lane groups: the row of neighbour cell (k & 1, k >> 1 & 1, k >> 2) of `cell`
 };
 
 RKF_DEV void nb_reset(BrickNbhd &n) {
@@ -637,10 +638,11 @@ RKF_DEV float sp_sample_nb(SpCtx &c, const BrickNbhd &n, f3 p) {
 // one-lane march. A lone ray then issues a step's 8 loads and their address
 // arithmetic as one instruction each instead of eight, which shortens the
 // wavefront's remaining steps (its few long rays set the launch's end).
-// The neighbourhood cache is spread over the group: lane k keeps the slot of
-// corner brick b0 + (k & 1, k >> 1 & 1, k >> 2) in nb.slot[0] and, for
-// k >= 1, the L1 block of neighbour cell (k & 1, k >> 1 & 1, k >> 2) of
-// nb.cell in nb.slot[1].
+// Each lane of a group loads one slot of the 2x2x2 neighbourhood (lane k:
+// corner brick b0 + (k & 1, k >> 1 & 1, k >> 2)) and the group exchanges
+// them once per refill, so every lane holds the whole cache and a sample
+// picks its corner's slot by selects; lane k >= 1 keeps the L1 block of
+// neighbour cell (k & 1, k >> 1 & 1, k >> 2) of nb.cell in nb.nrow.
 constexpr uint32_t kGroupRays = 8u;
 
 // DPP lane exchanges inside 8-lane groups: xor 1 and xor 2 (quad
@@ -687,7 +689,7 @@ RKF_DEV int sp_locate_g(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, floa
         if (e == 0) {
             n.cell = (uint32_t)cell;
             n.row = cw_hi;
-            n.slot[1] = ing ? nl : RKF_EMPTY_SLOT;
+            n.nrow = ing ? nl : RKF_EMPTY_SLOT;
         }
     }
     if (e > 0) {
@@ -702,7 +704,7 @@ RKF_DEV int sp_locate_g(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, floa
         // group, when it crosses a face); every lane: the brick's own row word
         const bool fx = (b[0] & 7u) == 7u, fy = (b[1] & 7u) == 7u, fz = (b[2] & 7u) == 7u;
         const uint32_t j = ((kx && fx) ? 1u : 0u) | ((ky && fy) ? 2u : 0u) | ((kz && fz) ? 4u : 0u);
-        const uint32_t nrow = lane_read(n.slot[1], base + j);
+        const uint32_t nrow = lane_read(n.nrow, base + j);
         const uint32_t rk = j == 0u ? n.row : nrow;
         const uint32_t bx = b[0] + kx, by = b[1] + ky, bz = b[2] + kz;
         const bool ok = bx < S.nb[0] && by < S.nb[1] && bz < S.nb[2] && rk != RKF_EMPTY_SLOT;
@@ -717,7 +719,9 @@ RKF_DEV int sp_locate_g(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, floa
         const uint32_t eb = (rw_hi >> (4u * octant)) & 15u;
         if (eb == 0) {
             n.key = key;
-            n.slot[0] = ok ? va : RKF_EMPTY_SLOT;
+            const uint32_t mine = ok ? va : RKF_EMPTY_SLOT;
+#pragma unroll
+            for (uint32_t m = 0; m < 8u; m++) n.slot[m] = lane_read(mine, base + m);
             return kLocCached;
         }
 #pragma unroll
@@ -749,8 +753,7 @@ RKF_DEV float nb_sample_g(const SparseDev &S, const BrickNbhd &n, const float g[
     const float tx = g[0] - (float)x0, ty = g[1] - (float)y0, tz = g[2] - (float)z0;
     const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
     const bool ox = (x0 & 7) == 7, oy = (y0 & 7) == 7, oz = (z0 & 7) == 7;
-    const uint32_t j = ((kx && ox) ? 1u : 0u) | ((ky && oy) ? 2u : 0u) | ((kz && oz) ? 4u : 0u);
-    const uint32_t sl = lane_read(n.slot[0], base + j);
+    const uint32_t sl = nb_pick(n, kx && ox, ky && oy, kz && oz);
     const bool live = sl < RKF_INTERIOR_SLOT;
     const uint32_t loc = (uint32_t)((x0 + (int32_t)kx) & 7) + (uint32_t)((y0 + (int32_t)ky) & 7) * 8u +
                          (uint32_t)((z0 + (int32_t)kz) & 7) * 64u;

@@ -6,7 +6,7 @@ namespace rkf {
 
 // Plain variant: one lane per pixel, 8x8 pixels per wavefront, each ray
 // marched to completion, then its shadow ray.
-__global__ __launch_bounds__(64) void sparse_march_plain(SparseMarchParams P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sparse_march_plain(SparseMarchParams P) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     view_pixel<true>(P, px, py, px < P.g.width && py < P.g.height);
