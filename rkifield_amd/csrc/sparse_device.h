@@ -246,7 +246,7 @@ RKF_DEV float sp_sample_bc(const SparseDev &S, const BrickCache &bc, f3 p, bool 
 // Nearest voxel floor(grid_pos * inv_vs) as ray_march.wgsl:540-596. Returns the
 // pool index or ~0.
 RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t &vox_o, uint32_t &cell_o,
-                            const BrickCache *bc = nullptr) {
+                            const BrickCache &bc) {
     float vs = S.vs;
     float be = vs * 8.0f;
     f3 gs = mk((float)S.nb[0] * be, (float)S.nb[1] * be, (float)S.nb[2] * be);
@@ -259,7 +259,7 @@ RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t
     int32_t cy = clampi((int32_t)floorf(gp.y * S.inv_vs), 0, S.tot[1] - 1);
     int32_t cz = clampi((int32_t)floorf(gp.z * S.inv_vs), 0, S.tot[2] - 1);
     bool in_cache = false;
-    uint32_t slot = bc ? bc_slot(*bc, cx, cy, cz, in_cache) : 0u;
+    uint32_t slot = bc_slot(bc, cx, cy, cz, in_cache);
     if (!in_cache) slot = sp_slot(S, cx, cy, cz);
     cell_o = ((uint32_t)cx >> 6) + ((uint32_t)cy >> 6) * S.cells[0] + ((uint32_t)cz >> 6) * S.cells[0] * S.cells[1];
     slot_o = slot;
@@ -269,7 +269,7 @@ RKF_DEV uint64_t sp_nearest(const SparseDev &S, f3 p, uint32_t &slot_o, uint32_t
     return (uint64_t)slot * 512u + lv;
 }
 
-RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p, const BrickCache *bc = nullptr) {
+RKF_DEV uint32_t sp_material(const SparseDev &S, f3 p, const BrickCache &bc) {
     uint32_t a, b, d;
     uint64_t i = sp_nearest(S, p, a, b, d, bc);
     return i == ~0ull ? 0u : (uint32_t)(S.mat[i] & 0x3Fu);
@@ -984,7 +984,7 @@ RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, c
     BrickCache bc;
     bc_init(S, hp, 3, bc);
     uint32_t mat = 0;
-    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p, &bc);
+    if (res.mat_d < S.vs * 16.0f) mat = sp_material(S, res.mat_p, bc);
     float gx = sp_sample_bc(S, bc, add(hp, mk(e, 0.0f, 0.0f)), true) - sp_sample_bc(S, bc, sub(hp, mk(e, 0.0f, 0.0f)), true);
     float gy = sp_sample_bc(S, bc, add(hp, mk(0.0f, e, 0.0f)), true) - sp_sample_bc(S, bc, sub(hp, mk(0.0f, e, 0.0f)), true);
     float gz = sp_sample_bc(S, bc, add(hp, mk(0.0f, 0.0f, e)), true) - sp_sample_bc(S, bc, sub(hp, mk(0.0f, 0.0f, e)), true);
@@ -992,7 +992,7 @@ RKF_DEV bool prim_finish(const SparseMarchParams &P, uint32_t px, uint32_t py, c
     float len = length(gv);
     f3 n = len < 1e-10f ? mk(0.0f, 1.0f, 0.0f) : mk(gv.x / len, gv.y / len, gv.z / len);
     uint32_t slot, vox, cell;
-    uint64_t vi = sp_nearest(S, hp, slot, vox, cell, &bc);
+    uint64_t vi = sp_nearest(S, hp, slot, vox, cell, bc);
     uint32_t sec = 0, blend = 0;
     if (S.aux && vi != ~0ull) {
         uint32_t a = S.aux[vi];
@@ -1214,7 +1214,7 @@ RKF_DEV void point_query(const SparseDev &S, f3 p, rkfm_point_result &out) {
     float rcp = 1.0f / len;
     rkfm_point_result r;
     r.distance = d;
-    r.material = sp_material(S, p, &bc);
+    r.material = sp_material(S, p, bc);
     r.inside = d < 0.0f ? 1u : 0u;
     if (isfinite(rcp) && rcp > 0.0f) {
         r.gradient[0] = dx * rcp;
