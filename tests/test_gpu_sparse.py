@@ -209,6 +209,37 @@ def test_sparse_march_parity(ctx, small_scene, variant, cull):
     assert ang.max() <= NORMAL_RAD_TOL
 
 
+@pytest.mark.parametrize("cull", [0, 1])
+def test_sparse_march_ragged_view(ctx, small_scene, cull):
+    """A view whose edge tiles are partly outside it: those wavefronts hand
+    their last rays to 8-lane groups with lanes that hold no pixel. Every
+    field bit-exact against the FP32 twin."""
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    w, h = 61, 45
+    cam = M.camera_look_at((5.5, 2.5, -6.5), (0, 0, 0), 60.0, w, h)
+    su = L.scene_uniforms(num_objects=1)
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1,
+                      light_dir=(np.array([0.5, 1.0, 0.3]) / np.linalg.norm([0.5, 1.0, 0.3])),
+                      march_flags=L.MARCH_TILE_CULL if cull else 0)
+    _say(f"case: sparse march ragged {w}x{h} cull {cull}")
+    gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
+    sc.march(cam, su, ext, gb)
+    gpu = gb.read()
+    res32 = O.sparse_trace(sp, cam, su, ext, w, h, prec=32)
+    assert res32["hit"].sum() > 200
+    edge = (np.arange(w * h) % w >= 56) | (np.arange(w * h) // w >= 40)
+    assert res32["hit"][edge].sum() > 10
+    pos, nrm, mat, mot = O.pack_gbuffer(res32)
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals", "shadow_steps", "shadow_evals"):
+        assert np.array_equal(gpu["debug"][f], res32[f]), f
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["material"], mat)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    assert np.array_equal(gpu["shadow"].view(np.uint32), res32["shadow"].astype(np.float32).view(np.uint32))
+
+
 def test_point_queries_parity(ctx, small_scene):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)
