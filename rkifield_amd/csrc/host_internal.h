@@ -55,7 +55,7 @@ struct rkfm_ctx {
     DevBuf march_hits, march_count;
     // Tile intervals of the last culled march: per-tile pairs, the large
     // rectangle records and their counter.
-    DevBuf tile_iv, tile_big, tile_big_count;
+    DevBuf tile_iv, tile_big, tile_big_count, tile_cnt, tile_order;
     // Edits: op records (and their pinned staging, reusable once edit_ev
     // has fired), the listed bricks, their modified flags, the list count.
     DevBuf edit_ops, edit_list, edit_mod, edit_count;

@@ -81,6 +81,7 @@ struct SparseMarchParams {
     uint32_t object_id;
     uint32_t tiles_x;          // 8x8 tiles per row of `tiles`
     const uint2 *tiles;        // per tile (t_lo, t_hi) float bits, or null
+    const uint32_t *order;     // plain kernel: tiles in launch order, or null (row-major)
 };
 
 // A cell whose tile rectangle is too large for one lane (tile binning).
@@ -125,8 +126,13 @@ hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm_sphere_cas
                                   rkfm_cast_result *out, hipStream_t stream);
 // Per-tile primary-ray intervals: clear, bin every L1 block's cell, then the
 // cells whose rectangle exceeds kTileInlineMax tiles (big holds up to
-// n_blocks records, big_count one counter).
+// n_blocks records, big_count one counter). cnt (or null): per tile, the
+// number of cells binned into it; the 512 words before cnt[0] must exist
+// (rkf_launch_sparse_tile_order's bins, zeroed here).
 hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam, float max_distance,
-                                   uint32_t width, uint32_t height, uint2 *tiles, rkf::TileBig *big,
+                                   uint32_t width, uint32_t height, uint2 *tiles, uint32_t *cnt, rkf::TileBig *big,
                                    uint32_t *big_count, hipStream_t stream);
+// Tiles by descending cell count (order holds n_tiles entries; cnt as filled
+// by rkf_launch_sparse_tiles, with its bins before it).
+hipError_t rkf_launch_sparse_tile_order(uint32_t *cnt, uint32_t n_tiles, uint32_t *order, hipStream_t stream);
 }

@@ -5,10 +5,18 @@
 namespace rkf {
 
 // Plain variant: one lane per pixel, 8x8 pixels per wavefront, each ray
-// marched to completion, then its shadow ray.
+// marched to completion, then its shadow ray. With a tile order (culled
+// views), workgroup i marches tile order[i]: the tiles with most occupied
+// cells start first, so the long ones do not start last.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sparse_march_plain(SparseMarchParams P) {
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    uint32_t bx = blockIdx.x, by = blockIdx.y;
+    if (P.order) {
+        const uint32_t tile = P.order[blockIdx.y * gridDim.x + blockIdx.x];
+        bx = tile % gridDim.x;
+        by = tile / gridDim.x;
+    }
+    uint32_t px = bx * 8u + (threadIdx.x & 7u);
+    uint32_t py = by * 8u + (threadIdx.x >> 3);
     view_pixel<true>(P, px, py, px < P.g.width && py < P.g.height);
 }
 
@@ -280,22 +288,31 @@ RKF_DEV bool tile_cell(const SparseDev &S, const TileCam &tc, uint32_t cell, flo
     return true;
 }
 
-RKF_DEV void tile_merge(uint2 *tiles, uint64_t i, float tn, float tf) {
+RKF_DEV void tile_merge(uint2 *tiles, uint32_t *cnt, uint64_t i, float tn, float tf) {
     uint32_t *w = reinterpret_cast<uint32_t *>(tiles + i);
     atomicMin(w, __float_as_uint(tn));
     atomicMax(w + 1, __float_as_uint(tf));
+    if (cnt) atomicAdd(cnt + i, 1u);
 }
 
-__global__ __launch_bounds__(256) void tiles_clear_kernel(uint2 *tiles, uint32_t n, uint32_t *big_count) {
+__global__ __launch_bounds__(256) void tiles_clear_kernel(uint2 *tiles, uint32_t *cnt, uint32_t *bins, uint32_t n,
+                                                          uint32_t *big_count) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i == 0u) *big_count = 0u;
-    if (i < n) tiles[i] = make_uint2(0x7F800000u, 0u);
+    if (bins && blockIdx.x == 0u) {
+        bins[threadIdx.x] = 0u;
+        bins[256u + threadIdx.x] = 0u;
+    }
+    if (i < n) {
+        tiles[i] = make_uint2(0x7F800000u, 0u);
+        if (cnt) cnt[i] = 0u;
+    }
 }
 
 // One lane per L1 block; rectangles above kTileInlineMax tiles go to `big`.
 __global__ __launch_bounds__(256) void tiles_bin_kernel(SparseDev S, rkf_camera_uniforms cam, float max_distance,
-                                                        uint32_t w, uint32_t h, uint2 *tiles, TileBig *big,
-                                                        uint32_t *big_count) {
+                                                        uint32_t w, uint32_t h, uint2 *tiles, uint32_t *cnt,
+                                                        TileBig *big, uint32_t *big_count) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= S.n_blocks) return;
     const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
@@ -305,7 +322,7 @@ __global__ __launch_bounds__(256) void tiles_bin_kernel(SparseDev S, rkf_camera_
     if (!tile_cell(S, tc, S.block_cell[i], max_distance, tiles_x, tiles_y, x0, x1, y0, y1, tn, tf)) return;
     if ((x1 - x0 + 1u) * (y1 - y0 + 1u) <= kTileInlineMax) {
         for (uint32_t y = y0; y <= y1; y++)
-            for (uint32_t x = x0; x <= x1; x++) tile_merge(tiles, (uint64_t)y * tiles_x + x, tn, tf);
+            for (uint32_t x = x0; x <= x1; x++) tile_merge(tiles, cnt, (uint64_t)y * tiles_x + x, tn, tf);
         return;
     }
     const uint32_t k = atomicAdd(big_count, 1u);
@@ -314,14 +331,61 @@ __global__ __launch_bounds__(256) void tiles_bin_kernel(SparseDev S, rkf_camera_
 
 // Large rectangles: one workgroup per record, grid-stride over the records.
 __global__ __launch_bounds__(256) void tiles_big_kernel(const TileBig *big, const uint32_t *big_count,
-                                                        uint32_t tiles_x, uint2 *tiles) {
+                                                        uint32_t tiles_x, uint2 *tiles, uint32_t *cnt) {
     const uint32_t n = *big_count;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
         const TileBig b = big[k];
         const uint32_t rw = b.x1 - b.x0 + 1u, area = rw * (b.y1 - b.y0 + 1u);
         for (uint32_t j = threadIdx.x; j < area; j += 256u)
-            tile_merge(tiles, (uint64_t)(b.y0 + j / rw) * tiles_x + (b.x0 + j % rw), b.tn, b.tf);
+            tile_merge(tiles, cnt, (uint64_t)(b.y0 + j / rw) * tiles_x + (b.x0 + j % rw), b.tn, b.tf);
     }
+}
+
+// Launch order of a culled view's tiles: by the number of occupied cells
+// binned into each (capped at 255), most first; within a count, any order.
+// A counting sort in two launches: per-workgroup LDS histograms summed into
+// `bins[0..255]`, then each workgroup ranks its tiles in LDS, reserves its
+// range in every count's slice with one atomic on `bins[256 + count]`, and
+// writes them (bins zeroed by tiles_clear_kernel).
+constexpr uint32_t kOrderBins = 256u;
+__global__ __launch_bounds__(256) void tiles_hist_kernel(const uint32_t *cnt, uint32_t n, uint32_t *bins) {
+    __shared__ uint32_t h[kOrderBins];
+    const uint32_t tid = threadIdx.x, i = blockIdx.x * 256u + tid;
+    h[tid] = 0u;
+    __syncthreads();
+    if (i < n) atomicAdd(&h[min(cnt[i], kOrderBins - 1u)], 1u);
+    __syncthreads();
+    if (h[tid]) atomicAdd(&bins[tid], h[tid]);
+}
+
+__global__ __launch_bounds__(256) void tiles_order_kernel(const uint32_t *cnt, uint32_t n, uint32_t *bins,
+                                                          uint32_t *order) {
+    __shared__ uint32_t start[kOrderBins], local[kOrderBins], wsum[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6, i = blockIdx.x * 256u + tid;
+    // exclusive prefix of the counts' totals in descending count order:
+    // thread t holds count 255 - t
+    const uint32_t v = bins[kOrderBins - 1u - tid];
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t d = 1u; d < 64u; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63u) wsum[wave] = incl;
+    local[tid] = 0u;
+    __syncthreads();
+    uint32_t before = 0u;
+    for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+    start[kOrderBins - 1u - tid] = before + incl - v;
+    uint32_t bin = 0u, rank = 0u;
+    if (i < n) {
+        bin = min(cnt[i], kOrderBins - 1u);
+        rank = atomicAdd(&local[bin], 1u);
+    }
+    __syncthreads();
+    if (local[tid]) start[tid] += atomicAdd(&bins[kOrderBins + tid], local[tid]);
+    __syncthreads();
+    if (i < n) order[start[bin] + rank] = i;
 }
 
 // Point queries: one lane per point.
@@ -460,15 +524,27 @@ extern "C" hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm
 
 extern "C" hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam,
                                               float max_distance, uint32_t width, uint32_t height, uint2 *tiles,
-                                              rkf::TileBig *big, uint32_t *big_count, hipStream_t stream) {
+                                              uint32_t *cnt, rkf::TileBig *big, uint32_t *big_count,
+                                              hipStream_t stream) {
     const uint32_t n_tiles = ((width + 7u) / 8u) * ((height + 7u) / 8u);
-    hipLaunchKernelGGL(rkf::tiles_clear_kernel, dim3((n_tiles + 255u) / 256u), dim3(256), 0, stream, tiles, n_tiles,
-                       big_count);
+    // with counts, the 512 words before them are the order's bins
+    hipLaunchKernelGGL(rkf::tiles_clear_kernel, dim3((n_tiles + 255u) / 256u), dim3(256), 0, stream, tiles, cnt,
+                       cnt ? cnt - 2u * rkf::kOrderBins : nullptr, n_tiles, big_count);
     if (S->n_blocks) {
         hipLaunchKernelGGL(rkf::tiles_bin_kernel, dim3((S->n_blocks + 255u) / 256u), dim3(256), 0, stream, *S, *cam,
-                           max_distance, width, height, tiles, big, big_count);
+                           max_distance, width, height, tiles, cnt, big, big_count);
         hipLaunchKernelGGL(rkf::tiles_big_kernel, dim3(512), dim3(256), 0, stream, (const rkf::TileBig *)big,
-                           (const uint32_t *)big_count, (width + 7u) / 8u, tiles);
+                           (const uint32_t *)big_count, (width + 7u) / 8u, tiles, cnt);
     }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_sparse_tile_order(uint32_t *cnt, uint32_t n_tiles, uint32_t *order,
+                                                   hipStream_t stream) {
+    uint32_t *bins = cnt - 2u * rkf::kOrderBins;
+    const dim3 grid((n_tiles + 255u) / 256u);
+    hipLaunchKernelGGL(rkf::tiles_hist_kernel, grid, dim3(256), 0, stream, (const uint32_t *)cnt, n_tiles, bins);
+    hipLaunchKernelGGL(rkf::tiles_order_kernel, grid, dim3(256), 0, stream, (const uint32_t *)cnt, n_tiles, bins,
+                       order);
     return hipGetLastError();
 }
