@@ -412,7 +412,11 @@ int rkfm_sparse_download(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
  * wavefront, hardware-scheduled; variant 1 = persistent wavefronts that
  * refill finished lanes; variant 2 = primary pass + compacted hit list +
  * shading/shadow pass (the context keeps a 36 B-per-pixel hit list). All
- * variants give the same results, bit for bit. */
+ * variants give the same results, bit for bit. With RKFM_MARCH_TILE_CULL the
+ * context keeps the per-tile intervals (8 B per 8x8 tile) and, for variant 0
+ * on frames of at most 65,536 tiles, a launch order of the tiles (most
+ * occupied cells first, 8 B per tile + 2 KiB); the order changes only which
+ * workgroup marches which tile, never a result. */
 int rkfm_sparse_march(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                       const rkf_camera_uniforms *camera,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext,
