@@ -459,6 +459,18 @@ int rkfm_sparse_tile_intervals(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
                                uint32_t height, float *tiles_dev,
                                void *hip_stream);
 
+/* No counterpart in the reference: the launch order rkfm_sparse_march uses
+ * for a culled view (variant 0, at most 65,536 tiles). counts_dev receives,
+ * per tile (row-major as above), the number of L2 cells binned into it;
+ * order_dev a permutation of the tile indices by descending
+ * min(count, 255) (within one count, any order). Frames above 65,536 tiles:
+ * RKFM_ERR_OUT_OF_RANGE. Requires |jitter| <= 0.5. */
+int rkfm_sparse_tile_order(rkfm_ctx *ctx, const rkfm_sparse_scene *scene,
+                           const rkf_camera_uniforms *camera,
+                           const rkf_scene_uniforms *su, uint32_t width,
+                           uint32_t height, uint32_t *counts_dev,
+                           uint32_t *order_dev, void *hip_stream);
+
 /* Point queries (SdfQueryable semantics, sdf_collision.rs:38-56; result
  * fields as SpatialQueryResult automation/types.rs:151-158 plus gradient). */
 typedef struct rkfm_point_result {

@@ -272,6 +272,10 @@ void orc_sparse_query(const orc_sparse_scene *s, const float *xyz, uint64_t n,
 void orc_sparse_tiles(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, uint32_t width, uint32_t height,
                       int prec, double *out);
+/* Per 8x8 tile, the number of L2 cells (holding an L1 block) binned into it
+ * by the FP32 tile binning: the key of the sparse kernel's tile launch order. */
+void orc_sparse_tile_counts(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
+                            const rkf_scene_uniforms *su, uint32_t width, uint32_t height, uint32_t *counts);
 void orc_sparse_shadow(const orc_sparse_scene *s, const float *origins, uint64_t n, const float L[3],
                        float max_dist, float k, int prec, float *visibility);
 /* Sphere casts as rkfm_sparse_sphere_cast (prec 32: FP32 twin, 64: FP64). */

@@ -358,6 +358,16 @@ def sparse_trace_parallel(sp: SparseInputs, cam, su, ext, width, height, p0=0, p
     return out, used
 
 
+def sparse_tile_counts(sp: SparseInputs, cam, su, width, height):
+    """Cells binned into each 8x8 tile (FP32 binning), shape (tiles_y, tiles_x)."""
+    tx, ty = (width + 7) // 8, (height + 7) // 8
+    out = np.zeros((ty, tx), np.uint32)
+    cam = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+    su = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+    lib().orc_sparse_tile_counts(C.byref(sp.s), _p(cam), _p(su), C.c_uint32(width), C.c_uint32(height), _p(out))
+    return out
+
+
 def sparse_tiles(sp: SparseInputs, cam, su, width, height, prec=32):
     """(t_lo, t_hi) per 8x8 tile, shape (tiles_y, tiles_x, 2); prec 32 as float32."""
     tx, ty = (width + 7) // 8, (height + 7) // 8

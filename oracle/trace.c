@@ -576,12 +576,19 @@ void orc_sparse_tiles(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
                       uint32_t w, uint32_t h, int prec, double *out) {
     uint64_t n = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u;
     if (prec == 64) {
-        sp_tiles_f64(s, cam, (double)su->max_distance, w, h, out);
+        sp_tiles_f64(s, cam, (double)su->max_distance, w, h, out, NULL);
         return;
     }
     float *t = (float *)malloc(n * sizeof(float));
-    sp_tiles_f32(s, cam, su->max_distance, w, h, t);
+    sp_tiles_f32(s, cam, su->max_distance, w, h, t, NULL);
     for (uint64_t i = 0; i < n; i++) out[i] = (double)t[i];
+    free(t);
+}
+
+void orc_sparse_tile_counts(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
+                            const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint32_t *counts) {
+    float *t = (float *)malloc((uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u * sizeof(float));
+    sp_tiles_f32(s, cam, su->max_distance, w, h, t, counts);
     free(t);
 }
 
@@ -596,10 +603,10 @@ void orc_sparse_trace(const orc_sparse_scene *s, const rkf_camera_uniforms *cam,
     if (tile_cull(ext)) {
         if (prec == 64) {
             t64 = (double *)malloc(nt * sizeof(double));
-            sp_tiles_f64(s, cam, (double)su->max_distance, w, h, t64);
+            sp_tiles_f64(s, cam, (double)su->max_distance, w, h, t64, NULL);
         } else {
             t32 = (float *)malloc(nt * sizeof(float));
-            sp_tiles_f32(s, cam, su->max_distance, w, h, t32);
+            sp_tiles_f32(s, cam, su->max_distance, w, h, t32, NULL);
         }
     }
     for (uint64_t p = p0; p < p1; p++) {
@@ -617,7 +624,7 @@ int orc_sparse_trace_parallel(const orc_sparse_scene *s, const rkf_camera_unifor
     float *t32 = NULL;
     if (tile_cull(ext)) {
         t32 = (float *)malloc((uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u) * 2u * sizeof(float));
-        sp_tiles_f32(s, cam, su->max_distance, w, h, t32);
+        sp_tiles_f32(s, cam, su->max_distance, w, h, t32, NULL);
     }
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);

@@ -782,13 +782,16 @@ static V3 F(cross3)(V3 a, V3 b) {
     return F(mk)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-/* iv: 2 values (lo, hi) per tile, row-major over ceil(w/8) x ceil(h/8). */
+/* iv: 2 values (lo, hi) per tile, row-major over ceil(w/8) x ceil(h/8);
+ * cnt (or NULL): per tile, the number of cells binned into it (the kernel's
+ * launch-order key). */
 static void F(sp_tiles)(const orc_sparse_scene *s, const rkf_camera_uniforms *cam, R maxd, uint32_t w, uint32_t h,
-                        R *iv) {
+                        R *iv, uint32_t *cnt) {
     uint32_t tx = (w + 7u) / 8u, ty = (h + 7u) / 8u;
     for (uint64_t i = 0; i < (uint64_t)tx * ty; i++) {
         iv[2 * i] = (R)INFINITY;
         iv[2 * i + 1] = 0;
+        if (cnt) cnt[i] = 0;
     }
     V3 f = F(mk)((R)cam->forward[0], (R)cam->forward[1], (R)cam->forward[2]);
     V3 r = F(mk)((R)cam->right[0], (R)cam->right[1], (R)cam->right[2]);
@@ -813,6 +816,7 @@ static void F(sp_tiles)(const orc_sparse_scene *s, const rkf_camera_uniforms *ca
                 R *t = iv + 2 * ((uint64_t)y * tx + x);
                 t[0] = FMIN(t[0], tn);
                 t[1] = FMAX(t[1], tf);
+                if (cnt) cnt[(uint64_t)y * tx + x]++;
             }
     }
 }

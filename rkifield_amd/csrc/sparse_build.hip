@@ -1358,6 +1358,45 @@ int rkfm_sparse_tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rk
     return tile_intervals(c, s, cam, su, w, h, (uint2 *)tiles, nullptr, stream ? (hipStream_t)stream : c->stream);
 }
 
+// The tile counts and launch order of a culled view into the context's
+// buffers (tile_iv, tile_cnt with its 512 sort words, tile_order).
+static int tile_order(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                      const rkf_scene_uniforms *su, uint32_t w, uint32_t h, hipStream_t st) {
+    const uint64_t n_tiles = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u);
+    int rc = c->tile_iv.reserve(n_tiles * 8u, false);
+    // 512 bin words of the order's counting sort, then one count per tile
+    if (!rc) rc = c->tile_cnt.reserve((n_tiles + 512u) * 4u, false);
+    if (!rc) rc = c->tile_order.reserve(n_tiles * 4u, false);
+    if (!rc) rc = tile_intervals(c, s, cam, su, w, h, (uint2 *)c->tile_iv.ptr, (uint32_t *)c->tile_cnt.ptr + 512u, st);
+    if (!rc)
+        rc = rkf_hip_check(rkf_launch_sparse_tile_order((uint32_t *)c->tile_cnt.ptr + 512u, (uint32_t)n_tiles,
+                                                        (uint32_t *)c->tile_order.ptr, st),
+                           "tile order launch");
+    return rc;
+}
+
+int rkfm_sparse_tile_order(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
+                           const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint32_t *counts, uint32_t *order,
+                           void *stream) {
+    if (!c || !s || !cam || !su || !counts || !order)
+        return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile order: null argument");
+    if (w == 0 || h == 0 || w > 32768u || h > 32768u) return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile order: %ux%u", w, h);
+    const uint64_t n_tiles = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u);
+    if (n_tiles > kOrderMaxTiles)
+        return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "tile order: %llu tiles (max %llu)", (unsigned long long)n_tiles,
+                             (unsigned long long)kOrderMaxTiles);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = tile_order(c, s, cam, su, w, h, st);
+    if (!rc)
+        rc = rkf_hip_check(hipMemcpyAsync(counts, (uint32_t *)c->tile_cnt.ptr + 512u, n_tiles * 4u,
+                                          hipMemcpyDeviceToDevice, st),
+                           "tile order counts copy");
+    if (!rc)
+        rc = rkf_hip_check(hipMemcpyAsync(order, c->tile_order.ptr, n_tiles * 4u, hipMemcpyDeviceToDevice, st),
+                           "tile order copy");
+    return rc;
+}
+
 int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, const rkfm_scene_ext *ext, const rkfm_gbuffer *out,
                       uint32_t variant, void *stream) {
@@ -1387,17 +1426,12 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
         // keeps neighbouring tiles' bricks in L2 together (2160p, 129,600
         // tiles: 1.69 ms row-major, 1.73 ms ordered).
         const bool order = variant == 0 && n_tiles <= kOrderMaxTiles;
-        rc = c->tile_iv.reserve(n_tiles * 8u, false);
-        // 512 bin words of the order's counting sort, then one count per tile
-        if (!rc && order) rc = c->tile_cnt.reserve((n_tiles + 512u) * 4u, false);
-        if (!rc && order) rc = c->tile_order.reserve(n_tiles * 4u, false);
-        if (!rc)
-            rc = tile_intervals(c, s, cam, su, out->width, out->height, (uint2 *)c->tile_iv.ptr,
-                                order ? (uint32_t *)c->tile_cnt.ptr + 512u : nullptr, st);
-        if (!rc && order)
-            rc = rkf_hip_check(rkf_launch_sparse_tile_order((uint32_t *)c->tile_cnt.ptr + 512u, (uint32_t)n_tiles,
-                                                            (uint32_t *)c->tile_order.ptr, st),
-                               "tile order launch");
+        if (order) {
+            rc = tile_order(c, s, cam, su, out->width, out->height, st);
+        } else {
+            rc = c->tile_iv.reserve(n_tiles * 8u, false);
+            if (!rc) rc = tile_intervals(c, s, cam, su, out->width, out->height, (uint2 *)c->tile_iv.ptr, nullptr, st);
+        }
         if (rc) return rc;
         P.tiles = (const uint2 *)c->tile_iv.ptr;
         P.tiles_x = tx;

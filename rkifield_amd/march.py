@@ -80,6 +80,7 @@ _SIGS = {
     "rkfm_sparse_query_points": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_sphere_cast": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
     "rkfm_sparse_tile_intervals": (C.c_int, [_VP, _VP, _VP, _VP, _U32, _U32, _VP, _VP]),
+    "rkfm_sparse_tile_order": (C.c_int, [_VP, _VP, _VP, _VP, _U32, _U32, _VP, _VP, _VP]),
     "rkfm_sparse_edit": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _U64, _VP, _VP]),
     "rkfm_sparse_contacts": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
     "rkfm_rkf_read_header": (C.c_int, [_VP, _U64, _VP, _VP, _U32]),
@@ -566,6 +567,19 @@ class SparseScene:
         _check(lib().rkfm_sparse_tile_intervals(self.ctx.h, self.h, _p(c), _p(s), width, height, d.ptr, None))
         self.ctx.sync()
         return d.download()
+
+    def tile_order(self, cam, su, width, height):
+        """(counts, order) of a culled view (rkfm_sparse_tile_order): cells
+        binned into each 8x8 tile, shape (tiles_y, tiles_x), and the tiles'
+        launch order (flat tile indices)."""
+        c = np.ascontiguousarray(cam, L.CAMERA_UNIFORMS)
+        s = np.ascontiguousarray(su, L.SCENE_UNIFORMS)
+        tx, ty = (width + 7) // 8, (height + 7) // 8
+        cd = self.ctx.array((ty, tx), np.uint32)
+        od = self.ctx.array((tx * ty,), np.uint32)
+        _check(lib().rkfm_sparse_tile_order(self.ctx.h, self.h, _p(c), _p(s), width, height, cd.ptr, od.ptr, None))
+        self.ctx.sync()
+        return cd.download(), od.download()
 
     def edit(self, ops, report=True, stream=None):
         """Applies CSG edits in order (rkfm_sparse_edit); with report=True

@@ -240,6 +240,28 @@ def test_sparse_march_ragged_view(ctx, small_scene, cull):
     assert np.array_equal(gpu["shadow"].view(np.uint32), res32["shadow"].astype(np.float32).view(np.uint32))
 
 
+def test_tile_order_matches_oracle(ctx, small_scene):
+    """rkfm_sparse_tile_order (the culled march's launch order): per-tile
+    cell counts equal the oracle's FP32 binning, the order is a permutation
+    of the tiles by descending min(count, 255); frames above 65,536 tiles are
+    refused."""
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    su = L.scene_uniforms(num_objects=1)
+    for eye, w, h in (((6.0, 3.0, 7.0), 96, 64), ((5.5, 2.5, -6.5), 61, 45)):
+        _say(f"case: tile order {w}x{h}")
+        cam = M.camera_look_at(eye, (0, 0, 0), 60.0, w, h)
+        counts, order = sc.tile_order(cam, su, w, h)
+        assert np.array_equal(counts, O.sparse_tile_counts(sp, cam, su, w, h))
+        assert len(np.unique(counts)) > 3
+        assert np.array_equal(np.sort(order), np.arange(counts.size))
+        key = np.minimum(counts.ravel()[order], 255).astype(np.int64)
+        assert np.all(np.diff(key) <= 0)
+    cam = M.camera_look_at((6.0, 3.0, 7.0), (0, 0, 0), 60.0, 4096, 4096)
+    with pytest.raises(M.RkfMarchError):
+        sc.tile_order(cam, su, 4096, 4096)
+
+
 def test_point_queries_parity(ctx, small_scene):
     prims, sc, lo = small_scene
     sp, _ = _oracle_inputs(sc)

@@ -67,6 +67,29 @@ def test_config1_full_frame(ctx, cfg1):
     assert (ang <= NORMAL_RAD_TOL).mean() >= 0.999 and ang.max() <= 1e-2
 
 
+def test_config1_tile_order_capped_counts(ctx, cfg1):
+    """A 64x48 view of the 4096-primitive scene puts hundreds of cells in a
+    tile, so the launch order's top count bin (255 and more) holds many
+    tiles: counts equal the oracle's, the order is a permutation by
+    descending min(count, 255), and the march in that order is bit-exact
+    against the FP32 twin."""
+    sc, si = cfg1
+    W, H = 64, 48
+    cam, su, ext = scenes.config1_view(W, H)
+    say(f"case: config1 tile order {W}x{H}")
+    counts, order = sc.tile_order(cam, su, W, H)
+    assert np.array_equal(counts, O.sparse_tile_counts(si, cam, su, W, H))
+    say(f"counts: max {int(counts.max())}, {int((counts >= 255).sum())} of {counts.size} tiles at 255 or more")
+    assert (counts > 255).sum() >= 2
+    assert np.array_equal(np.sort(order), np.arange(counts.size))
+    assert np.all(np.diff(np.minimum(counts.ravel()[order], 255).astype(np.int64)) <= 0)
+    gb = M.GBuffer(ctx, W, H, debug=True, shadow=True)
+    sc.march(cam, su, ext, gb)
+    gpu = gb.read()
+    res, _ = O.sparse_trace_parallel(si, cam, su, ext, W, H)
+    assert compare_sparse(gpu, res) > 100
+
+
 def test_config1_sphere_casts_match_oracle(ctx, cfg1):
     """64k sphere casts through the 256 m scene (radii U[0.02, 1] m, from
     points U in the scene box, up to 64 m): bit-exact against the FP32 twin."""
