@@ -163,9 +163,17 @@ int rkfm_rkf_decode_lod(const void *file, uint64_t len, uint32_t lod, uint32_t *
         return rkf_set_error(RKFM_ERR_FORMAT, "rkf: LOD %u data block outside the file", lod);
     const uint8_t *blk = f + e.data_offset;
     const uint64_t usz = rd32(blk);
-    const uint64_t nmap = (uint64_t)e.brick_dims[0] * e.brick_dims[1] * e.brick_dims[2];
-    const uint64_t need = nmap * 4 + (uint64_t)e.brick_count * 512 * sizeof(rkf_voxel_sample);
-    if (usz < need)
+    // An LZ4 sequence expands at most 255x (a 1-byte match-length code, a
+    // 255-byte extension), so a larger size prefix is corrupt, not data; the
+    // bound keeps a damaged file from sizing a multi-GiB buffer.
+    if (usz > (uint64_t)(e.compressed_size - 4) * 255 + 16)
+        return rkf_set_error(RKFM_ERR_FORMAT, "rkf: LOD %u: size prefix %llu exceeds what %u bytes can hold", lod,
+                             (unsigned long long)usz, e.compressed_size - 4);
+    uint64_t nmap = 0, need = 0;
+    const bool wraps = __builtin_mul_overflow((uint64_t)e.brick_dims[0] * e.brick_dims[1], (uint64_t)e.brick_dims[2], &nmap) ||
+                       __builtin_mul_overflow(nmap, (uint64_t)4, &need) ||
+                       __builtin_add_overflow(need, (uint64_t)e.brick_count * 512 * sizeof(rkf_voxel_sample), &need);
+    if (wraps || usz < need)
         return rkf_set_error(RKFM_ERR_FORMAT, "rkf: decompressed data is %llu bytes, expected at least %llu",
                              (unsigned long long)usz, (unsigned long long)need);
     std::vector<uint8_t> raw(usz);
@@ -251,6 +259,9 @@ int rkfm_sparse_from_rkf(rkfm_ctx *c, const void *file, uint64_t len, uint32_t l
     if (lod >= h.lod_count)
         return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "rkf: LOD index %u out of range (max %u)", lod, h.lod_count);
     const rkfm_rkf_lod &e = lods[lod];
+    // validate first: a damaged LOD table must not size the vectors below
+    rc = rkfm_rkf_decode_lod(file, len, lod, nullptr, nullptr);
+    if (rc) return rc;
     std::vector<uint32_t> map((uint64_t)e.brick_dims[0] * e.brick_dims[1] * e.brick_dims[2]);
     std::vector<rkf_voxel_sample> bricks((uint64_t)e.brick_count * 512);
     rc = rkfm_rkf_decode_lod(file, len, lod, map.data(), bricks.data());

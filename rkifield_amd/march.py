@@ -913,10 +913,15 @@ def rkf_decode_lod(data, lod=0):
         _check(lib().rkfm_rkf_decode_lod(buf.ctypes.data, buf.size, lod, m.ctypes.data, None))
     e = lods[lod]
     dims = np.array(e["brick_dims"], np.uint32)
-    m = np.zeros(max(int(np.prod(dims)), 1), np.uint32)
-    b = np.zeros(max(int(e["brick_count"]) * 512, 1), L.VOXEL_SAMPLE)
+    # validate (decode into nothing) before sizing arrays from the LOD table,
+    # whose fields a damaged file can set to anything
+    _check(lib().rkfm_rkf_decode_lod(buf.ctypes.data, buf.size, lod, None, None))
+    nmap = int(dims[0]) * int(dims[1]) * int(dims[2])
+    nb = int(e["brick_count"]) * 512
+    m = np.zeros(max(nmap, 1), np.uint32)
+    b = np.zeros(max(nb, 1), L.VOXEL_SAMPLE)
     _check(lib().rkfm_rkf_decode_lod(buf.ctypes.data, buf.size, lod, m.ctypes.data, b.ctypes.data))
-    return dims, m[: int(np.prod(dims))], b[: int(e["brick_count"]) * 512]
+    return dims, m[:nmap], b[:nb]
 
 
 def rkf_encode(levels, aabb_min=(0, 0, 0), aabb_max=(1, 1, 1), material_ids=(), analytical=(0, (0, 0, 0, 0))):

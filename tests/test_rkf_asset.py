@@ -133,3 +133,38 @@ def test_lz4_cross_decodes_with_the_oracle_and_rejects_corruption():
             assert err.code == M.ERR["FORMAT"]
             bad += 1
     assert bad > 0
+
+
+def test_truncated_and_overwritten_files_fail_cleanly():
+    """A file cut at any length is refused with a format error, and random
+    overwrites of the header and LOD table either decode to arrays of the
+    stated sizes or are refused: the reader never reads past the buffer."""
+    rng = np.random.default_rng(3)
+    lvl = (0.02, (3, 1, 2), np.array([0, L.EMPTY_SLOT, 1, 2, L.EMPTY_SLOT, 3], np.uint32),
+           np.concatenate([_vs(0.5, 1), _vs(-0.5, 2), _vs(1.5, 3), _vs(0.25, 4)]))
+    out = M.rkf_encode([lvl, _small_lod(0.04)])
+
+    def read_all(f):
+        M.rkf_read_header(f)
+        for lod in range(2):
+            M.rkf_decode_lod(f, lod)
+
+    read_all(out)
+    for n in range(len(out)):
+        with pytest.raises(M.RkfMarchError) as e:
+            read_all(out[:n])
+        assert e.value.code == M.ERR["FORMAT"], n
+    table_end = 128 + 40 * 2  # 128-byte header, one 40-byte entry per LOD
+    refused = 0
+    for k in range(300):
+        c = bytearray(out)
+        for _ in range(1 + k % 3):
+            c[int(rng.integers(0, table_end))] = int(rng.integers(0, 256))
+        try:
+            for lod in range(2):
+                dims, m, b = M.rkf_decode_lod(bytes(c), lod)
+                assert m.size == int(np.prod(dims)) and b.size % 512 == 0
+        except M.RkfMarchError as err:
+            assert err.code in (M.ERR["FORMAT"], M.ERR["OUT_OF_RANGE"])
+            refused += 1
+    assert refused > 0
