@@ -217,7 +217,10 @@ int rkfm_sync(rkfm_ctx *ctx);
 void *rkfm_stream(rkfm_ctx *ctx);
 
 /* GpuScene uploads (render.rs:106-137, gpu_scene.rs:141-195). The device
- * buffers grow as needed; first_slot / offset address the destination. */
+ * buffers grow as needed; first_slot / offset address the destination.
+ * Voxels and map entries are indexed in 32 bits, as the reference's u32
+ * indices are: RKFM_ERR_OUT_OF_RANGE past slot 2^23 - 2 (a 32 GiB pool) or
+ * map entry 2^32 - 2. */
 int rkfm_upload_brick_pool(rkfm_ctx *ctx, const rkf_voxel_sample *bricks,
                            uint64_t n_bricks, uint64_t first_slot);
 int rkfm_upload_brick_maps(rkfm_ctx *ctx, const uint32_t *entries,

@@ -52,6 +52,22 @@ RKF_DEV float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), h
 RKF_DEV int32_t clampi(int32_t x, int32_t lo, int32_t hi) { return x < lo ? lo : (x > hi ? hi : x); }
 // WGSL mix: e1 * (1 - e3) + e2 * e3.
 RKF_DEV float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// DPP lane exchanges inside aligned 8-lane groups: xor 1 and xor 2 (quad
+// permutations) and the half-row mirror (lane i <-> 7 - i), which equals
+// xor 4 once lanes 0-3 and lanes 4-7 each hold one value. Every lane of the
+// group must be active.
+template <int kCtrl> RKF_DEV uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xF, 0xF, false);
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141;
+RKF_DEV float dpp_xor1(float v) { return __uint_as_float(dpp_u32<kDppXor1>(__float_as_uint(v))); }
+RKF_DEV float dpp_xor2(float v) { return __uint_as_float(dpp_u32<kDppXor2>(__float_as_uint(v))); }
+RKF_DEV float dpp_half_mirror(float v) { return __uint_as_float(dpp_u32<kDppHalfMirror>(__float_as_uint(v))); }
+// Value of v in lane src of the wavefront (src must be active).
+RKF_DEV uint32_t lane_read(uint32_t v, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
 RKF_DEV f3 normalize(f3 a) {
     float l = length(a);
     return f3{a.x / l, a.y / l, a.z / l};

@@ -157,8 +157,15 @@ static int upload(rkfm_ctx *c, DevBuf &b, const void *src, uint64_t bytes, uint6
     return RKFM_OK;
 }
 
+constexpr uint64_t kMaxPoolSlots = (1ull << 23) - 1ull;
+
 int rkfm_upload_brick_pool(rkfm_ctx *c, const rkf_voxel_sample *bricks, uint64_t n_bricks, uint64_t first_slot) {
     if (!c || (!bricks && n_bricks)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "upload_brick_pool: null argument");
+    // The drop-in kernel indexes voxels in 32 bits: at most 2^23 - 1 slots
+    // (a 32 GiB pool; the reference's u32 indices address less).
+    if (first_slot > kMaxPoolSlots || n_bricks > kMaxPoolSlots - first_slot)
+        return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "upload_brick_pool: slots past %llu",
+                             (unsigned long long)kMaxPoolSlots);
     const uint64_t bb = 512ull * sizeof(rkf_voxel_sample);
     int rc = upload(c, c->pool, bricks, n_bricks * bb, first_slot * bb, true);
     if (rc) return rc;
@@ -168,6 +175,9 @@ int rkfm_upload_brick_pool(rkfm_ctx *c, const rkf_voxel_sample *bricks, uint64_t
 
 int rkfm_upload_brick_maps(rkfm_ctx *c, const uint32_t *e, uint64_t n, uint64_t offset) {
     if (!c || (!e && n)) return rkf_set_error(RKFM_ERR_INVALID_ARG, "upload_brick_maps: null argument");
+    // map entries are indexed in 32 bits (brick_map_offset is a u32)
+    if (offset > 0xFFFFFFFFull || n > 0xFFFFFFFFull - offset)
+        return rkf_set_error(RKFM_ERR_OUT_OF_RANGE, "upload_brick_maps: entries past 2^32 - 1");
     int rc = upload(c, c->maps, e, n * 4, offset * 4, true);
     if (rc) return rc;
     c->n_maps = c->maps.size / 4;

@@ -20,8 +20,9 @@
 
 namespace rkf {
 
-// ray_march.wgsl:228-235, software trilinear.
-RKF_DEV float coarse(const RefParams &P, f3 cr) {
+// ray_march.wgsl:228-235, software trilinear. L: lanes per ray (8: the
+// ray's lanes share the corner loads; k is the lane's index in the ray).
+template <uint32_t L> RKF_DEV float coarse(const RefParams &P, f3 cr, uint32_t k) {
     if (!P.coarse) return 0.0f;
     const rkf_coarse_field_uniforms &u = P.cu;
     f3 fp = sub(cr, mk(u.origin_cam_rel[0], u.origin_cam_rel[1], u.origin_cam_rel[2]));
@@ -43,6 +44,20 @@ RKF_DEV float coarse(const RefParams &P, f3 cr) {
     auto cf = [&](int32_t x, int32_t y, int32_t z) {
         return h2f(P.coarse[(uint64_t)x + (uint64_t)y * dx + (uint64_t)z * dxy]);
     };
+    if (L == 8u) {
+        // The ray's 8 lanes load one corner each (lane k: corner (k & 1,
+        // k >> 1 & 1, k >> 2)) and reduce the trilinear mix across the
+        // group, each level's mix with the sequential code's operand order,
+        // so every lane ends with the sequential value.
+        const uint32_t kx = k & 1u, ky = (k >> 1) & 1u, kz = k >> 2;
+        const float c = cf(kx ? i1[0] : i0[0], ky ? i1[1] : i0[1], kz ? i1[2] : i0[2]);
+        const float p1 = dpp_xor1(c);
+        const float c1 = kx ? mix(p1, c, f[0]) : mix(c, p1, f[0]);
+        const float p2 = dpp_xor2(c1);
+        const float c2 = ky ? mix(p2, c1, f[1]) : mix(c1, p2, f[1]);
+        const float p3 = dpp_half_mirror(c2);
+        return kz ? mix(p3, c2, f[2]) : mix(c2, p3, f[2]);
+    }
     float c00 = mix(cf(i0[0], i0[1], i0[2]), cf(i1[0], i0[1], i0[2]), f[0]);
     float c10 = mix(cf(i0[0], i1[1], i0[2]), cf(i1[0], i1[1], i0[2]), f[0]);
     float c01 = mix(cf(i0[0], i0[1], i1[2]), cf(i1[0], i0[1], i1[2]), f[0]);
@@ -56,9 +71,68 @@ struct MarchRes {
     uint32_t mat, obj_id, obj_idx, steps;
 };
 
+// ray_march.wgsl:944-962: kBisect halvings of [lo, hi], each keeping the
+// half whose start is outside. With 8 lanes per ray (k: the lane's index in
+// the ray, from lane `base`) up to three halvings are one round: the 7
+// midpoints the next three halvings can take (a binary tree: node n's
+// halves are nodes 2n + 1, [lo, mid], and 2n + 2, [mid, hi]) are sampled at
+// once, lane k taking node k, each midpoint computed by the additions and
+// halvings the one-lane loop makes on that path; the signs then pick the
+// path. Evaluations count only the path's samples, as in the loop.
+template <uint32_t L>
 RKF_DEV void bisect(RefCtx &c, f3 o, f3 sd, float lo, float hi, const rkf_gpu_object &obj, uint32_t oi,
-                    uint32_t best_mat, uint32_t best_id, MarchRes &r) {
+                    uint32_t best_mat, uint32_t best_id, MarchRes &r, uint32_t k, uint32_t base) {
     uint32_t bm = best_mat;
+    if (L == 8u) {
+        for (uint32_t left = kBisect; left > 0u;) {
+            const uint32_t depth = left < 3u ? left : 3u, nodes = (1u << depth) - 1u;
+            // this lane's node: the halvings from the root (bit i set: step i keeps [mid, hi])
+            uint32_t dirs = 0, dk = 0;
+            for (uint32_t n = k; n > 0u && n < nodes; n = (n - 1u) >> 1) {
+                dirs = (dirs << 1) | ((n & 1u) ? 0u : 1u);
+                dk++;
+            }
+            float l = lo, h = hi;
+            for (uint32_t i = 0; i < dk; i++) {
+                const float mid = (l + h) * 0.5f;
+                if ((dirs >> i) & 1u) l = mid;
+                else h = mid;
+            }
+            bool neg = false, counted = false;
+            uint32_t mm = 0;
+            if (k < nodes) {
+                const float mid = (l + h) * 0.5f;
+                const uint32_t e0 = c.evals;
+                neg = eval_obj(c, add(o, muls(sd, mid)), obj, oi, mm) < 0.0f;
+                counted = c.evals != e0;
+                c.evals = e0;
+            }
+            const uint32_t negs = (uint32_t)(__ballot(neg) >> base) & 0xFFu;
+            const uint32_t cnts = (uint32_t)(__ballot(counted) >> base) & 0xFFu;
+            uint32_t n = 0, last_kept = nodes;
+            for (uint32_t i = 0; i < depth; i++) {
+                const float mid = (lo + hi) * 0.5f;
+                c.evals += (cnts >> n) & 1u;
+                if ((negs >> n) & 1u) {
+                    hi = mid;
+                    n = 2u * n + 1u;
+                } else {
+                    lo = mid;
+                    last_kept = n;
+                    n = 2u * n + 2u;
+                }
+            }
+            const uint32_t km = lane_read(mm, base + (last_kept < nodes ? last_kept : 0u));
+            if (last_kept < nodes) bm = km;
+            left -= depth;
+        }
+        r.t = (lo + hi) * 0.5f;
+        r.hit = true;
+        r.mat = bm;
+        r.obj_id = best_id;
+        r.obj_idx = oi;
+        return;
+    }
     for (uint32_t b = 0; b < kBisect; b++) {
         float mid = (lo + hi) * 0.5f;
         f3 mp = add(o, muls(sd, mid));
@@ -85,7 +159,27 @@ constexpr uint32_t kNoObj = 0xFFFFFFFFu;
 // Combined, every lane holds the first object attaining the minimum over
 // all of them, the object ray_march.wgsl:929-938 keeps (strictly smaller
 // value, or the same value at a lower index).
+// Each exchange is symmetric (distances are finite or kMaxFloat, object
+// indices distinct unless both kNoObj), so after the xor-1 and xor-2 levels
+// lanes 0-3 and 4-7 each hold one value and the half-row mirror serves as
+// the xor-4 level: for 8 lanes the exchanges are DPP moves, not LDS round
+// trips.
 template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j) {
+    if (L == 8u) {
+        auto level = [&](auto xch) {
+            const float pd = __uint_as_float(xch(__float_as_uint(d)));
+            const uint32_t pm = xch(m), pj = xch(j);
+            if (pd < d || (pd == d && pj < j)) {
+                d = pd;
+                m = pm;
+                j = pj;
+            }
+        };
+        level([](uint32_t v) { return dpp_u32<kDppXor1>(v); });
+        level([](uint32_t v) { return dpp_u32<kDppXor2>(v); });
+        level([](uint32_t v) { return dpp_u32<kDppHalfMirror>(v); });
+        return;
+    }
 #pragma unroll
     for (uint32_t off = 1; off < L; off <<= 1) {
         const float pd = __shfl_xor(d, (int)off);
@@ -127,7 +221,7 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
         r.steps++;
         f3 pos = add(o, muls(sd, t));
         if (MODE == 0) {
-            float cd = coarse(P, sub(pos, cam));
+            float cd = coarse<L>(P, sub(pos, cam), share);
             if (cd > kCoarseNear) {
                 prev_dist = kMaxFloat;
                 t += cd;
@@ -210,7 +304,8 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
             const rkf_gpu_object &bo =
                 MODE == 0 ? tobj[best_j] : P.objects[best_idx < P.su.num_objects ? best_idx : 0u];
             const uint32_t e0 = c.evals;
-            bisect(c, o, sd, prev_t, t, bo, best_idx, best_mat, best_id, r);
+            // the workgroup is one 64-lane wavefront: threadIdx.x is the lane
+            bisect<L>(c, o, sd, prev_t, t, bo, best_idx, best_mat, best_id, r, share, threadIdx.x - share);
             bis = c.evals - e0;
             return;
         }

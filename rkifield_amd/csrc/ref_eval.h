@@ -57,13 +57,17 @@ RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool g
     const int32_t X[2] = {clampi(x0, 0, tot[0] - 1), clampi(x0 + 1, 0, tot[0] - 1)};
     const int32_t Y[2] = {clampi(y0, 0, tot[1] - 1), clampi(y0 + 1, 0, tot[1] - 1)};
     const int32_t Z[2] = {clampi(z0, 0, tot[2] - 1), clampi(z0 + 1, 0, tot[2] - 1)};
+    // Indices in 32 bits: the uploads hold n_maps and n_pool_voxels below
+    // 2^32, so an offset + flat sum that carries, or a slot of 2^23 or more,
+    // lies past its buffer exactly when the 64-bit index would.
+    const uint32_t n_maps = (uint32_t)P.n_maps, n_pool = (uint32_t)P.n_pool_voxels;
     uint32_t ent[8], raw[8];
     bool inm[8], ok[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8u; k++) {
         const uint32_t cx = (uint32_t)X[k & 1u], cy = (uint32_t)Y[(k >> 1) & 1u], cz = (uint32_t)Z[k >> 2];
-        const uint64_t i = (uint64_t)off + ((cx >> 3) + (cy >> 3) * dims[0] + (cz >> 3) * dims[0] * dims[1]);
-        inm[k] = i < P.n_maps;
+        const uint32_t i = off + ((cx >> 3) + (cy >> 3) * dims[0] + (cz >> 3) * dims[0] * dims[1]);
+        inm[k] = i >= off && i < n_maps;
         ent[k] = P.maps[inm[k] ? i : 0u];
     }
 #pragma unroll
@@ -74,8 +78,8 @@ RKF_DEV float sample_voxelized(RefCtx &c, f3 lp, const rkf_gpu_object &o, bool g
 #pragma unroll
     for (uint32_t k = 0; k < 8u; k++) {
         const uint32_t cx = (uint32_t)X[k & 1u], cy = (uint32_t)Y[(k >> 1) & 1u], cz = (uint32_t)Z[k >> 2];
-        const uint64_t idx = (uint64_t)ent[k] * 512u + (cx & 7u) + (cy & 7u) * 8u + (cz & 7u) * 64u;
-        ok[k] = ent[k] < RKF_INTERIOR_SLOT && idx < P.n_pool_voxels;
+        const uint32_t idx = (ent[k] << 9) | ((cx & 7u) + (cy & 7u) * 8u + (cz & 7u) * 64u);
+        ok[k] = ent[k] < (1u << 23) && idx < n_pool;
         raw[k] = P.pool[ok[k] ? idx : 0u].word0;
     }
     float cv[8];

@@ -645,22 +645,6 @@ RKF_DEV float sp_sample_nb(SpCtx &c, const BrickNbhd &n, f3 p) {
 // neighbour cell (k & 1, k >> 1 & 1, k >> 2) of nb.cell in nb.nrow.
 constexpr uint32_t kGroupRays = 8u;
 
-// DPP lane exchanges inside 8-lane groups: xor 1 and xor 2 (quad
-// permutations) and the half-row mirror (lane i <-> 7 - i), which equals
-// xor 4 once lanes 0-3 and lanes 4-7 each hold one value.
-RKF_DEV float dpp_xor1(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
-}
-RKF_DEV float dpp_xor2(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
-}
-RKF_DEV float dpp_half_mirror(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
-}
-RKF_DEV uint32_t lane_read(uint32_t v, uint32_t src) {
-    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
-}
-
 // sp_locate for a group (lane k of the group whose first lane is `base`):
 // same decisions, same cache contents (spread as above).
 RKF_DEV int sp_locate_g(const SparseDev &S, f3 pos, f3 sd, f3 inv, float t, float &t_exit, BrickNbhd &n,

@@ -175,3 +175,22 @@ def test_empty_scene_and_errors(ctx):
     with pytest.raises(M.RkfMarchError) as e:
         M.RayMarchPass(ctx).dispatch(gs, gb2, tc, cf)  # tile grid built for 16x16
     assert e.value.code == M.ERR["LAYOUT"]
+
+
+def test_upload_index_limits(ctx):
+    """Voxels and map entries are indexed in 32 bits (the reference's u32
+    indices): uploads past slot 2^23 - 2 or map entry 2^32 - 2 are refused
+    before any device memory is touched; the last allowed ones are taken."""
+    _say("case: upload limits")
+    gs = M.GpuScene(ctx)
+    brick = np.zeros(512, L.VOXEL_SAMPLE)
+    for first in ((1 << 23) - 1, 1 << 23, 1 << 40):
+        with pytest.raises(M.RkfMarchError) as e:
+            gs.upload_brick_pool(brick, first_slot=first)
+        assert e.value.code == M.ERR["OUT_OF_RANGE"]
+    for off in ((1 << 32) - 1, 1 << 32):
+        with pytest.raises(M.RkfMarchError) as e:
+            gs.upload_brick_maps(np.zeros(1, np.uint32), offset=off)
+        assert e.value.code == M.ERR["OUT_OF_RANGE"]
+    gs.upload_brick_pool(brick, first_slot=0)
+    gs.upload_brick_maps(np.zeros(4, np.uint32), offset=0)
