@@ -124,6 +124,9 @@ int rkfm_destroy(rkfm_ctx *c) {
     c->tile_iv.release();
     c->tile_big.release();
     c->tile_big_count.release();
+    c->tile_cnt.release();
+    c->tile_order.release();
+    c->ref_order.release();
     c->edit_ops.release();
     c->edit_list.release();
     c->edit_mod.release();
@@ -222,8 +225,23 @@ int rkfm_set_scene(rkfm_ctx *c, const rkf_scene_uniforms *su, const rkfm_scene_e
     return RKFM_OK;
 }
 
+// The tiled march's launch order, derived from the tile lists wherever they
+// are made (tiles by descending object count; nothing else reads it).
+static int ref_order(rkfm_ctx *c, uint64_t nt) {
+    c->ref_order_n = 0;
+    int rc = c->ref_order.reserve(nt * 4, false);
+    if (rc) return rc;
+    rc = rkf_hip_check(rkf_launch_ref_tile_order((const uint32_t *)c->tile_counts.ptr, (uint32_t)nt,
+                                                 (uint32_t *)c->ref_order.ptr, c->stream),
+                       "tile order launch");
+    if (rc) return rc;
+    c->ref_order_n = nt;
+    return RKFM_OK;
+}
+
 int rkfm_set_tiles(rkfm_ctx *c, const uint32_t *idx, const uint32_t *counts, uint32_t tx, uint32_t ty) {
     if (!c) return rkf_set_error(RKFM_ERR_INVALID_ARG, "set_tiles: null ctx");
+    c->ref_order_n = 0;
     if (tx == 0 || ty == 0) {
         c->tiles_x = c->tiles_y = 0;
         return RKFM_OK;
@@ -237,7 +255,7 @@ int rkfm_set_tiles(rkfm_ctx *c, const uint32_t *idx, const uint32_t *counts, uin
     if (rc) return rc;
     c->tiles_x = tx;
     c->tiles_y = ty;
-    return RKFM_OK;
+    return ref_order(c, nt);
 }
 
 int rkfm_set_coarse_field(rkfm_ctx *c, const uint16_t *cells, const rkf_coarse_field_uniforms *u) {
@@ -266,6 +284,7 @@ int rkfm_build_tiles(rkfm_ctx *c, uint32_t w, uint32_t h) {
                              c->n_objects);
     uint32_t tx = (w + 15u) / 16u, ty = (h + 15u) / 16u;
     uint64_t nt = (uint64_t)tx * ty;
+    c->ref_order_n = 0;
     int rc = c->tile_idx.reserve(nt * RKF_TILE_MAX_OBJECTS * 4, false);
     if (!rc) rc = c->tile_counts.reserve(nt * 4, false);
     if (rc) return rc;
@@ -285,7 +304,7 @@ int rkfm_build_tiles(rkfm_ctx *c, uint32_t w, uint32_t h) {
     if (rc) return rc;
     c->tiles_x = tx;
     c->tiles_y = ty;
-    return RKFM_OK;
+    return ref_order(c, nt);
 }
 
 int rkfm_build_coarse_field(rkfm_ctx *c, const float origin[3], const uint32_t dims[3], float vs) {
@@ -398,6 +417,7 @@ int rkfm_dispatch(rkfm_ctx *c, const rkfm_gbuffer *out, void *stream) {
     P.tile_idx = (const uint32_t *)c->tile_idx.ptr;
     P.tile_counts = (const uint32_t *)c->tile_counts.ptr;
     P.n_tiles = c->tiles_x * c->tiles_y;
+    P.tile_order = mode == 0 && c->ref_order_n == P.n_tiles ? (const uint32_t *)c->ref_order.ptr : nullptr;
     P.coarse = c->coarse_set ? (const uint16_t *)c->coarse.ptr : nullptr;
     P.cu = c->coarse_u;
     P.cam = c->cam;

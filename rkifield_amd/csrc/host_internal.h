@@ -40,6 +40,8 @@ struct rkfm_ctx {
     uint64_t memory_budget = 0;
     // GpuScene (reference layout)
     DevBuf pool, maps, objects, bvh, tile_idx, tile_counts, coarse;
+    DevBuf ref_order;          // the tiled march's launch order (ref_order_n tiles)
+    uint64_t ref_order_n = 0;
     uint64_t n_pool_voxels = 0, n_maps = 0;
     uint32_t n_objects = 0, n_bvh = 0, tiles_x = 0, tiles_y = 0;
     rkf_camera_uniforms cam{};

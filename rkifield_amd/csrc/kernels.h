@@ -17,6 +17,7 @@ struct RefParams {
     const uint32_t *tile_idx;
     const uint32_t *tile_counts;
     uint32_t n_tiles;
+    const uint32_t *tile_order;  // tiled mode: tiles by descending object count (nullptr: row-major launch)
     const uint16_t *coarse;  // nullptr: no coarse field
     rkf_coarse_field_uniforms cu;
     rkf_camera_uniforms cam;
@@ -111,6 +112,7 @@ hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t stream);
 hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream);
 hipError_t rkf_launch_gather16(void *dst, const void *src, uint64_t n_lines, uint64_t n, hipStream_t stream);
 hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream);
+hipError_t rkf_launch_ref_tile_order(const uint32_t *counts, uint32_t n_tiles, uint32_t *order, hipStream_t stream);
 hipError_t rkf_launch_coarse(const rkf::CoarseParams *P, hipStream_t stream);
 hipError_t rkf_launch_sparse_march(const rkf::SparseMarchParams *P, uint32_t variant, uint32_t waves,
                                    hipStream_t stream);

@@ -164,7 +164,9 @@ constexpr uint32_t kNoObj = 0xFFFFFFFFu;
 // lanes 0-3 and 4-7 each hold one value and the half-row mirror serves as
 // the xor-4 level: for 8 lanes the exchanges are DPP moves, not LDS round
 // trips.
-template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j) {
+// G > 1: groups of G lanes already hold one value each (levels below G are
+// skipped).
+template <uint32_t L, uint32_t G = 1> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j) {
     if (L == 8u) {
         auto level = [&](auto xch) {
             const float pd = __uint_as_float(xch(__float_as_uint(d)));
@@ -175,9 +177,9 @@ template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j)
                 j = pj;
             }
         };
-        level([](uint32_t v) { return dpp_u32<kDppXor1>(v); });
-        level([](uint32_t v) { return dpp_u32<kDppXor2>(v); });
-        level([](uint32_t v) { return dpp_u32<kDppHalfMirror>(v); });
+        if (G <= 1u) level([](uint32_t v) { return dpp_u32<kDppXor1>(v); });
+        if (G <= 2u) level([](uint32_t v) { return dpp_u32<kDppXor2>(v); });
+        if (G <= 4u) level([](uint32_t v) { return dpp_u32<kDppHalfMirror>(v); });
         return;
     }
 #pragma unroll
@@ -199,7 +201,7 @@ template <uint32_t L> RKF_DEV void split_min(float &d, uint32_t &m, uint32_t &j)
 // tiled and brute object loops are shared by the ray's L lanes (lane `share`
 // takes every L-th object). bis: evaluations of the bisection (made by every
 // lane of the ray).
-template <uint32_t MODE, uint32_t L>
+template <uint32_t MODE, uint32_t L, uint32_t G>
 RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
                         uint32_t share, MarchRes &r, uint32_t &bis) {
     const RefParams &P = *c.P;
@@ -231,18 +233,21 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
         float min_dist = kMaxFloat;
         uint32_t best_mat = 0, best_id = 0, best_idx = 0, best_j = 0;
         if (MODE == 0) {
+            // groups of G lanes, each sampling one object per pass (group q:
+            // objects q, q + L / G, ...)
+            const uint32_t q = share / G, sg = share % G;
             float bd = kMaxFloat;
             uint32_t bm = 0, bj = kNoObj;
-            for (uint32_t i = share; i < count; i += L) {
+            for (uint32_t i = q; i < count; i += L / G) {
                 uint32_t m;
-                const float d = eval_obj(c, pos, tobj[i], toi[i], m);
+                const float d = eval_obj<G>(c, pos, tobj[i], toi[i], m, sg, threadIdx.x - sg);
                 if (d < bd) {
                     bd = d;
                     bm = m;
                     bj = i;
                 }
             }
-            split_min<L>(bd, bm, bj);
+            split_min<L, G>(bd, bm, bj);
             if (bj != kNoObj) {
                 const uint32_t oi = toi[bj];
                 min_dist = bd;
@@ -326,11 +331,11 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
 
 // march_core for one ray; c.evals then counts the ray's evaluations once
 // (the L lanes' shares of the object loops plus one bisection).
-template <uint32_t MODE, uint32_t L>
+template <uint32_t MODE, uint32_t L, uint32_t G = 1>
 RKF_DEV void march(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, const uint32_t *toi, uint32_t count,
                    uint32_t share, MarchRes &r) {
     uint32_t bis = 0;
-    march_core<MODE, L>(c, o, dir, tobj, toi, count, share, r, bis);
+    march_core<MODE, L, G>(c, o, dir, tobj, toi, count, share, r, bis);
     if (L > 1) {
         uint32_t e = c.evals - bis;
 #pragma unroll
@@ -389,6 +394,9 @@ template <uint32_t L> RKF_DEV f3 normal_for_object(RefCtx &c, f3 pos, uint32_t o
 template <uint32_t L> struct RefBlock {
     static constexpr uint32_t w = L <= 2u ? 8u : L <= 8u ? 4u : 2u, h = (64u / L) / w;
 };
+// Workgroups per 16x16 object tile in the ordered tiled launch (8 lanes per
+// ray: 4x2-pixel workgroups).
+constexpr uint32_t kRefTileBlocks = 32u;
 template <uint32_t MODE, uint32_t L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
     static_assert(MODE != 1 || L == 1, "the BVH walk prunes by the running minimum: one lane per ray");
@@ -396,15 +404,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     __shared__ rkf_gpu_object s_obj[RKF_TILE_MAX_OBJECTS];
     __shared__ uint32_t s_oi[RKF_TILE_MAX_OBJECTS];
     const uint32_t lane = threadIdx.x, ray = lane / L, share = lane % L;
-    uint32_t px = blockIdx.x * BW + ray % BW;
-    uint32_t py = blockIdx.y * BH + ray / BW;
     const uint32_t W = P.g.width, H = P.g.height;
+    uint32_t bx = blockIdx.x, by = blockIdx.y;
+    if (MODE == 0 && P.tile_order) {
+        // 1-D launch, (16 / BW) x (16 / BH) workgroups per 16x16 tile, tiles
+        // in the given order; workgroups of edge tiles past the image leave.
+        static_assert(MODE != 0 || (16u % BW == 0u && 16u % BH == 0u && (16u / BW) * (16u / BH) == kRefTileBlocks),
+                      "tile blocks");
+        const uint32_t ntx = (W + 15u) / 16u;
+        const uint32_t t = P.tile_order[blockIdx.x / kRefTileBlocks], sub = blockIdx.x % kRefTileBlocks;
+        bx = (t % ntx) * (16u / BW) + sub % (16u / BW);
+        by = (t / ntx) * (16u / BH) + sub / (16u / BW);
+        if (bx * BW >= W || by * BH >= H) return;
+    }
+    uint32_t px = bx * BW + ray % BW;
+    uint32_t py = by * BH + ray / BW;
     uint32_t count = 0;
     if (MODE == 0) {
         // Stage the tile's object list and records (64 lanes, one dword of a
         // 256-byte record each; records past num_objects are never read).
         const uint32_t ntx = (W + 15u) / 16u;
-        const uint32_t tile = ((blockIdx.y * BH) / 16u) * ntx + (blockIdx.x * BW) / 16u;
+        const uint32_t tile = ((by * BH) / 16u) * ntx + (bx * BW) / 16u;
         count = tile < P.n_tiles ? P.tile_counts[tile] : 0u;
         if (count > RKF_TILE_MAX_OBJECTS) count = RKF_TILE_MAX_OBJECTS;
         if (lane < count) s_oi[lane] = P.tile_idx[(uint64_t)tile * RKF_TILE_MAX_OBJECTS + lane];
@@ -431,7 +451,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     f3 o, dir;
     camera_ray(P.cam, W, H, px, py, o, dir);
     MarchRes r;
-    march<MODE, L>(c, o, dir, s_obj, s_oi, count, share, r);
+    if (MODE == 0 && L == 8u) {
+        // Lanes per object sample by the tile's object count (uniform in the
+        // workgroup): all 8 on one object, 4 on two, 2 on up to four, else 1.
+        if (count <= 1u) march<MODE, L, 8>(c, o, dir, s_obj, s_oi, count, share, r);
+        else if (count <= 2u) march<MODE, L, 4>(c, o, dir, s_obj, s_oi, count, share, r);
+        else if (count <= 4u) march<MODE, L, 2>(c, o, dir, s_obj, s_oi, count, share, r);
+        else march<MODE, L, 1>(c, o, dir, s_obj, s_oi, count, share, r);
+    } else {
+        march<MODE, L>(c, o, dir, s_obj, s_oi, count, share, r);
+    }
     // the ray's L lanes hold the same results from here on; the first stores
     if (!r.hit) {
         if (share == 0) {
@@ -473,6 +502,8 @@ extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t 
     if (P->mode == 0) {
         using B = rkf::RefBlock<kTiledLanes>;
         dim3 tg((P->g.width + B::w - 1u) / B::w, (P->g.height + B::h - 1u) / B::h, 1);
+        if (P->tile_order) tg = dim3(P->n_tiles * rkf::kRefTileBlocks, 1, 1);
+        if (tg.x == 0u) return hipSuccess;
         hipLaunchKernelGGL((rkf::ray_march_ref_kernel<0, kTiledLanes>), tg, dim3(64, 1, 1), 0, stream, *P);
     } else if (P->mode == 1) {
         hipLaunchKernelGGL((rkf::ray_march_ref_kernel<1, 1>), grid, dim3(64, 1, 1), 0, stream, *P);

@@ -84,6 +84,32 @@ __global__ __launch_bounds__(256) void tile_cull_kernel(TileCullParams P) {
     if (tid == 0) P.counts[t] = cnt;
 }
 
+// Launch order of the drop-in's tiled march: the tiles by descending object
+// count (counting sort, one workgroup; counts above the list capacity sort
+// with it), so the workgroups of the costliest tiles start first. Rays are
+// independent: the order changes no result.
+constexpr uint32_t kOrderThreads = 1024u;
+__global__ __launch_bounds__(kOrderThreads) void ref_tile_order_kernel(const uint32_t *counts, uint32_t n,
+                                                                       uint32_t *order) {
+    constexpr uint32_t kBins = RKF_TILE_MAX_OBJECTS + 1u;
+    __shared__ uint32_t next[kBins];
+    const uint32_t tid = threadIdx.x;
+    if (tid < kBins) next[tid] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kOrderThreads) atomicAdd(&next[min(counts[i], kBins - 1u)], 1u);
+    __syncthreads();
+    if (tid == 0) {  // first position of each count, highest count first
+        uint32_t at = 0;
+        for (uint32_t b = kBins; b-- > 0u;) {
+            const uint32_t h = next[b];
+            next[b] = at;
+            at += h;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kOrderThreads) order[atomicAdd(&next[min(counts[i], kBins - 1u)], 1u)] = i;
+}
+
 // coarse_field.rs:76-79 (aabb_unsigned_distance) and :240-266.
 __global__ __launch_bounds__(256) void coarse_kernel(CoarseParams P) {
     uint64_t n = (uint64_t)P.dims[0] * P.dims[1] * P.dims[2];
@@ -153,6 +179,13 @@ extern "C" hipError_t rkf_launch_copy(void *dst, const void *src, uint64_t bytes
 
 extern "C" hipError_t rkf_launch_tile_cull(const rkf::TileCullParams *P, hipStream_t stream) {
     hipLaunchKernelGGL(rkf::tile_cull_kernel, dim3(P->tiles_x, P->tiles_y, 1), dim3(256, 1, 1), 0, stream, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rkf_launch_ref_tile_order(const uint32_t *counts, uint32_t n_tiles, uint32_t *order,
+                                                hipStream_t stream) {
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(rkf::ref_tile_order_kernel, dim3(1), dim3(rkf::kOrderThreads), 0, stream, counts, n_tiles, order);
     return hipGetLastError();
 }
 
