@@ -395,8 +395,8 @@ template <uint32_t L> struct RefBlock {
     static constexpr uint32_t w = L <= 2u ? 8u : L <= 8u ? 4u : 2u, h = (64u / L) / w;
 };
 // Workgroups per 16x16 object tile in the ordered tiled launch (8 lanes per
-// ray: 4x2-pixel workgroups).
-constexpr uint32_t kRefTileBlocks = 32u;
+// ray: 4x2-pixel workgroups), and the XCDs the dispatcher deals them to.
+constexpr uint32_t kRefTileBlocks = 32u, kXcds = 8u;
 template <uint32_t MODE, uint32_t L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void ray_march_ref_kernel(RefParams P) {
     static_assert(MODE != 1 || L == 1, "the BVH walk prunes by the running minimum: one lane per ray");
@@ -407,12 +407,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     const uint32_t W = P.g.width, H = P.g.height;
     uint32_t bx = blockIdx.x, by = blockIdx.y;
     if (MODE == 0 && P.tile_order) {
-        // 1-D launch, (16 / BW) x (16 / BH) workgroups per 16x16 tile, tiles
-        // in the given order; workgroups of edge tiles past the image leave.
+        // 1-D launch in the given tile order, (16 / BW) x (16 / BH) = 32
+        // workgroups per 16x16 tile. Workgroups go to the XCDs round-robin
+        // (workgroup b to XCD b % 8), so each run of 256 workgroups takes 8
+        // tiles and gives XCD x all 32 workgroups of the run's tile x: a
+        // tile's rays sample its objects' bricks through one L2. Slots past
+        // the last tile, and workgroups of edge tiles past the image, leave.
         static_assert(MODE != 0 || (16u % BW == 0u && 16u % BH == 0u && (16u / BW) * (16u / BH) == kRefTileBlocks),
                       "tile blocks");
         const uint32_t ntx = (W + 15u) / 16u;
-        const uint32_t t = P.tile_order[blockIdx.x / kRefTileBlocks], sub = blockIdx.x % kRefTileBlocks;
+        const uint32_t run = blockIdx.x / (kRefTileBlocks * kXcds), in_run = blockIdx.x % (kRefTileBlocks * kXcds);
+        const uint32_t slot = run * kXcds + in_run % kXcds, sub = in_run / kXcds;
+        if (slot >= P.n_tiles) return;
+        const uint32_t t = P.tile_order[slot];
         bx = (t % ntx) * (16u / BW) + sub % (16u / BW);
         by = (t / ntx) * (16u / BH) + sub / (16u / BW);
         if (bx * BW >= W || by * BH >= H) return;
@@ -502,7 +509,7 @@ extern "C" hipError_t rkf_launch_ref_march(const rkf::RefParams *P, hipStream_t 
     if (P->mode == 0) {
         using B = rkf::RefBlock<kTiledLanes>;
         dim3 tg((P->g.width + B::w - 1u) / B::w, (P->g.height + B::h - 1u) / B::h, 1);
-        if (P->tile_order) tg = dim3(P->n_tiles * rkf::kRefTileBlocks, 1, 1);
+        if (P->tile_order) tg = dim3((P->n_tiles + rkf::kXcds - 1u) / rkf::kXcds * rkf::kXcds * rkf::kRefTileBlocks, 1, 1);
         if (tg.x == 0u) return hipSuccess;
         hipLaunchKernelGGL((rkf::ray_march_ref_kernel<0, kTiledLanes>), tg, dim3(64, 1, 1), 0, stream, *P);
     } else if (P->mode == 1) {

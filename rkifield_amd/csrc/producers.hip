@@ -85,29 +85,47 @@ __global__ __launch_bounds__(256) void tile_cull_kernel(TileCullParams P) {
 }
 
 // Launch order of the drop-in's tiled march: the tiles by descending object
-// count (counting sort, one workgroup; counts above the list capacity sort
-// with it), so the workgroups of the costliest tiles start first. Rays are
-// independent: the order changes no result.
-constexpr uint32_t kOrderThreads = 1024u;
+// count, tiles of one count in row-major order (a stable counting sort in
+// one workgroup; counts above the list capacity sort with it), so the
+// workgroups of the costliest tiles start first and neighbouring tiles of
+// one count, which share their objects' bricks, run side by side. Rays are
+// independent: the order changes no result. Thread t sorts the contiguous
+// run of tiles [t m, t m + m): pass 1 counts the run per count into column t
+// of a (count, thread) table, one exclusive scan over the table (highest
+// count first, threads in order within a count) gives each (count, thread)
+// its first position, pass 2 writes the run's tiles in order.
+constexpr uint32_t kOrderThreads = 256u;
 __global__ __launch_bounds__(kOrderThreads) void ref_tile_order_kernel(const uint32_t *counts, uint32_t n,
                                                                        uint32_t *order) {
     constexpr uint32_t kBins = RKF_TILE_MAX_OBJECTS + 1u;
-    __shared__ uint32_t next[kBins];
-    const uint32_t tid = threadIdx.x;
-    if (tid < kBins) next[tid] = 0u;
+    __shared__ uint32_t tab[kBins * kOrderThreads];  // [(kBins - 1 - count) * threads + t]
+    __shared__ uint32_t part[kOrderThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t m = (n + kOrderThreads - 1u) / kOrderThreads;
+    const uint32_t j0 = min(t * m, n), j1 = min(j0 + m, n);
+    for (uint32_t b = 0; b < kBins; b++) tab[b * kOrderThreads + t] = 0u;
+    for (uint32_t j = j0; j < j1; j++) tab[(kBins - 1u - min(counts[j], kBins - 1u)) * kOrderThreads + t]++;
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += kOrderThreads) atomicAdd(&next[min(counts[i], kBins - 1u)], 1u);
+    // exclusive scan of the table in index order: thread t owns entries [t kBins, t kBins + kBins)
+    uint32_t sum = 0;
+    for (uint32_t e = 0; e < kBins; e++) sum += tab[t * kBins + e];
+    part[t] = sum;
     __syncthreads();
-    if (tid == 0) {  // first position of each count, highest count first
-        uint32_t at = 0;
-        for (uint32_t b = kBins; b-- > 0u;) {
-            const uint32_t h = next[b];
-            next[b] = at;
-            at += h;
-        }
+    for (uint32_t d = 1; d < kOrderThreads; d <<= 1) {  // inclusive scan of the 256 run sums
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t at = part[t] - sum;
+    for (uint32_t e = 0; e < kBins; e++) {
+        const uint32_t h = tab[t * kBins + e];
+        tab[t * kBins + e] = at;
+        at += h;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += kOrderThreads) order[atomicAdd(&next[min(counts[i], kBins - 1u)], 1u)] = i;
+    for (uint32_t j = j0; j < j1; j++)
+        order[tab[(kBins - 1u - min(counts[j], kBins - 1u)) * kOrderThreads + t]++] = j;
 }
 
 // coarse_field.rs:76-79 (aabb_unsigned_distance) and :240-266.
