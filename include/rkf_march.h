@@ -233,7 +233,9 @@ int rkfm_set_scene(rkfm_ctx *ctx, const rkf_scene_uniforms *scene,
                    const rkfm_scene_ext *ext /* NULL = reference */);
 /* Per-tile object lists as produced by tile_object_cull.wgsl:117-204:
  * idx holds tiles_x*tiles_y*32 u32, counts tiles_x*tiles_y u32. Passing
- * tiles_x == 0 clears the lists (strategy AUTO then falls back to BVH/brute). */
+ * tiles_x == 0 clears the lists (strategy AUTO then falls back to BVH/brute).
+ * Here and in rkfm_build_tiles the context also orders the tiles for the
+ * tiled march's launch (descending object count; no effect on results). */
 int rkfm_set_tiles(rkfm_ctx *ctx, const uint32_t *idx, const uint32_t *counts,
                    uint32_t tiles_x, uint32_t tiles_y);
 /* Coarse field (coarse_field.rs:240-307): dims.x*dims.y*dims.z binary16
