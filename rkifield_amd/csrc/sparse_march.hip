@@ -7,23 +7,20 @@ namespace rkf {
 // Plain variant: one lane per pixel, 8x8 pixels per wavefront, each ray
 // marched to completion, then its shadow ray. With a tile order (culled
 // views) the tiles with most occupied cells start first, so the long ones
-// do not start last. Workgroups go to the 8 XCDs round-robin (workgroup b to
-// XCD b % 8), so each run of 8 x 16 workgroups hands XCD x the 16
-// consecutive order slots 16 x .. 16 x + 15 of the run (tiles the sort wrote
-// side by side often lie on the same rows and share bricks in that XCD's
-// L2); a last partial run keeps the plain mapping.
+// do not start last; without one, tiles go in row-major order. Workgroups go
+// to the 8 XCDs round-robin (workgroup b to XCD b % 8), so each run of
+// 8 x 16 workgroups hands XCD x the 16 consecutive slots 16 x .. 16 x + 15
+// of the run: 16 neighbouring tiles of a row, or tiles the sort wrote side
+// by side (often on the same rows), which share bricks in that XCD's L2. A
+// last partial run keeps the plain mapping.
 constexpr uint32_t kXcds = 8u, kXcdRun = 16u;
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sparse_march_plain(SparseMarchParams P) {
-    uint32_t bx = blockIdx.x, by = blockIdx.y;
-    if (P.order) {
-        uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
-        const uint32_t run = b / (kXcds * kXcdRun), in_run = b % (kXcds * kXcdRun);
-        if ((run + 1u) * kXcds * kXcdRun <= gridDim.x * gridDim.y)
-            b = run * kXcds * kXcdRun + (in_run % kXcds) * kXcdRun + in_run / kXcds;
-        const uint32_t tile = P.order[b];
-        bx = tile % gridDim.x;
-        by = tile / gridDim.x;
-    }
+    uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t run = b / (kXcds * kXcdRun), in_run = b % (kXcds * kXcdRun);
+    if ((run + 1u) * kXcds * kXcdRun <= gridDim.x * gridDim.y)
+        b = run * kXcds * kXcdRun + (in_run % kXcds) * kXcdRun + in_run / kXcds;
+    const uint32_t tile = P.order ? P.order[b] : b;
+    const uint32_t bx = tile % gridDim.x, by = tile / gridDim.x;
     uint32_t px = bx * 8u + (threadIdx.x & 7u);
     uint32_t py = by * 8u + (threadIdx.x >> 3);
     view_pixel<true>(P, px, py, px < P.g.width && py < P.g.height);
