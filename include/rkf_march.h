@@ -245,7 +245,9 @@ int rkfm_set_coarse_field(rkfm_ctx *ctx, const uint16_t *f16_cells,
 
 /* Producers of the march inputs, on the GPU (tile_object_cull.wgsl:117-204 and
  * CoarseField::populate coarse_field.rs:240-266). Tile lists are ordered by
- * ascending object index, keeping the first 32. */
+ * ascending object index, keeping the first 32. Both launch asynchronously on
+ * the context's stream (rkfm_stream): an rkfm_dispatch on another stream must
+ * be ordered after them (rkfm_sync, or an event recorded on that stream). */
 int rkfm_build_tiles(rkfm_ctx *ctx, uint32_t width, uint32_t height);
 int rkfm_build_coarse_field(rkfm_ctx *ctx, const float origin[3],
                             const uint32_t dims[3], float voxel_size);

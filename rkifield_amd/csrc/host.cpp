@@ -255,7 +255,10 @@ int rkfm_set_tiles(rkfm_ctx *c, const uint32_t *idx, const uint32_t *counts, uin
     if (rc) return rc;
     c->tiles_x = tx;
     c->tiles_y = ty;
-    return ref_order(c, nt);
+    rc = ref_order(c, nt);
+    if (rc) return rc;
+    // a host upload: complete on return, like the lists themselves
+    return rkf_hip_check(hipStreamSynchronize(c->stream), "set_tiles sync");
 }
 
 int rkfm_set_coarse_field(rkfm_ctx *c, const uint16_t *cells, const rkf_coarse_field_uniforms *u) {
