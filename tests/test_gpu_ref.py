@@ -194,3 +194,37 @@ def test_upload_index_limits(ctx):
         assert e.value.code == M.ERR["OUT_OF_RANGE"]
     gs.upload_brick_pool(brick, first_slot=0)
     gs.upload_brick_maps(np.zeros(4, np.uint32), offset=0)
+
+
+def test_host_tile_lists(ctx):
+    """Tile lists made on the host and handed over with rkfm_set_tiles (which
+    also makes the launch order from them): the oracle's cull, then lists with
+    each tile's objects reversed, about a third of the tiles emptied and one
+    tile naming 40 objects (past the 32-entry capacity: the first 32 are
+    used) -- each bit-exact against the FP32 twin given the same lists."""
+    sc = scenes.spheres_scene(seed=2, n=40, width=96, height=64, half_cube=4.0, cam_dist=12.0)
+    _say("case: host tile lists 96x64, 40 spheres")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    idx, counts = O.tile_cull(sc.cam, sc.su, sc.objects, sc.width, sc.height)
+    assert counts.max() > 8 and (counts == 1).any() and (counts == 0).any()
+    lists = idx.reshape(-1, 32).copy()
+    c2 = counts.copy()
+    for t in range(c2.size):
+        lists[t, :c2[t]] = lists[t, :c2[t]][::-1].copy()
+    rng = np.random.default_rng(7)
+    c2[rng.random(c2.size) < 0.3] = 0
+    full = int(np.argmax(counts))
+    lists[full] = np.arange(32, dtype=np.uint32)
+    c2[full] = 40
+    cells, u = cf.read()
+    for name, ti, tcn in (("oracle cull", idx, counts), ("reversed, emptied, over capacity", lists.reshape(-1), c2)):
+        _say(f"case: host tile lists: {name}")
+        tc.set_lists(ti, tcn)
+        gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+        M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+        inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                          tile_idx=np.ascontiguousarray(ti, np.uint32), tile_counts=np.ascontiguousarray(tcn, np.uint32),
+                          coarse=cells, coarse_u=u)
+        res = O.ref_trace(inp, prec=32)
+        assert res["hit"].sum() > 100
+        _compare(gb.read(), res, sc.width, sc.height)
