@@ -16,6 +16,7 @@ Errors: every C call returns a status; a non-zero status raises
 cannot fail from the caller's view (wgpu validation errors are logged);
 here invalid inputs are reported instead of rendered.
 """
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -27,6 +28,25 @@ from . import layouts as L
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librkf_march.so")
 _LIB = None
+
+# Servers and contexts still open when the interpreter exits: Python does
+# not promise to run __del__ for them, so they are closed here, servers
+# (whose worker thread uses the context) before contexts (whose graphs are
+# destroyed first), while the HIP runtime is still up.
+_OPEN_SERVERS = weakref.WeakSet()
+_OPEN_CONTEXTS = weakref.WeakSet()
+
+
+def _close_open():
+    for group in (_OPEN_SERVERS, _OPEN_CONTEXTS):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(_close_open)
 
 
 class RkfMarchError(RuntimeError):
@@ -160,6 +180,7 @@ class Context:
         _check(lib().rkfm_create(_p(cfg), C.byref(h)))
         self.h = h.value
         self.graphs = weakref.WeakSet()  # batch graphs, destroyed before the context
+        _OPEN_CONTEXTS.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -772,6 +793,7 @@ class Server:
         _check(lib().rkfm_server_create(ctx.h, handles.ctypes.data, handles.size, cfg.ctypes.data, C.byref(h)))
         self.h = h.value
         self._keep = {}
+        _OPEN_SERVERS.add(self)
 
     # request builders -------------------------------------------------------
     @staticmethod
