@@ -234,11 +234,17 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
         uint32_t best_mat = 0, best_id = 0, best_idx = 0, best_j = 0;
         if (MODE == 0) {
             // groups of G lanes, each sampling one object per pass (group q:
-            // objects q, q + L / G, ...)
-            const uint32_t q = share / G, sg = share % G;
+            // objects q, q + L / G, ...). With one lane per object, a last
+            // pass that would hold a single object (9, 17 or 25 objects) is
+            // sampled by all the ray's lanes instead, after the minimum over
+            // the full passes: that object has the highest index, so it
+            // replaces the minimum only when strictly smaller, as in the
+            // one-lane loop.
+            const bool lone_last = G == 1u && L == 8u && count > L && count % L == 1u;
+            const uint32_t q = share / G, sg = share % G, n_split = lone_last ? count - 1u : count;
             float bd = kMaxFloat;
             uint32_t bm = 0, bj = kNoObj;
-            for (uint32_t i = q; i < count; i += L / G) {
+            for (uint32_t i = q; i < n_split; i += L / G) {
                 uint32_t m;
                 const float d = eval_obj<G>(c, pos, tobj[i], toi[i], m, sg, threadIdx.x - sg);
                 if (d < bd) {
@@ -248,6 +254,16 @@ RKF_DEV void march_core(RefCtx &c, f3 o, f3 dir, const rkf_gpu_object *tobj, con
                 }
             }
             split_min<L, G>(bd, bm, bj);
+            if (G == 1u && L == 8u && lone_last) {
+                const uint32_t i = count - 1u;
+                uint32_t m;
+                const float d = eval_obj<8u>(c, pos, tobj[i], toi[i], m, share, threadIdx.x - share);
+                if (d < bd) {
+                    bd = d;
+                    bm = m;
+                    bj = i;
+                }
+            }
             if (bj != kNoObj) {
                 const uint32_t oi = toi[bj];
                 min_dist = bd;
