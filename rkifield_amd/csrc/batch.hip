@@ -309,7 +309,7 @@ int cull_enqueue(const rkfm_batch_item *items, const std::vector<uint32_t> &view
         const rkf::SparseDev D = it.scene->dev();
         hipError_t e = rkf_launch_sparse_tiles(&D, &it.camera, it.su.max_distance, it.out.width, it.out.height,
                                                (uint2 *)(base + L.off[k]), nullptr, (rkf::TileBig *)(base + L.big_off),
-                                               (uint32_t *)(base + L.count_off), st);
+                                               (uint32_t *)(base + L.count_off), true, st);
         if (e != hipSuccess) return rkf_hip_check(e, "batch tile intervals");
     }
     return RKFM_OK;

@@ -58,6 +58,12 @@ struct rkfm_ctx {
     // Tile intervals of the last culled march: per-tile pairs, the large
     // rectangle records and their counter.
     DevBuf tile_iv, tile_big, tile_big_count, tile_cnt, tile_order;
+    // Leading tile_iv entries (with tile_big_count) and tile_cnt entries (with
+    // the 512 sort words before them) that the last plain march put back to
+    // the cleared state, on stream clean_stream: the next culled view on that
+    // stream bins into them without a clearing launch.
+    uint64_t iv_clean = 0, cnt_clean = 0;
+    hipStream_t clean_stream = nullptr;
     // Edits: op records (and their pinned staging, reusable once edit_ev
     // has fired), the listed bricks, their modified flags, the list count.
     DevBuf edit_ops, edit_list, edit_mod, edit_count;

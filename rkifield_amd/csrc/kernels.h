@@ -83,6 +83,12 @@ struct SparseMarchParams {
     uint32_t tiles_x;          // 8x8 tiles per row of `tiles`
     const uint2 *tiles;        // per tile (t_lo, t_hi) float bits, or null
     const uint32_t *order;     // plain kernel: tiles in launch order, or null (row-major)
+    // Plain kernel, optional: after marching its tile, each workgroup puts
+    // the tile's interval (and count) back to the cleared state, and the
+    // first workgroup clears the sort bins and the large-rectangle count, so
+    // the next culled view's binning needs no clearing launch. Null: left.
+    uint2 *reset_iv;
+    uint32_t *reset_cnt, *reset_bins, *reset_big;
 };
 
 // A cell whose tile rectangle is too large for one lane (tile binning).
@@ -133,7 +139,7 @@ hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm_sphere_cas
 // (rkf_launch_sparse_tile_order's bins, zeroed here).
 hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam, float max_distance,
                                    uint32_t width, uint32_t height, uint2 *tiles, uint32_t *cnt, rkf::TileBig *big,
-                                   uint32_t *big_count, hipStream_t stream);
+                                   uint32_t *big_count, bool clear, hipStream_t stream);
 // Tiles by descending cell count (order holds n_tiles entries; cnt as filled
 // by rkf_launch_sparse_tiles, with its bins before it).
 hipError_t rkf_launch_sparse_tile_order(uint32_t *cnt, uint32_t n_tiles, uint32_t *order, hipStream_t stream);

@@ -89,7 +89,7 @@ extern "C" int rkfm_sparse_march_noalloc(rkfm_ctx *c, const rkfm_sparse_scene *s
         // tiles_clear_kernel zeroes the counter and every interval: nothing
         // of the previous call's contents is read
         rc = rkf_hip_check(rkf_launch_sparse_tiles(&D, cam, su->max_distance, out->width, out->height, tiles, nullptr, big,
-                                                   big_count, st),
+                                                   big_count, true, st),
                            "march_noalloc tile intervals");
         if (rc) return rc;
         P.tiles = tiles;

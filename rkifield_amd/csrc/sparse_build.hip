@@ -1336,17 +1336,21 @@ int rkf_tile_camera_check(const rkf_camera_uniforms *cam) {
 
 constexpr uint64_t kOrderMaxTiles = 65536u;
 
+// clear = false: tiles, cnt (with its sort words) and the big-rectangle
+// count are known to be cleared already.
 static int tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
                           const rkf_scene_uniforms *su, uint32_t w, uint32_t h, uint2 *tiles, uint32_t *cnt,
-                          hipStream_t st) {
+                          bool clear, hipStream_t st) {
     int rc = rkf_tile_camera_check(cam);
+    const void *count0 = c->tile_big_count.ptr;
     if (!rc) rc = c->tile_big.reserve((uint64_t)s->info.n_blocks * sizeof(rkf::TileBig) + 64, false);
     if (!rc) rc = c->tile_big_count.reserve(256, false);
     if (rc) return rc;
+    if (c->tile_big_count.ptr != count0) clear = true;  // a fresh allocation holds no cleared count
     rkf::SparseDev D = s->dev();
     return rkf_hip_check(rkf_launch_sparse_tiles(&D, cam, su->max_distance, w, h, tiles, cnt,
                                                  (rkf::TileBig *)c->tile_big.ptr,
-                                                 (uint32_t *)c->tile_big_count.ptr, st),
+                                                 (uint32_t *)c->tile_big_count.ptr, clear, st),
                          "tile intervals launch");
 }
 
@@ -1355,7 +1359,8 @@ int rkfm_sparse_tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rk
     if (!c || !s || !cam || !su || !tiles) return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: null argument");
     if (w == 0 || h == 0 || w > 32768u || h > 32768u)
         return rkf_set_error(RKFM_ERR_INVALID_ARG, "tile intervals: %ux%u", w, h);
-    return tile_intervals(c, s, cam, su, w, h, (uint2 *)tiles, nullptr, stream ? (hipStream_t)stream : c->stream);
+    c->iv_clean = 0;  // the big-rectangle count is shared with the context's views
+    return tile_intervals(c, s, cam, su, w, h, (uint2 *)tiles, nullptr, true, stream ? (hipStream_t)stream : c->stream);
 }
 
 // The tile counts and launch order of a culled view into the context's
@@ -1363,11 +1368,17 @@ int rkfm_sparse_tile_intervals(rkfm_ctx *c, const rkfm_sparse_scene *s, const rk
 static int tile_order(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_uniforms *cam,
                       const rkf_scene_uniforms *su, uint32_t w, uint32_t h, hipStream_t st) {
     const uint64_t n_tiles = (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u);
+    const void *iv0 = c->tile_iv.ptr, *cnt0 = c->tile_cnt.ptr;
     int rc = c->tile_iv.reserve(n_tiles * 8u, false);
     // 512 bin words of the order's counting sort, then one count per tile
     if (!rc) rc = c->tile_cnt.reserve((n_tiles + 512u) * 4u, false);
     if (!rc) rc = c->tile_order.reserve(n_tiles * 4u, false);
-    if (!rc) rc = tile_intervals(c, s, cam, su, w, h, (uint2 *)c->tile_iv.ptr, (uint32_t *)c->tile_cnt.ptr + 512u, st);
+    const bool clean = c->tile_iv.ptr == iv0 && c->tile_cnt.ptr == cnt0 && c->clean_stream == st &&
+                       c->iv_clean >= n_tiles && c->cnt_clean >= n_tiles;
+    c->iv_clean = c->cnt_clean = 0;
+    if (!rc)
+        rc = tile_intervals(c, s, cam, su, w, h, (uint2 *)c->tile_iv.ptr, (uint32_t *)c->tile_cnt.ptr + 512u, !clean,
+                            st);
     if (!rc)
         rc = rkf_hip_check(rkf_launch_sparse_tile_order((uint32_t *)c->tile_cnt.ptr + 512u, (uint32_t)n_tiles,
                                                         (uint32_t *)c->tile_order.ptr, st),
@@ -1429,13 +1440,26 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
         if (order) {
             rc = tile_order(c, s, cam, su, out->width, out->height, st);
         } else {
+            const void *iv0 = c->tile_iv.ptr;
             rc = c->tile_iv.reserve(n_tiles * 8u, false);
-            if (!rc) rc = tile_intervals(c, s, cam, su, out->width, out->height, (uint2 *)c->tile_iv.ptr, nullptr, st);
+            const bool clean = c->tile_iv.ptr == iv0 && c->clean_stream == st && c->iv_clean >= n_tiles;
+            c->iv_clean = 0;
+            if (!rc)
+                rc = tile_intervals(c, s, cam, su, out->width, out->height, (uint2 *)c->tile_iv.ptr, nullptr, !clean,
+                                    st);
         }
         if (rc) return rc;
         P.tiles = (const uint2 *)c->tile_iv.ptr;
         P.tiles_x = tx;
         if (order) P.order = (const uint32_t *)c->tile_order.ptr;
+        if (variant == 0) {  // the plain march puts what it read back to the cleared state
+            P.reset_iv = (uint2 *)c->tile_iv.ptr;
+            P.reset_big = (uint32_t *)c->tile_big_count.ptr;
+            if (order) {
+                P.reset_cnt = (uint32_t *)c->tile_cnt.ptr + 512u;
+                P.reset_bins = (uint32_t *)c->tile_cnt.ptr;
+            }
+        }
     }
     if (variant == 1 && !c->tuned_waves) {
         rc = rkf_hip_check(rkf_occupancy_waves(0, &c->tuned_waves), "occupancy");
@@ -1453,8 +1477,14 @@ int rkfm_sparse_march(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkf_camera_
                                                            shade_waves, st),
                              "sparse_march launch");
     }
-    return rkf_hip_check(rkf_launch_sparse_march(&P, variant, c->tuned_waves, st),
-                         "sparse_march launch");
+    rc = rkf_hip_check(rkf_launch_sparse_march(&P, variant, c->tuned_waves, st), "sparse_march launch");
+    if (!rc && P.reset_iv) {  // cleared again once this launch is done, on st
+        const uint64_t n_tiles = (uint64_t)P.tiles_x * ((out->height + 7u) / 8u);
+        c->iv_clean = n_tiles;
+        if (P.reset_cnt) c->cnt_clean = n_tiles;
+        c->clean_stream = st;
+    }
+    return rc;
 }
 
 int rkfm_sparse_contacts(rkfm_ctx *c, const rkfm_sparse_scene *s, const rkfm_body *bodies, uint32_t n,

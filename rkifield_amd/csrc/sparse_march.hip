@@ -14,6 +14,8 @@ namespace rkf {
 // by side (often on the same rows), which share bricks in that XCD's L2. A
 // last partial run keeps the plain mapping.
 constexpr uint32_t kXcds = 8u, kXcdRun = 16u;
+// bins of the tile order's counting sort (counts capped at 255)
+constexpr uint32_t kOrderBins = 256u;
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sparse_march_plain(SparseMarchParams P) {
     uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
     const uint32_t run = b / (kXcds * kXcdRun), in_run = b % (kXcds * kXcdRun);
@@ -24,6 +26,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sp
     uint32_t px = bx * 8u + (threadIdx.x & 7u);
     uint32_t py = by * 8u + (threadIdx.x >> 3);
     view_pixel<true>(P, px, py, px < P.g.width && py < P.g.height);
+    // the tile's interval was read (and used) above; nothing else reads it
+    if (P.reset_iv && threadIdx.x == 0u) {
+        P.reset_iv[tile] = make_uint2(0x7F800000u, 0u);
+        if (P.reset_cnt) P.reset_cnt[tile] = 0u;
+    }
+    if (P.reset_big && blockIdx.x == 0u && blockIdx.y == 0u) {
+        if (threadIdx.x == 0u) *P.reset_big = 0u;
+        if (P.reset_bins)
+            for (uint32_t i = threadIdx.x; i < 2u * kOrderBins; i += 64u) P.reset_bins[i] = 0u;
+    }
 }
 
 // Tuned variant: persistent wavefronts with per-lane ray refill and no
@@ -353,7 +365,6 @@ __global__ __launch_bounds__(256) void tiles_big_kernel(const TileBig *big, cons
 // `bins[0..255]`, then each workgroup ranks its tiles in LDS, reserves its
 // range in every count's slice with one atomic on `bins[256 + count]`, and
 // writes them (bins zeroed by tiles_clear_kernel).
-constexpr uint32_t kOrderBins = 256u;
 __global__ __launch_bounds__(256) void tiles_hist_kernel(const uint32_t *cnt, uint32_t n, uint32_t *bins) {
     __shared__ uint32_t h[kOrderBins];
     const uint32_t tid = threadIdx.x, i = blockIdx.x * 256u + tid;
@@ -530,12 +541,14 @@ extern "C" hipError_t rkf_launch_sparse_cast(const rkf::SparseDev *S, const rkfm
 
 extern "C" hipError_t rkf_launch_sparse_tiles(const rkf::SparseDev *S, const rkf_camera_uniforms *cam,
                                               float max_distance, uint32_t width, uint32_t height, uint2 *tiles,
-                                              uint32_t *cnt, rkf::TileBig *big, uint32_t *big_count,
+                                              uint32_t *cnt, rkf::TileBig *big, uint32_t *big_count, bool clear,
                                               hipStream_t stream) {
     const uint32_t n_tiles = ((width + 7u) / 8u) * ((height + 7u) / 8u);
-    // with counts, the 512 words before them are the order's bins
-    hipLaunchKernelGGL(rkf::tiles_clear_kernel, dim3((n_tiles + 255u) / 256u), dim3(256), 0, stream, tiles, cnt,
-                       cnt ? cnt - 2u * rkf::kOrderBins : nullptr, n_tiles, big_count);
+    // with counts, the 512 words before them are the order's bins; clear =
+    // false when the caller knows them cleared (the last march reset them)
+    if (clear)
+        hipLaunchKernelGGL(rkf::tiles_clear_kernel, dim3((n_tiles + 255u) / 256u), dim3(256), 0, stream, tiles, cnt,
+                           cnt ? cnt - 2u * rkf::kOrderBins : nullptr, n_tiles, big_count);
     if (S->n_blocks) {
         hipLaunchKernelGGL(rkf::tiles_bin_kernel, dim3((S->n_blocks + 255u) / 256u), dim3(256), 0, stream, *S, *cam,
                            max_distance, width, height, tiles, cnt, big, big_count);

@@ -355,3 +355,44 @@ def test_sphere_casts_match_oracle(ctx, small_scene):
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
     assert 500 < got["hit"].sum() < n - 500
     assert (got["steps"] > 5).sum() > 500
+
+
+def _check_view(gpu, res32):
+    pos, nrm, mat, mot = O.pack_gbuffer(res32)
+    for f in ("brick_slot", "voxel_index", "cell_index", "steps", "evals", "shadow_steps", "shadow_evals"):
+        assert np.array_equal(gpu["debug"][f], res32[f]), f
+    assert np.array_equal(gpu["position"].view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(gpu["normal"], nrm)
+    assert np.array_equal(gpu["material"], mat)
+    assert np.array_equal(gpu["motion"].view(np.uint32), mot.view(np.uint32))
+    assert np.array_equal(gpu["shadow"].view(np.uint32), res32["shadow"].astype(np.float32).view(np.uint32))
+
+
+def test_culled_views_back_to_back(ctx, small_scene):
+    """Culled views marched one after another bin into the tile buffers the
+    previous plain march put back to the cleared state, with no clearing
+    launch: repeated views, a smaller view after a larger one, a larger one
+    after a smaller one, and the tile-order and tile-interval entry points
+    (which leave those buffers in use) in between -- every view bit-exact
+    against the FP32 twin, and the tile counts right after a march equal to
+    the oracle's."""
+    prims, sc, lo = small_scene
+    sp, _ = _oracle_inputs(sc)
+    su = L.scene_uniforms(num_objects=1)
+    ld = np.array([0.5, 1.0, 0.3]) / np.linalg.norm([0.5, 1.0, 0.3])
+    ext = L.scene_ext(hit_eps_mode=L.EPS_RELATIVE, shadow_enable=1, light_dir=ld, march_flags=L.MARCH_TILE_CULL)
+    a, b, c = ((6.0, 3.0, 7.0), 96, 64), ((5.5, 2.5, -6.5), 61, 45), ((-6.0, 4.0, 5.0), 40, 24)
+    steps = [(a, None), (a, None), (b, None), (b, "order"), (b, None), (c, "intervals"), (c, None), (c, None), (a, None)]
+    for k, ((eye, w, h), between) in enumerate(steps):
+        cam = M.camera_look_at(eye, (0, 0, 0), 60.0, w, h)
+        if between == "order":
+            counts, _ = sc.tile_order(cam, su, w, h)  # right after a march of this view
+            assert np.array_equal(counts, O.sparse_tile_counts(sp, cam, su, w, h))
+        elif between == "intervals":
+            sc.tile_intervals(cam, su, w, h)
+        _say(f"case: back-to-back culled view {k}: {w}x{h} ({between or 'march only'} before)")
+        gb = M.GBuffer(ctx, w, h, debug=True, shadow=True)
+        sc.march(cam, su, ext, gb)
+        res32 = O.sparse_trace(sp, cam, su, ext, w, h, prec=32)
+        assert res32["hit"].sum() > 50
+        _check_view(gb.read(), res32)
