@@ -228,3 +228,22 @@ def test_host_tile_lists(ctx):
         res = O.ref_trace(inp, prec=32)
         assert res["hit"].sum() > 100
         _compare(gb.read(), res, sc.width, sc.height)
+
+
+def test_drop_in_1080p_parity(ctx):
+    """The tiled drop-in at 1920x1080 (8,160 tiles in their object-count
+    launch order, each tile's 32 workgroups on one XCD, a partial last tile
+    row): every pixel bit-exact against the FP32 twin."""
+    sc = scenes.spheres_scene(seed=0, width=1920, height=1080)
+    _say("case: drop-in 1920x1080, 64 spheres")
+    gs, tc, cf, ext = _upload(ctx, sc, L.STRATEGY_TILED)
+    gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+    M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    idx, counts = tc.read()
+    assert counts.size == 120 * 68 and counts.max() >= 7
+    cells, u = cf.read()
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=u)
+    res32, _ = O.ref_trace_parallel(inp)
+    assert res32["hit"].sum() > 100000
+    _compare(gb.read(), res32, sc.width, sc.height)
