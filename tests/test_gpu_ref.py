@@ -230,6 +230,55 @@ def test_host_tile_lists(ctx):
         _compare(gb.read(), res, sc.width, sc.height)
 
 
+def test_host_tile_lists_every_count(ctx):
+    """Every tile object count 0..33 (33 past the 32-entry capacity), so
+    every lanes-per-object class of the tiled march runs: whole-group
+    samples (1 object), 4/2-lane groups (2-4), one lane per object over 1-4
+    passes, and the one-object last pass of 9, 17 and 25 objects. The busy
+    counts go to the tiles the scene's objects cover, their culled objects
+    last in the list (so the last object is a real one) after objects that
+    miss the tile; some lists repeat their first object as their last (ties:
+    the earlier entry wins) or name an object index past the scene's
+    objects. Bit-exact against the FP32 twin given the same lists."""
+    sc = scenes.spheres_scene(seed=4, n=40, width=144, height=80, half_cube=4.0, cam_dist=12.0)
+    _say("case: host tile lists, every count 0..33, 144x80, 40 spheres")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    ntx, nty = 9, 5
+    idx, cc = O.tile_cull(sc.cam, sc.su, sc.objects, sc.width, sc.height)
+    cull = idx.reshape(-1, 32)
+    first = [9, 17, 25, 33, 32, 1, 2, 3, 4, 5, 8, 16, 24, 10, 18, 26]
+    by_count = first + [c for c in range(34) if c not in first] + list(range(34))
+    lists = np.zeros((ntx * nty, 32), np.uint32)
+    counts = np.zeros(ntx * nty, np.uint32)
+    for rank, t in enumerate(np.argsort(-cc.astype(np.int64), kind="stable")):
+        c = by_count[rank]
+        k = min(c, 32)
+        real = [int(x) for x in cull[t, :min(int(cc[t]), 32)]]
+        fill = [j for j in range(40) if j not in real]
+        objs = ((fill[:max(0, k - len(real))] + real)[-k:] if k else []) + fill + list(range(40))
+        objs = objs[:32]
+        if k > 1 and rank % 3 == 0:
+            objs[k - 1] = objs[0]
+        if k > 2 and rank % 5 == 0:
+            objs[k // 2] = 40 + int(t)
+        lists[t] = objs
+        counts[t] = c
+    assert sorted(set(counts.tolist())) == list(range(34))
+    tc.set_lists(lists.reshape(-1), counts)
+    gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+    M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+    cells, u = cf.read()
+    inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                      tile_idx=np.ascontiguousarray(lists.reshape(-1)), tile_counts=counts,
+                      coarse=cells, coarse_u=u)
+    res = O.ref_trace(inp, prec=32)
+    py, px = np.divmod(np.arange(sc.width * sc.height), sc.width)
+    hits = np.bincount((py // 16) * ntx + px // 16, weights=res["hit"].astype(np.float64), minlength=ntx * nty)
+    for c in (9, 17, 25, 32, 33):
+        assert hits[counts == c].sum() > 0, c
+    _compare(gb.read(), res, sc.width, sc.height)
+
+
 def test_drop_in_1080p_parity(ctx):
     """The tiled drop-in at 1920x1080 (8,160 tiles in their object-count
     launch order, each tile's 32 workgroups on one XCD, a partial last tile
