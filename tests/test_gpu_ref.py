@@ -279,6 +279,43 @@ def test_host_tile_lists_every_count(ctx):
     _compare(gb.read(), res, sc.width, sc.height)
 
 
+def test_host_coarse_fields(ctx):
+    """Coarse fields handed over with rkfm_set_coarse_field, including ones
+    that are not distance fields (any field gives the FP32 twin's result bit
+    for bit): the GPU-built field; that field with noise of up to +-0.6 and
+    spikes of 40 on random cells (different hits); the field scaled by 0.25;
+    a field with an infinite cell; and a negative inv_voxel_size in the
+    uniforms (every point reads as outside the field)."""
+    sc = scenes.spheres_scene(seed=6, n=24, width=160, height=96, half_cube=4.0, cam_dist=11.0)
+    _say("case: host coarse fields 160x96, 24 spheres")
+    gs, tc, cf, ext = _upload(ctx, sc)
+    idx, counts = tc.read()
+    base, u = cf.read()
+    rng = np.random.default_rng(11)
+    f = base.view(np.float16).astype(np.float32)
+    noisy = f + rng.uniform(-0.6, 0.6, f.size).astype(np.float32)
+    noisy[rng.random(f.size) < 0.01] = 40.0
+    inf = f.copy()
+    inf[f.size // 2] = np.inf
+    u_neg = u.copy()
+    u_neg["inv_voxel_size"] = -u["inv_voxel_size"]
+    cases = (("built", base, u), ("noise and spikes", np.abs(noisy).astype(np.float16).view(np.uint16), u),
+             ("scaled 0.25", (f * 0.25).astype(np.float16).view(np.uint16), u),
+             ("infinite cell", inf.astype(np.float16).view(np.uint16), u),
+             ("negative inv_voxel_size", base, u_neg))
+    for name, cells, uu in cases:
+        _say(f"case: host coarse field: {name}")
+        cells = np.ascontiguousarray(cells, np.uint16)
+        cf.upload_cells(cells, uu)
+        gb = M.GBuffer(ctx, sc.width, sc.height, debug=True)
+        M.RayMarchPass(ctx).dispatch(gs, gb, tc, cf)
+        inp = O.RefInputs(sc.pool, sc.maps, sc.objects, sc.cam, sc.su, ext, sc.width, sc.height,
+                          tile_idx=idx, tile_counts=counts, coarse=cells, coarse_u=uu)
+        res = O.ref_trace(inp, prec=32)
+        assert res["hit"].sum() > 100
+        _compare(gb.read(), res, sc.width, sc.height)
+
+
 def test_drop_in_1080p_parity(ctx):
     """The tiled drop-in at 1920x1080 (8,160 tiles in their object-count
     launch order, each tile's 32 workgroups on one XCD, a partial last tile
