@@ -10,20 +10,42 @@
  * twice from trace_impl.h: an FP64 oracle (depth / normal tolerances) and an
  * FP32 twin with the kernels' exact operation order (bit-exact integer fields).
  *
- * Pins (what fixes each function, checked in tests/test_oracle.py):
+ * Pins (what fixes each function; tests/test_oracle.py, tests/test_oracle_pins.py,
+ * tests/test_rkf_asset.py):
  *   f16 encode/decode          voxel.rs tests (1.5, -0.5, f16::MAX, +inf) and an
  *                              exhaustive comparison with IEEE binary16 (numpy)
  *   brick_index / flatten      brick.rs / brick_map.rs test vectors (0,7,56,448,511; 59)
  *   pool slot order            brick_pool.rs tests (0,1,2,3)
  *   sample_brick_trilinear     sampling.rs:88-292 test vectors
- *   voxelize_sdf               golden fixture scenes/test_cross.rkf, bit-exact
+ *   voxelize_sdf, .rkf reader  golden fixture scenes/test_cross.rkf (224 bricks), bit-exact
+ *                              through an LZ4 decoder independent of the product's;
+ *                              asset_file_tests.rs round trips
  *   sample_voxelized (global)  brute force: equals sample_brick_trilinear at points
  *                              more than one voxel from brick faces; constant field
- *   march                      closed form: voxelised sphere hit depth -> ray-sphere
- *                              intersection within O(vs); all-EMPTY map misses;
- *                              FP64 vs FP32 agreement
+ *   analytic primitives        each of the six read at points offset along its normal
+ *   transforms                 rotated, non-uniformly scaled sphere: hits on the
+ *                              ray-ellipsoid intersection, closed-form ellipsoid normal
+ *   march                      voxelised sphere depth -> ray-sphere intersection
+ *                              within O(vs); all-EMPTY map misses; FP64 vs FP32
+ *                              agreement; a planted mistake is caught
+ *   bisection                  planar crossing, t known to step * 2^-8
  *   normals                    closed form: sphere normal (p-c)/r within O(vs)
+ *   motion                     camera translated along its right axis: pixel shift
+ *   smin                       smin(a, a, k) = a - k/4; the |a - b| >= k branch
+ *   soft shadow                unoccluded ray 1 exactly; through a voxelised sphere 0;
+ *                              past a plane at known height: the min(k d / t) form
+ *   point query, sphere cast   ground plane: distance y, inside y < 0, gradient
+ *                              (0,1,0); exact contact distance and normal
+ *   DDA extents, tile intervals  brute force and soundness by dense sampling
+ *   edits, contacts            cpu_apply.rs / sdf_collision.rs tests, closed forms
  *   coarse field / tile cull   closed form AABB distances; projection of known boxes
+ *
+ * Parity unpinned (nothing but the restatement fixes them): the hardware
+ * trilinear coarse-field sampler (restated as a software sampler); glam's
+ * Mat4::inverse (supplied by the host) and Quat * Vec3 (scalar formula
+ * restated); naga's lowering of normalize, mix and unpack2x16float; the BVH
+ * topology of Bvh::build's SAH; lz4_flex's exact compressed bytes (any valid
+ * block decodes).
  */
 #ifndef RKF_ORACLE_H
 #define RKF_ORACLE_H
