@@ -468,8 +468,9 @@ def main():
                     help="config1/config4: march without per-tile ray intervals")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--served-requests", type=int, default=1000,
-                    help="config1: requests of the served-vs-plain replay reported beside the frame (0 = skip)")
+    ap.add_argument("--served-requests", type=int, default=4000,
+                    help="config1: requests of the served-vs-plain replay reported beside the frame (the config3 "
+                         "trace's 4000 by default; 0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also launch the plain copy kernel on this many MiB (profiler counter calibration)")
     ap.add_argument("--gather-mib", type=int, default=0,
